@@ -1,0 +1,292 @@
+#!/usr/bin/env python3
+"""Benchmark of the CT projector pair and the beam-hardening gradient on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+
+A step is one forward projection plus one adjoint (backprojection) of the 1024x1024
+fan-beam slice (360 views over 360 degrees, 1024 bins, source distance 4096 px), inputs
+resident in HBM.  The printed JSON line's `value` is voxel-ray updates per second of that
+pair in units of 1e9 (GUPS): 2 * nnz(A) * K / t.  Extra fields:
+  * bhc: polychromatic Poisson gradient evaluations per second (512x512 fan slice, 360
+    views x 729 bins, d = 2000, E = 20 spline bases; one evaluation = forward
+    projection, FP64 NLL + per-ray gradient weights, adjoint);
+  * batched: 64 ragged slices (mixed parallel/fan) in one launch per operator vs one
+    launch per slice, fwd+adj GUPS of both;
+  * roofline: the dominant kernel's algorithmic bytes (4 B per voxel-ray update) per
+    launch over its average launch time (HIP events on the launch stream), against
+    8 TB/s; traffic from the committed rocprofv3 counter summary when present;
+  * cpu_baseline: the reference's own pthreads cpuPrj (prj/cpuPrj.c, built into
+    oracle/_ref) on the same slice, timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FAN1024 = (1024, 1024, 360, 360, 1.0, 1.0, 4096.0)
+PAR256 = (256, 367, 180, 360, 1.0, 1.0, 0.0)
+BHC512 = (512, 729, 360, 360, 1.0, 1.0, 2000.0)
+PAR512 = (512, 729, 720, 1440, 1.0, 1.0, 0.0)
+WORKLOADS = {"fan1024": FAN1024, "par256": PAR256, "fan512": BHC512, "par512": PAR512}
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def ragged_geoms(seed=4, count=64):
+    """64 slices: n from {128..1024}, views U{90..720}, parallel or fan with equal odds,
+    bins = ceil(n*sqrt(2)) + 4; fan d = 4n over 360 degrees (prjFull = views), parallel
+    over 180 degrees (prjFull = 2 views)."""
+    import numpy as np
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(count):
+        n = int(rng.choice([128, 256, 384, 512, 768, 1024]))
+        na = int(rng.integers(90, 721))
+        fan = bool(rng.integers(0, 2))
+        W = int(np.ceil(n * np.sqrt(2))) + 4
+        out.append((n, W, na, na, 1.0, 1.0, 4.0 * n) if fan else (n, W, na, 2 * na, 1.0, 1.0, 0.0))
+    return out
+
+
+def timed(fn, reps, stream):
+    """Average ms per call of fn() with HIP events on `stream` (the launch stream)."""
+    import torch
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def traffic_from_profiles(kernel_tag):
+    """HBM bytes per launch from profiles/<tag>_pmc.json (FETCH_SIZE and WRITE_SIZE of
+    two rocprofv3 --pmc runs, KiB per dispatch, each divided by the counter's reading
+    per KiB of a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", f"{kernel_tag}_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        p = json.load(f)
+    try:
+        fetch = p["FETCH_SIZE_KiB"] / p["copy_fetch_factor"]
+        write = p["WRITE_SIZE_KiB"] / p["copy_write_factor"]
+        return (fetch + write) * 1024.0
+    except (KeyError, ZeroDivisionError):
+        return None
+
+
+def cpu_baseline(geo, img, sino, reps=3):
+    """Reference cpuPrj (64 pthreads, prj/cpuPrj.c:40) fwd + adj on the same slice."""
+    import numpy as np
+    import oracle
+    n, W, npj, pf, ds, er, d = geo
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    if oracle.ref_available():
+        kind = "reference"
+        run = lambda: (oracle.ref_run(n, W, npj, pf, ds, er, d, img, oracle.FWD_BIT),  # noqa: E731
+                       oracle.ref_run(n, W, npj, pf, ds, er, d, sino, oracle.BWD_BIT))
+        threads = 64
+    else:
+        kind = "port"
+        cf = oracle.conf(*geo)
+        run = lambda: (oracle.forward(cf, img), oracle.adjoint(cf, sino))  # noqa: E731
+        threads = min(8, cores or 1)
+    run()  # warm-up
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), kind, threads, cores
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="fan1024", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extras", action="store_true", help="skip the bhc and batched legs")
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (PMC scaling reference)")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(slices shard across GPUs with no exchange; see the design notes). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from imgrecsrc_amd.bhc import kappa_logspace, normalize_ie, poissLAlpha, poissLI, spline_knots
+    from imgrecsrc_amd.phantom import ellipse_phantom, gaussian, rasterise, ellipse_params
+    from imgrecsrc_amd.projector import Projector
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    dev = torch.device("cuda", 0)
+    stream = torch.cuda.current_stream()
+    geo = WORKLOADS[args.workload]
+    n, W, npj = geo[0], geo[1], geo[2]
+    log(f"workload {args.workload} {geo}")
+    P = Projector([geo])
+    img_np = ellipse_phantom(n, 2)
+    img = torch.from_numpy(img_np).to(dev).reshape(-1)
+    sino = torch.empty(P.sino_elems, dtype=torch.float32, device=dev)
+    back = torch.empty(P.img_elems, dtype=torch.float32, device=dev)
+    nnz = P.nnz()
+    log(f"nnz(A) = {nnz}")
+
+    if args.calib_copy:
+        a = torch.empty(64 << 20, dtype=torch.float32, device=dev)
+        b = torch.empty_like(a)
+        a.fill_(1.0)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("calibration copy: 3 x 256 MiB")
+
+    def step():
+        P.forward(img, out=sino)
+        P.adjoint(sino, out=back)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = 1e3 * wall / args.steps
+    value = 2.0 * nnz * args.steps / wall / 1e9
+    log(f"fwd+adj: {ms_step:.3f} ms/step, {value:.1f} GUPS")
+
+    # per-kernel averages with HIP events on the launch stream
+    reps = max(5, min(args.steps, 50))
+    t_fwd = timed(lambda: P.forward(img, out=sino), reps, stream)
+    t_adj = timed(lambda: P.adjoint(sino, out=back), reps, stream)
+    dom_name, dom_ms = ("k_adjoint", t_adj) if t_adj >= t_fwd else ("k_forward", t_fwd)
+    alg_bytes = 4.0 * nnz
+    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    tag = f"{'adjoint' if dom_name == 'k_adjoint' else 'forward'}_{args.workload}"
+    traffic = traffic_from_profiles(tag)
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "kernel": dom_name, "kernel_ms": round(dom_ms, 4),
+                "algorithmic_bytes": alg_bytes,
+                "other_kernel_ms": round(t_fwd if dom_name == "k_adjoint" else t_adj, 4)}
+
+    # <Ax, y> = <x, A^T y> on seeded Gaussian x, y
+    x = torch.from_numpy(gaussian((n, n), 21)).to(dev).reshape(-1)
+    y = torch.from_numpy(gaussian((npj, W), 22)).to(dev).reshape(-1)
+    lhs = torch.dot(P.forward(x).double(), y.double()).item()
+    rhs = torch.dot(x.double(), P.adjoint(y).double()).item()
+    adj_rel = abs(lhs - rhs) / abs(lhs)
+
+    out = {
+        "metric": "fwd+adjoint voxel-ray updates/s (GUPS), 1024^2 fan slice",
+        "value": round(value, 2), "unit": "GUPS", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded ellipse phantom, FP32)",
+        "config": {"workload": f"{args.workload}: n={n} prjWidth={W} np={npj} "
+                               f"prjFull={geo[3]} d={geo[6]}", "nnz": nnz,
+                   "parallelism": "single GPU"},
+        "adjoint_test_rel": adj_rel,
+        "roofline": roofline,
+    }
+
+    if not args.no_extras:
+        # ---- beam-hardening gradient evaluations (512^2 fan, E = 20)
+        log("bhc leg")
+        bg = BHC512
+        PB = Projector([bg])
+        knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
+        xs = np.arange(20)
+        Ie = normalize_ie(np.exp(-0.5 * ((xs - 7.5) / 3.0) ** 2) + 0.05, knots)
+        alpha_np = rasterise(ellipse_params(3, bg[0]), bg[0]).astype(np.float32)
+        alpha = torch.from_numpy(alpha_np).to(dev).reshape(-1)
+        raw = PB.forward(alpha)
+        Ts = float(3.0 / raw.max().item())
+        from imgrecsrc_amd.bhc import Spline
+        BL = Spline("b1", knots).polyIout(raw * Ts, Ie, want=1)[0]
+        g = torch.Generator(device="cpu").manual_seed(5)
+        counts = torch.poisson(1e5 * BL.double().cpu(), generator=g)
+        Imea = (counts / counts.max()).float().to(dev)
+        a_eval = alpha * 0.8 + 0.05
+        for _ in range(3):
+            poissLAlpha(Imea, Ie, a_eval, PB, Ts, knots)
+        bnnz = PB.nnz()
+        t_ga = timed(lambda: poissLAlpha(Imea, Ie, a_eval, PB, Ts, knots), 20, stream)
+        t_gi = timed(lambda: poissLI(Imea, Ie, PB, Ts, knots, alpha=a_eval), 20, stream)
+        roof_evals = HBM_PEAK_GBS * 1e9 / (8.0 * bnnz + 12.0 * PB.sino_elems)
+        out["bhc"] = {"workload": "fan 512^2, 360 views x 729 bins, d=2000, E=20",
+                      "grad_alpha_evals_per_s": round(1e3 / t_ga, 1),
+                      "grad_alpha_ms": round(t_ga, 4),
+                      "grad_ie_evals_per_s": round(1e3 / t_gi, 1), "grad_ie_ms": round(t_gi, 4),
+                      "roofline_evals_per_s": round(roof_evals, 1),
+                      "frac": round((1e3 / t_ga) / roof_evals, 4), "nnz": bnnz}
+
+        # ---- 64 ragged slices: one launch per operator vs one launch per slice
+        log("batched leg")
+        geoms = ragged_geoms()
+        PBat = Projector(geoms)
+        imgs = [torch.from_numpy(ellipse_phantom(gq[0], 100 + i)).to(dev).reshape(-1)
+                for i, gq in enumerate(geoms)]
+        bimg = PBat.pack_images(imgs)
+        bsino = torch.empty(PBat.sino_elems, dtype=torch.float32, device=dev)
+        bback = torch.empty(PBat.img_elems, dtype=torch.float32, device=dev)
+        tot_nnz = PBat.nnz()
+        singles = [Projector([gq]) for gq in geoms]
+        s_out = [torch.empty(s.sino_elems, dtype=torch.float32, device=dev) for s in singles]
+        i_out = [torch.empty(s.img_elems, dtype=torch.float32, device=dev) for s in singles]
+
+        def batched():
+            PBat.forward(bimg, out=bsino)
+            PBat.adjoint(bsino, out=bback)
+
+        def plain():
+            for k, s in enumerate(singles):
+                s.forward(imgs[k], out=s_out[k])
+                s.adjoint(s_out[k], out=i_out[k])
+
+        batched()
+        plain()
+        t_b = timed(batched, 3, stream)
+        t_p = timed(plain, 3, stream)
+        out["batched"] = {"workload": "64 ragged slices, n in {128..1024}, views 90..720, "
+                                      "mixed parallel/fan", "nnz": tot_nnz,
+                          "batched_ms": round(t_b, 3), "plain_ms": round(t_p, 3),
+                          "batched_GUPS": round(2 * tot_nnz / t_b / 1e6, 1),
+                          "plain_GUPS": round(2 * tot_nnz / t_p / 1e6, 1),
+                          "speedup": round(t_p / t_b, 3)}
+
+    if not args.no_cpu_baseline:
+        log("cpu baseline (reference cpuPrj)")
+        sino_np = sino.cpu().numpy()
+        t_cpu, kind, threads, cores = cpu_baseline(geo, img_np, sino_np)
+        out["cpu_baseline"] = {"value": round(2.0 * nnz / t_cpu / 1e9, 3), "unit": "GUPS",
+                               "cores": cores, "threads": threads, "kind": kind,
+                               "sample": f"one fwd + one adj of the {args.workload} slice, "
+                                         f"median of 3 after 1 warm-up ({t_cpu:.2f} s/pair)"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

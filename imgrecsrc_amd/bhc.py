@@ -1,0 +1,163 @@
+"""Polychromatic Poisson beam-hardening model on the device.
+
+Mirrors of the reference's MATLAB contract, computed by libctprj:
+
+* ``Spline('b1', knots).polyIout(s, Ie)`` -> (BL, sBL, ssBL)   -- bhc/Spline.m:11-28,122-203
+* ``poissLAlpha(Imea, Ie, alpha, proj, Ts, knots)`` -> (f, g, hw) -- bhc/poissLAlpha.m
+* ``poissLI(Imea, alpha_or_s, Ie, proj, Ts, knots)`` -> (f, g, H) -- bhc/poissLI.m
+
+``kappa_logspace`` / ``spline_knots`` reproduce the mass-attenuation grid and the
+outer knots of bhc/beamhardenSpline.m:138-139,164-165; ``normalize_ie`` the
+``Ie = Ie / polyIout(0, Ie)`` step of bhc/beamhardenSpline.m:182.
+The FP64 values (NLL, the I-epsilon gradient and Hessian) come back as float64 tensors.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import MAX_E, check, dbl_array, lib
+from .projector import Projector, _ptr, _stream_ptr
+
+
+def kappa_logspace(E: int, lo: float = 0.03, hi: float = 30.0) -> np.ndarray:
+    """E mass-attenuation nodes log-spaced over [lo, hi] (bhc/Spline.m:16 default range)."""
+    return np.logspace(np.log10(lo), np.log10(hi), E)
+
+
+def kappa_logspan(E: int, logspan: float = 3.0) -> np.ndarray:
+    """bhc/beamhardenSpline.m:138-139: logspace(-floor(E/2)/(E-1)*logspan,
+    floor(E/2-0.5)/(E-1)*logspan, E)."""
+    return np.logspace(-np.floor(E / 2) / (E - 1) * logspan,
+                       np.floor(E / 2 - 0.5) / (E - 1) * logspan, E)
+
+
+def spline_knots(kappa) -> np.ndarray:
+    """[kappa_1/q, kappa_1..kappa_E, kappa_E*q], q = kappa_2/kappa_1
+    (bhc/beamhardenSpline.m:164-165)."""
+    k = np.asarray(kappa, np.float64).ravel()
+    q = k[1] / k[0]
+    return np.concatenate([[k[0] / q], k, [k[-1] * q]])
+
+
+def basis_at_zero(knots) -> np.ndarray:
+    """phi_j(0) = (k_{j+2} - k_j)/2: the area of hat j."""
+    k = np.asarray(knots, np.float64)
+    return (k[2:] - k[:-2]) / 2.0
+
+
+def normalize_ie(Ie, knots) -> np.ndarray:
+    """Ie / polyIout(0, Ie) so that BL(0) = 1 (bhc/beamhardenSpline.m:182)."""
+    Ie = np.asarray(Ie, np.float64)
+    return Ie / float(basis_at_zero(knots) @ Ie)
+
+
+def _knots_ie(knots, Ie):
+    k = np.asarray(knots, np.float64).ravel()
+    E = k.size - 2
+    if E < 1 or E > MAX_E:
+        raise ValueError(f"need 3..{MAX_E + 2} knots, got {k.size}")
+    kp, kh = dbl_array(k)
+    if Ie is None:
+        return E, kp, kh, None, None
+    ie = np.asarray(Ie, np.float64).ravel()
+    if ie.size != E:
+        raise ValueError(f"Ie has {ie.size} entries, knots define {E} bases")
+    ip, ih = dbl_array(ie)
+    return E, kp, kh, ip, ih
+
+
+def _f32(t: torch.Tensor, n: int, what: str) -> torch.Tensor:
+    if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float32:
+        raise TypeError(f"{what} must be a float32 CUDA tensor")
+    if t.numel() != n:
+        raise ValueError(f"{what} has {t.numel()} elements, expected {n}")
+    return t.contiguous()
+
+
+class Spline:
+    """``Spline('b1', knots)`` of bhc/Spline.m with the polyIout closure on the device."""
+
+    def __init__(self, sType: str, knots):
+        if sType.lower() != "b1":
+            raise NotImplementedError(f"spectrum basis '{sType}': only 'b1' runs on the device")
+        self.sType = "b1"
+        self.knots = np.asarray(knots, np.float64).ravel()
+
+    def polyIout(self, s: torch.Tensor, Ie=None, want: int = 3):
+        """Ie given: (BL, sBL, ssBL) per ray value.  Ie None: the count x E basis matrix."""
+        n = s.numel()
+        s = _f32(s.reshape(-1), n, "s")
+        E, kp, _kh, ip, _ih = _knots_ie(self.knots, Ie)
+        if Ie is None:
+            A = torch.empty((n, E), dtype=torch.float32, device=s.device)
+            check(lib().ctprj_b1_iout(kp, E, None, _ptr(s), n, _ptr(A), None, None,
+                                      _stream_ptr()))
+            return A
+        outs = [torch.empty(n, dtype=torch.float32, device=s.device) for _ in range(want)]
+        ptrs = [_ptr(o) for o in outs] + [None] * (3 - want)
+        check(lib().ctprj_b1_iout(kp, E, ip, _ptr(s), n, *ptrs, _stream_ptr()))
+        return tuple(outs)
+
+
+def poissLAlpha(Imea: torch.Tensor, Ie, alpha: torch.Tensor, proj: Projector, Ts: float,
+                knots, want_grad: bool = True, want_hess: bool = False):
+    """[f, g(, hw)] of bhc/poissLAlpha.m with Phi = Ts*A, Phit = Ts*A^T, polyIout = b1.
+
+    f: float64 scalar tensor (FP64 reduction); g: float32 image; hw: Hessian ray weight."""
+    alpha = _f32(alpha.reshape(-1), proj.img_elems, "alpha")
+    Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    dev = alpha.device
+    f = torch.zeros(1, dtype=torch.float64, device=dev)
+    g = torch.empty(proj.img_elems, dtype=torch.float32, device=dev) if want_grad else None
+    hw = torch.empty(proj.sino_elems, dtype=torch.float32, device=dev) if want_hess else None
+    check(lib().ctprj_bhc_grad_alpha(proj.handle, _ptr(alpha), _ptr(Imea), kp, E, ip,
+                                     float(Ts), _ptr(f), _ptr(g) if g is not None else None,
+                                     _ptr(hw) if hw is not None else None, _stream_ptr()))
+    out = [f[0]]
+    if want_grad:
+        out.append(g)
+    if want_hess:
+        out.append(hw)
+    return tuple(out) if len(out) > 1 else out[0]
+
+
+def alpha_rays(proj: Projector, sraw: torch.Tensor, Imea: torch.Tensor, Ie, knots, Ts: float,
+               want_w: bool):
+    """Per-ray pass of poissLAlpha on a stored projection sraw = A*alpha: (f, w or None)."""
+    sraw = _f32(sraw.reshape(-1), proj.sino_elems, "sraw")
+    Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    f = torch.zeros(1, dtype=torch.float64, device=sraw.device)
+    w = torch.empty_like(sraw) if want_w else None
+    check(lib().ctprj_bhc_alpha_rays(proj.handle, _ptr(sraw), _ptr(Imea), kp, E, ip, float(Ts),
+                                     _ptr(f), _ptr(w) if w is not None else None, None,
+                                     _stream_ptr()))
+    return f[0], w
+
+
+def poissLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
+            alpha: torch.Tensor | None = None, sraw: torch.Tensor | None = None):
+    """[f, g, H] of bhc/poissLI.m with A = polyIout(Phi(alpha), []) never formed.
+
+    Give alpha (projected inside) or sraw = A*alpha.  Returns float64 tensors."""
+    if (alpha is None) == (sraw is None):
+        raise ValueError("give exactly one of alpha and sraw")
+    Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    dev = Imea.device
+    f = torch.zeros(1, dtype=torch.float64, device=dev)
+    g = torch.zeros(E, dtype=torch.float64, device=dev)
+    H = torch.zeros((E, E), dtype=torch.float64, device=dev)
+    a = _ptr(_f32(alpha.reshape(-1), proj.img_elems, "alpha")) if alpha is not None else None
+    s = _ptr(_f32(sraw.reshape(-1), proj.sino_elems, "sraw")) if sraw is not None else None
+    check(lib().ctprj_bhc_grad_ie(proj.handle, a, s, _ptr(Imea), kp, E, ip, float(Ts), _ptr(f),
+                                  _ptr(g), _ptr(H), _stream_ptr()))
+    return f[0], g, H
+
+
+__all__ = ["Spline", "poissLAlpha", "poissLI", "alpha_rays", "kappa_logspace", "kappa_logspan",
+           "spline_knots", "basis_at_zero", "normalize_ie"]

@@ -1,0 +1,71 @@
+"""Build libctprj.so (HIP, gfx950) in-tree with hipcc.
+
+    python -m imgrecsrc_amd.build           # -> imgrecsrc_amd/lib/libctprj.so
+
+Every HIP source is compiled with packed-FP32 instruction selection disabled and FP
+contraction off: the forward and adjoint kernels must evaluate each footprint weight
+with the same instruction sequence, so no FMA may be formed behind the code's back.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+LIBDIR = os.path.join(PKG, "lib")
+LIB = os.path.join(LIBDIR, "libctprj.so")
+ROOT = os.path.dirname(PKG)
+
+SOURCES = ["ctprj_kernels.hip", "bhc_kernels.hip", "ctprj_api.hip", "bhc_api.hip"]
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
+    "-I", os.path.join(ROOT, "include"),
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError(f"command failed ({r.returncode}): {' '.join(cmd)}")
+    # hipcc's host pass does not know the packed-fp32 target feature; drop that noise
+    noise = ("packed-fp32-ops", "not a recognized feature", "ignoring feature")
+    for line in (r.stdout + r.stderr).splitlines():
+        if line.strip() and not any(n in line for n in noise):
+            sys.stderr.write(line + "\n")
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(LIBDIR, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
+        cmd = [HIPCC, *FLAGS, "-c", path, "-o", obj]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    for o in objs:
+        os.remove(o)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

@@ -1,0 +1,94 @@
+// Device-side data layout and the pixel/strip footprint weight shared by every
+// projector kernel.  Forward and adjoint evaluate a weight through the same inline
+// functions on the same float inputs (per-ray table entries + integer pixel
+// coordinates), so A and A^T are bitwise transposes of each other.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ctprj {
+
+// Margin (pixels) added to every support bound.  Bounds only select candidates: the
+// weight itself is exactly 0 outside the footprint, so a margin never changes a result.
+constexpr float kSupportMargin = 0.02f;
+
+// One slice of a plan.  Offsets are element offsets into the packed arrays.
+struct SliceDesc {
+    int n, W, np, fan;     // image edge, bins per view, views, 1 = fan beam
+    int H;                 // (n-1)/2: pixel coordinates run over [-H, H]
+    int half;              // n/2: coordinate -> index offset
+    int pC;                // W/2: index of the bin at t = 0
+    int blo, bhi;          // bins that carry rays: [pC-(W-1)/2, pC+(W-1)/2]
+    float dSize, invDSize; // detector pitch
+    float d, invd;         // source distance (fan), 0 for parallel
+    float hbd;             // half beam width on the detector: dSize*effectiveRate/2
+    float er;              // effectiveRate (outputs are divided by it)
+    float fbp_scale;       // PI/np/dSize^2 (prj/cpuPrj.c:161,304)
+    long long img_off, sino_off, view_off, ray_off;
+};
+
+// Per-ray footprint constants.  Parallel beam: one entry per view (dc unused, the bin
+// position is (b-pC)*dSize).  Fan beam: one entry per (view, bin).
+struct __attribute__((aligned(16))) Ray {
+    float cosR, sinR;  // unit normal of the ray line: dist = x*cosR + y*sinR - dc
+    float dc;          // offset of the ray line from the rotation centre
+    float hbc;         // half strip width at the isocentre, perpendicular to the ray
+    float h;           // trapezoid height / effectiveRate: 1/(max(|c|,|s|)*er)
+    float b1;          // plateau half width (max-min)/2
+    float s;           // ramp width min(|c|,|s|)
+    float k;           // 1/(2 s), 0 when s == 0
+};
+
+// Cumulative integral, divided by the height, of the unit-area trapezoid footprint of a
+// pixel (the projection of the unit square on the detector normal) up to offset u:
+//   0 for u <= -b2, ramp (u+b2)^2/(2s) up to -b1, plateau, ramp down, c for u >= b2,
+// with b2 = b1 + s.  Written with clamps (v_med3_f32), no branches.
+__device__ __forceinline__ float gcum(float u, float b1, float b2, float s, float b1x2,
+                                      float k)
+{
+    const float u1 = __builtin_amdgcn_fmed3f(u + b2, 0.0f, s);
+    const float u2 = __builtin_amdgcn_fmed3f(u + b1, 0.0f, b1x2);
+    const float u3 = __builtin_amdgcn_fmed3f(u - b1, 0.0f, s);
+    return (u2 + u3) + k * ((u1 - u3) * (u1 + u3));
+}
+
+// Area of overlap of the unit pixel centred `dist` from the ray line with the strip of
+// half width hb (prj/utils.h:12-92 computes the same area by a 20-way case split).
+__device__ __forceinline__ float strip_weight(float dist, float hb, const Ray& r)
+{
+    const float b2 = r.b1 + r.s;
+    const float b1x2 = r.b1 + r.b1;
+    const float gp = gcum(hb - dist, r.b1, b2, r.s, b1x2, r.k);
+    const float gm = gcum(-hb - dist, r.b1, b2, r.s, b1x2, r.k);
+    return r.h * (gp - gm);
+}
+
+// Signed distance of pixel (x, y) from the ray line.
+__device__ __forceinline__ float ray_dist(float x, float y, float cosR, float sinR, float dc)
+{
+    return fmaf(x, cosR, fmaf(y, sinR, -dc));
+}
+
+// Fan beam: strip width at the pixel relative to the isocentre width,
+// 1 + (x sinT - y cosT)/d (prj/cpuPrj.c:549, 254).
+__device__ __forceinline__ float fan_depth_scale(float x, float y, float cosT, float sinT,
+                                                 float invd)
+{
+    return fmaf(fmaf(x, sinT, -(y * cosT)), invd, 1.0f);
+}
+
+// Compensated accumulation of a partial sum (Kahan): keeps the long sums over rows /
+// views at FP32 rounding of the result instead of growing with the term count.
+struct KahanAcc {
+    float s = 0.0f, c = 0.0f;
+    __device__ __forceinline__ void add(float v)
+    {
+        const float y = v - c;
+        const float t = s + y;
+        c = (t - s) - y;
+        s = t;
+    }
+};
+
+}  // namespace ctprj

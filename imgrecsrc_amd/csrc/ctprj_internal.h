@@ -1,0 +1,78 @@
+// Internal (non-ABI) declarations shared by the host files of libctprj.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ctprj.h"
+#include "ctprj_device.h"
+
+namespace ctprj {
+
+struct FwdItem {
+    int slice, view, bin0, pad;
+};
+struct AdjItem {
+    int slice, row0, col0, pad;
+};
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+struct Plan {
+    int device = 0;
+    int nslices = 0;
+    std::vector<ctprj_geom> geoms;
+    std::vector<long long> img_offsets, sino_offsets;
+    long long img_elems = 0, sino_elems = 0;
+    int max_W = 0;
+    ctprj_tune tune{64, 16, 0};
+
+    SliceDesc* d_desc = nullptr;
+    float4* d_views = nullptr;
+    Ray* d_rays = nullptr;
+    FwdItem* d_fwd_items = nullptr;
+    AdjItem* d_adj_items = nullptr;
+    AdjItem* d_tr_items = nullptr;
+    int2* d_fbp_items = nullptr;
+    float* d_fbp_kern = nullptr;
+    long long* d_fbp_kern_off = nullptr;
+    float* d_fbp_prew = nullptr;
+    long long* d_fbp_prew_off = nullptr;
+    int n_fwd_items = 0, n_adj_items = 0, n_tr_items = 0, n_fbp_items = 0;
+
+    float* d_imgT = nullptr;     // transposed image (forward, steep views)
+    float* d_sino_ws = nullptr;  // sinogram workspace (FBP filter output, BHC ray values)
+    double* d_red = nullptr;     // reduction partials (beam-hardening)
+    double* d_red_out = nullptr; // reduced sums
+    long long red_cap = 0;       // doubles held by d_red
+
+    ~Plan();
+    int build_fwd_items();
+    int build_adj_items();
+    int launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count);
+    int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp);
+    int launch_fbp(const float* d_sino, float* d_img, hipStream_t st);
+};
+
+// Kernel launchers (ctprj_kernels.hip).
+void launch_build_rays(const SliceDesc* desc, int nslices, const float4* views,
+                       const float4* bins, const long long* bin_off, const long long* ray_prefix,
+                       Ray* rays, long long nray, hipStream_t st);
+void launch_transpose(int nitems, const SliceDesc* desc, const AdjItem* items, const float* in,
+                      float* out, hipStream_t st);
+void launch_forward_kernel(bool count, int block, int nitems, const SliceDesc* desc,
+                           const FwdItem* items, const float4* views, const Ray* rays,
+                           const float* img, const float* imgT, float* sino, hipStream_t st);
+void launch_adjoint_kernel(int tile, bool fbp, int nitems, const SliceDesc* desc,
+                           const AdjItem* items, const float4* views, const Ray* rays,
+                           const float* sino, float* img, hipStream_t st);
+void launch_fbp_filter(int nitems, size_t shmem, const SliceDesc* desc, const int2* items,
+                       const float* kern, const long long* kern_off, const float* prew,
+                       const long long* prew_off, const float* sino, float* out,
+                       hipStream_t st);
+void launch_scale(float* p, long long n, float s, hipStream_t st);
+
+}  // namespace ctprj

@@ -1,0 +1,177 @@
+/*
+ * libctprj -- MI355X (gfx950) X-ray CT projector, backprojector and polychromatic
+ * Poisson beam-hardening likelihood.  Plain C ABI: pointers, sizes and int status codes.
+ *
+ * Conventions follow the reference projector (prj/prj.h:8-57, prj/cpuPrj.c):
+ *   image  n x n float, row-major, pixel (x, y) at [(y + n/2) * n + x + n/2],
+ *          x, y in [-(n-1)/2, (n-1)/2] (integer division); for even n, row 0 and
+ *          column 0 are never read by the forward and are written as 0 by the adjoint;
+ *   sinogram np x prjWidth float, view-major; view a at theta = a*2*PI/prjFull with
+ *          PI = 3.14159265359f; bin b at t = (b - prjWidth/2) * dSize; for an even
+ *          prjWidth, bin 0 is written as 0 by the forward and never read by the adjoint;
+ *   beam   width dSize*effectiveRate; outputs are divided by effectiveRate;
+ *          d > 0: fan beam, source at distance d from the rotation centre, virtual flat
+ *          detector through the centre; d <= 0: parallel beam.
+ *
+ * All entry points return 0 on success or a negative CTPRJ_E* code; ctprj_last_error()
+ * returns the message of the last failure on the calling thread.
+ */
+#ifndef CTPRJ_H
+#define CTPRJ_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* command bits, prj/prj.h:30-32 */
+#define CTPRJ_FWD_BIT (1 << 1)
+#define CTPRJ_BWD_BIT (1 << 2)
+#define CTPRJ_FBP_BIT (1 << 3)
+
+/* status codes */
+#define CTPRJ_OK 0
+#define CTPRJ_EINVAL (-1)   /* bad argument / geometry */
+#define CTPRJ_ENODEV (-2)   /* no usable gfx950 device */
+#define CTPRJ_ENOMEM (-3)   /* device or host allocation failed */
+#define CTPRJ_EHIP (-4)     /* HIP runtime error */
+#define CTPRJ_ESTATE (-5)   /* call out of order (e.g. prj_run before prj_setup) */
+
+/* Maximum spectrum basis size accepted by the beam-hardening entry points. */
+#define CTPRJ_MAX_E 32
+
+/* Geometry of one slice: the fields of struct prjConf, prj/prj.h:42-57. */
+typedef struct ctprj_geom {
+    int n;               /* image is n x n */
+    int prjWidth;        /* detector bins per view */
+    int np;              /* views computed */
+    int prjFull;         /* views per 360 degrees */
+    float dSize;         /* detector pitch, pixels */
+    float effectiveRate; /* beam width / pitch */
+    float d;             /* source-to-centre distance (pixels); <= 0 selects parallel */
+} ctprj_geom;
+
+typedef struct ctprj_plan ctprj_plan;
+
+/* ------------------------------------------------------------------------------
+ * Reference-compatible single-slice interface (host pointers, synchronous).
+ * ------------------------------------------------------------------------------ */
+
+/* Replaces setup() of prj/prj.h:60 / prj/cpuPrj.c:618-653 (also prj/gpuPrj.cu:794-865).
+ * Configures the process-wide slice geometry used by prj_run. */
+int prj_setup(int n, int prjWidth, int np, int prjFull, float dSize, float effectiveRate,
+              float d);
+
+/* Replaces showSetup() of prj/prj.h:61 / prj/cpuPrj.c:655-663: prints the geometry. */
+int prj_show(void);
+
+/* Replaces cpuPrj()/gpuPrj() of prj/prj.h:63-65 (prj/cpuPrj.c:679-806,
+ * prj/gpuPrj.cu:887-1047).  cmd is CTPRJ_FWD_BIT (img -> sino), CTPRJ_BWD_BIT
+ * (sino -> img) or CTPRJ_FBP_BIT (sino -> img).  Host pointers; copies to and from the
+ * device and returns when the result is in host memory. */
+int prj_run(float* img, float* sino, int cmd);
+
+/* Reads back the geometry configured by prj_setup (the MEX 'config' echo,
+ * prj/mPrj.c:160-170). */
+int prj_get_geom(ctprj_geom* out);
+
+/* ------------------------------------------------------------------------------
+ * Plan (handle) interface: device pointers, caller's HIP stream, many slices per call.
+ * No reference counterpart as one call; each slice behaves as cpuPrj on that slice.
+ * ------------------------------------------------------------------------------ */
+
+/* Plan for `nslices` independent slices (ragged sizes and mixed parallel/fan allowed).
+ * Images of all slices are packed back to back in one float array, likewise sinograms;
+ * see ctprj_plan_layout.  Allocates the plan's device tables and workspace on the
+ * current HIP device. */
+int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_plan** plan);
+int ctprj_plan_destroy(ctprj_plan* plan);
+
+/* Total packed sizes (elements) and, when the arrays are non-NULL, per-slice element
+ * offsets (nslices entries each). */
+int ctprj_plan_layout(const ctprj_plan* plan, long long* img_elems, long long* sino_elems,
+                      long long* img_offsets, long long* sino_offsets);
+
+/* Number of non-zeros of the system matrix A of every slice, summed (the voxel-ray
+ * updates of one application), computed on the device. */
+int ctprj_plan_nnz(ctprj_plan* plan, long long* nnz, void* stream);
+
+/* Kernel shape knobs (the autotuner's search space). */
+typedef struct ctprj_tune {
+    int fwd_block;   /* threads per forward work-group: 64, 128 or 256 */
+    int adj_tile;    /* adjoint pixel tile edge: 8 or 16 */
+    int flags;       /* bit0: use the 8-fold symmetric kernels where the geometry allows */
+} ctprj_tune;
+int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
+int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);
+/* Times every candidate shape on this plan's geometry (HIP events, `reps` runs each),
+ * keeps the fastest and, if cache_path is non-NULL, records it in that text file keyed
+ * by (geometry, device arch). */
+int ctprj_plan_autotune(ctprj_plan* plan, int reps, const char* cache_path, void* stream);
+
+/* sino = A img  (prj/cpuPrj.c:319-616 semantics), all slices in one launch. */
+int ctprj_forward(ctprj_plan* plan, const float* d_img, float* d_sino, void* stream);
+/* img = A^T sino (prj/cpuPrj.c:46-317 semantics), all slices in one launch. */
+int ctprj_adjoint(ctprj_plan* plan, const float* d_sino, float* d_img, void* stream);
+/* FBP command (prj/cpuPrj.c:718-784): fan pre-weight, ramp filter, weighted adjoint. */
+int ctprj_fbp(ctprj_plan* plan, const float* d_sino, float* d_img, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * Polychromatic Poisson beam-hardening model (bhc/poissLAlpha.m, bhc/poissLI.m,
+ * bhc/Spline.m:b1Iout).  `knots` holds the E+2 hat-basis nodes
+ * [kappa_1/q, kappa_1..kappa_E, kappa_E*q] (bhc/beamhardenSpline.m:164-165); Ie the E
+ * basis coefficients; Ts the operator scale (Phi = Ts*A, tomography/ConfigCT.m:118-119).
+ * Host arrays for knots/Ie (E <= CTPRJ_MAX_E); device arrays for images/sinograms.
+ * NLL and I-epsilon reductions are FP64; projections are FP32.
+ * ------------------------------------------------------------------------------ */
+
+/* Spectrum sum of bhc/Spline.m:122-203 on device: for every ray value s[i],
+ * BL[i] = sum_j Ie[j] phi_j(s[i]), sBL = d/ds BL, ssBL = d2/ds2 BL.  Any of the outputs
+ * may be NULL.  With Ie == NULL the basis matrix phi_j(s[i]) is written to BL as
+ * count x E (row-major) instead. */
+int ctprj_b1_iout(const double* knots, int E, const double* Ie, const float* d_s,
+                  long long count, float* d_BL, float* d_sBL, float* d_ssBL, void* stream);
+
+/* [f, g] = poissLAlpha(Imea, Ie, alpha, Phi, Phit, polyIout) (bhc/poissLAlpha.m:1-13):
+ * s = Ts*A*alpha, f = sum(BL - Imea) - sum Imea*log(BL/Imea) (FP64, 0*log(.) = 0 for
+ * Imea = 0), grad = Ts*A^T[(1 - Imea/BL) * sBL].  d_f: one device double.
+ * d_grad may be NULL (cost only: one forward projection).  d_hw, if non-NULL, receives
+ * the Hessian ray weight (sBL/BL)^2*Imea + ssBL*(1 - Imea/BL) (bhc/poissLAlpha.m:15). */
+int ctprj_bhc_grad_alpha(ctprj_plan* plan, const float* d_alpha, const float* d_Imea,
+                         const double* knots, int E, const double* Ie, float Ts,
+                         double* d_f, float* d_grad, float* d_hw, void* stream);
+
+/* The per-ray part of poissLAlpha on a given projection: with s = Ts * d_sraw
+ * (d_sraw = A*alpha, np*prjWidth per slice, packed), writes d_f (one device double) and,
+ * when non-NULL, d_w = Ts*(1 - Imea/BL)*sBL (may alias d_sraw) and d_hw (Hessian ray
+ * weight).  grad = A^T d_w is then ctprj_adjoint(d_w).  Lets a caller keep only the
+ * projection between a cost and a gradient call: the spectral exponentials are
+ * recomputed, never stored. */
+int ctprj_bhc_alpha_rays(ctprj_plan* plan, const float* d_sraw, const float* d_Imea,
+                         const double* knots, int E, const double* Ie, float Ts, double* d_f,
+                         float* d_w, float* d_hw, void* stream);
+
+/* [f, g, H] = poissLI(Imea, A, Ie) with A = polyIout(Phi(alpha), []) (bhc/poissLI.m,
+ * bhc/beamhardenSpline.m:447): Ir = A*Ie, f as above, g = A^T(1 - Imea/Ir) (E doubles),
+ * H = A^T diag(Imea/Ir^2) A (E x E doubles, row-major) -- FP64 sums, written to device
+ * memory; the ray x basis matrix A is never formed.  Ray values are s = Ts * d_sraw with
+ * d_sraw = A*alpha (packed); pass d_sraw = NULL together with d_alpha to project inside.
+ * d_f / d_g / d_H may be NULL. */
+int ctprj_bhc_grad_ie(ctprj_plan* plan, const float* d_alpha, const float* d_sraw,
+                      const float* d_Imea, const double* knots, int E, const double* Ie,
+                      float Ts, double* d_f, double* d_g, double* d_H, void* stream);
+
+/* ------------------------------------------------------------------------------
+ * Library state.
+ * ------------------------------------------------------------------------------ */
+const char* ctprj_last_error(void);
+/* Device name / arch string of the current device (e.g. "gfx950"); 0 on success. */
+int ctprj_device_arch(char* buf, int buflen);
+/* Library version string. */
+const char* ctprj_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CTPRJ_H */

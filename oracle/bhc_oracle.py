@@ -1,0 +1,275 @@
+"""FP64 NumPy restatement of the beam-hardening model -- TEST INFRASTRUCTURE ONLY.
+
+Follows, line by line:
+  * ``b1_iout``  -- bhc/Spline.m:122-203 (closed form + 3rd-order Taylor branch below
+                    |s| < |EPS*temp1(1)/temp1(end)|^(1/3), EPS = 1e-6)
+  * ``b0_iout``  -- bhc/Spline.m:205-262
+  * ``dis_iout`` -- bhc/Spline.m:264-302
+  * ``poiss_l_alpha`` -- bhc/poissLAlpha.m:1-19
+  * ``poiss_l_i``     -- bhc/poissLI.m:1-26
+  * knots / kappa grid -- bhc/beamhardenSpline.m:138-139,164-165,182
+
+Pinned without MATLAB (the reference's BHC code is MATLAB and cannot run here) by:
+quadrature of phi_j(s) = int hat_j(kappa) exp(-kappa s) dkappa (scipy.integrate.quad),
+the closed value phi_j(0) = (c-a)/2, an independent stable evaluation
+(``b1_iout_quad``), central finite differences of the likelihoods, and the identity
+g . Ie = sum(Ir - Imea) of poissLI.  Exact MATLAB outputs of the Taylor branch are
+"parity unpinned" beyond those checks.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def b1_iout(k, s, I=None, nout=3):
+    """bhc/Spline.m:122-203.  I None -> basis matrices (len(s) x E)."""
+    EPS = 1e-6
+    k = np.asarray(k, np.float64).ravel()
+    s = np.asarray(s, np.float64).ravel()
+    E = len(k) - 2
+    shape = (len(s),) if I is not None else (len(s), E)
+    BL = np.zeros(shape)
+    sBL = np.zeros(shape)
+    ssBL = np.zeros(shape)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        expksC = np.exp(-k[0] * s)
+        expksR = np.exp(-k[1] * s)
+        s2, s3, s4, abss = s ** 2, s ** 3, s ** 4, np.abs(s)
+        for i in range(E):
+            a, b, c = k[i], k[i + 1], k[i + 2]
+            expksL = expksC
+            expksC = expksR
+            expksR = np.exp(-c * s)
+            temp = -(expksL - expksC) / (s2 * (a - b)) + (expksC - expksR) / (s2 * (b - c))
+            t1 = np.array([
+                -a / 2 + c / 2,
+                (a ** 3 - b ** 3) / (6 * (a - b)) - (b ** 3 - c ** 3) / (6 * (b - c)),
+                -(a ** 4 - b ** 4) / (24 * (a - b)) + (b ** 4 - c ** 4) / (24 * (b - c)),
+                (a ** 5 - b ** 5) / (120 * (a - b)) - (b ** 5 - c ** 5) / (120 * (b - c)),
+            ])
+            thr = abs(EPS * t1[0] / t1[-1]) ** (1.0 / (len(t1) - 1))
+            idx = abss < thr
+            temp[idx] = t1[0] + np.stack([s[idx], s2[idx], s3[idx]], 1) @ t1[1:]
+            if I is not None:
+                BL += temp * I[i]
+            else:
+                BL[:, i] = temp
+            if nout > 1:
+                temp = (2 * (expksL - expksC)) / (s3 * (a - b)) + (a * expksL - b * expksC) / (s2 * (a - b)) \
+                    - (2 * (expksC - expksR)) / (s3 * (b - c)) - (b * expksC - c * expksR) / (s2 * (b - c))
+                t1 = np.array([
+                    (a ** 3 - b ** 3) / (6 * (a - b)) - (b ** 3 - c ** 3) / (6 * (b - c)),
+                    -(a ** 4 - b ** 4) / (12 * (a - b)) + (b ** 4 - c ** 4) / (12 * (b - c)),
+                    (a ** 5 - b ** 5) / (40 * (a - b)) - (b ** 5 - c ** 5) / (40 * (b - c)),
+                    -(a ** 6 - b ** 6) / (180 * (a - b)) + (b ** 6 - c ** 6) / (180 * (b - c)),
+                ])
+                thr = abs(EPS * t1[0] / t1[-1]) ** (1.0 / (len(t1) - 1))
+                idx = abss < thr
+                temp[idx] = t1[0] + np.stack([s[idx], s2[idx], s3[idx]], 1) @ t1[1:]
+                if I is not None:
+                    sBL += temp * I[i]
+                else:
+                    sBL[:, i] = temp
+            if nout > 2:
+                temp = -(6 * expksL - 6 * expksC + s2 * (a ** 2 * expksL - b ** 2 * expksC)
+                         + s * (4 * a * expksL - 4 * b * expksC)) / (s4 * (a - b)) \
+                    + (6 * expksC - 6 * expksR + s2 * (b ** 2 * expksC - c ** 2 * expksR)
+                       + s * (4 * b * expksC - 4 * c * expksR)) / (s4 * (b - c))
+                t1 = np.array([
+                    -((a + b) * (a ** 2 + b ** 2)) / 12 + ((b + c) * (b ** 2 + c ** 2)) / 12,
+                    (a ** 4 + a ** 3 * b + a ** 2 * b ** 2 + a * b ** 3 + b ** 4) / 20
+                    + (-b ** 4 - b ** 3 * c - b ** 2 * c ** 2 - b * c ** 3 - c ** 4) / 20,
+                    -((a + b) * (a ** 2 + a * b + b ** 2) * (a ** 2 - a * b + b ** 2)) / 60
+                    + ((b + c) * (b ** 2 + b * c + c ** 2) * (b ** 2 - b * c + c ** 2)) / 60,
+                    (a ** 6 + a ** 5 * b + a ** 4 * b ** 2 + a ** 3 * b ** 3 + a ** 2 * b ** 4 + a * b ** 5 + b ** 6) / 252
+                    + (-b ** 6 - b ** 5 * c - b ** 4 * c ** 2 - b ** 3 * c ** 3 - b ** 2 * c ** 4 - b * c ** 5 - c ** 6) / 252,
+                ])
+                thr = abs(EPS * t1[0] / t1[-1]) ** (1.0 / (len(t1) - 1))
+                idx = abss < thr
+                temp[idx] = t1[0] + np.stack([s[idx], s2[idx], s3[idx]], 1) @ t1[1:]
+                if I is not None:
+                    ssBL += temp * I[i]
+                else:
+                    ssBL[:, i] = temp
+    return BL, sBL, ssBL
+
+
+def b1_iout_quad(k, s, j):
+    """phi_j(s), phi_j'(s), phi_j''(s) by adaptive quadrature of the hat (independent pin)."""
+    from scipy.integrate import quad
+    a, b, c = k[j], k[j + 1], k[j + 2]
+
+    def hat(x):
+        return (x - a) / (b - a) if x <= b else (c - x) / (c - b)
+
+    out = []
+    for p in range(3):
+        f = lambda x, p=p: ((-x) ** p) * hat(x) * np.exp(-x * s)  # noqa: E731
+        v = quad(f, a, b, epsabs=0, epsrel=1e-13, limit=200)[0] + \
+            quad(f, b, c, epsabs=0, epsrel=1e-13, limit=200)[0]
+        out.append(v)
+    return tuple(out)
+
+
+def _moments(z):
+    """M_n(z) = int_0^1 u^n exp(-z u) du, n = 0..3, vectorised over z (FP64).
+    |z| <= 2: 18-term series; else the upward recurrence."""
+    z = np.asarray(z, np.float64)
+    small = np.abs(z) <= 2.0
+    M = np.zeros((4,) + z.shape)
+    zs = np.where(small, z, 0.0)
+    t = np.ones_like(z)
+    a = [np.ones_like(z), np.full_like(z, 0.5), np.full_like(z, 1 / 3), np.full_like(z, 0.25)]
+    for k in range(1, 19):
+        t = t * (-zs / k)
+        for n in range(4):
+            a[n] = a[n] + t / (k + n + 1)
+    zl = np.where(small, 1.0, z)
+    e = np.exp(-zl)
+    m0 = (1 - e) / zl
+    m1 = (m0 - e) / zl
+    m2 = (2 * m1 - e) / zl
+    m3 = (3 * m2 - e) / zl
+    for n, big in enumerate((m0, m1, m2, m3)):
+        M[n] = np.where(small, a[n], big)
+    return M
+
+
+def b1_iout_stable(k, s, I=None, nout=3):
+    """The B1 spectrum transform by segment moments, smooth in s (no branch switch):
+    substituting kappa = k_j + Delta_j u in phi_j(s) = int hat_j(kappa) e^{-kappa s} dkappa.
+    Same contract as b1_iout; pinned against quadrature (b1_iout_quad)."""
+    k = np.asarray(k, np.float64).ravel()
+    s = np.asarray(s, np.float64).ravel()
+    E = len(k) - 2
+    P = np.zeros((3, len(s), E))
+    for j in range(E + 1):
+        kj, dj = k[j], k[j + 1] - k[j]
+        w = dj * np.exp(-kj * s)
+        m0, m1, m2, m3 = _moments(dj * s)
+        if j >= 1:
+            P[0, :, j - 1] += w * (m0 - m1)
+            P[1, :, j - 1] += -w * (kj * (m0 - m1) + dj * (m1 - m2))
+            P[2, :, j - 1] += w * (kj * kj * (m0 - m1) + 2 * kj * dj * (m1 - m2) + dj * dj * (m2 - m3))
+        if j < E:
+            P[0, :, j] += w * m1
+            P[1, :, j] += -w * (kj * m1 + dj * m2)
+            P[2, :, j] += w * (kj * kj * m1 + 2 * kj * dj * m2 + dj * dj * m3)
+    if I is None:
+        return P[0], P[1], P[2]
+    I = np.asarray(I, np.float64)
+    return P[0] @ I, P[1] @ I, P[2] @ I
+
+
+def b0_iout(k, s, I=None, nout=3):
+    """bhc/Spline.m:205-262 (piecewise-constant basis)."""
+    EPS = 1e-6
+    k = np.asarray(k, np.float64).ravel()
+    s = np.asarray(s, np.float64).ravel()
+    E = len(k) - 1
+    shape = (len(s),) if I is not None else (len(s), E)
+    BL, sBL, ssBL = np.zeros(shape), np.zeros(shape), np.zeros(shape)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        expksR = np.exp(-k[0] * s)
+        s2, s3, abss = s ** 2, s ** 3, np.abs(s)
+        for i in range(E):
+            a, b = k[i], k[i + 1]
+            expksL = expksR
+            expksR = np.exp(-b * s)
+            temp = (expksL - expksR) / s
+            t1 = np.array([b - a, a ** 2 / 2 - b ** 2 / 2, b ** 3 / 6 - a ** 3 / 6, a ** 4 / 24 - b ** 4 / 24])
+            thr = abs(EPS * t1[0] / t1[-1]) ** (1 / 3)
+            idx = abss < thr
+            temp[idx] = t1[0] + np.stack([s[idx], s2[idx], s3[idx]], 1) @ t1[1:]
+            if I is not None:
+                BL += temp * I[i]
+            else:
+                BL[:, i] = temp
+            if nout > 1:
+                temp = -((s * a + 1) * expksL - (s * b + 1) * expksR) / s2
+                t1 = np.array([a ** 2 / 2 - b ** 2 / 2, b ** 3 / 3 - a ** 3 / 3, a ** 4 / 8 - b ** 4 / 8, b ** 5 / 30 - a ** 5 / 30])
+                thr = abs(EPS * t1[0] / t1[-1]) ** (1 / 3)
+                idx = abss < thr
+                temp[idx] = t1[0] + np.stack([s[idx], s2[idx], s3[idx]], 1) @ t1[1:]
+                if I is not None:
+                    sBL += temp * I[i]
+                else:
+                    sBL[:, i] = temp
+            if nout > 2:
+                temp = (((a * s + 2) * a * s + 2) * expksL - ((b * s + 2) * b * s + 2) * expksR) / s3
+                t1 = np.array([(b ** 3 - a ** 3) / 3, (a ** 4 - b ** 4) / 4, (b ** 5 - a ** 5) / 10, (a ** 6 - b ** 6) / 36])
+                thr = abs(EPS * t1[0] / t1[-1]) ** (1 / 3)
+                idx = abss < thr
+                temp[idx] = t1[0] + t1[1] * s[idx] + t1[2] * s2[idx]
+                if I is not None:
+                    ssBL += temp * I[i]
+                else:
+                    ssBL[:, i] = temp
+    return BL, sBL, ssBL
+
+
+def dis_iout(k, s, I=None, nout=3):
+    """bhc/Spline.m:264-302 (discrete kappa)."""
+    k = np.asarray(k, np.float64).ravel()
+    s = np.asarray(s, np.float64).ravel()
+    dk = (k[2:] - k[:-2]) / 2
+    kk = k[1:-1]
+    shape = (len(s),) if I is not None else (len(s), len(kk))
+    BL, sBL, ssBL = np.zeros(shape), np.zeros(shape), np.zeros(shape)
+    for i in range(len(kk)):
+        temp = np.exp(-kk[i] * s) * dk[i]
+        if I is not None:
+            BL += temp * I[i]
+        else:
+            BL[:, i] = temp
+        temp = temp * (-kk[i])
+        if I is not None:
+            sBL += temp * I[i]
+        else:
+            sBL[:, i] = temp
+        temp = temp * (-kk[i])
+        if I is not None:
+            ssBL += temp * I[i]
+        else:
+            ssBL[:, i] = temp
+    return BL, sBL, ssBL
+
+
+def _nll(BL, Imea):
+    """sum(BL-Imea) - Imea'*log(BL./Imea) with 0*log(.) = 0 for Imea = 0."""
+    BL = np.asarray(BL, np.float64)
+    Imea = np.asarray(Imea, np.float64)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        err = np.where(Imea > 0, np.log(BL / np.where(Imea > 0, Imea, 1.0)), 0.0)
+    return float(np.sum(BL - Imea) - Imea @ err)
+
+
+def poiss_l_alpha(Imea, Ie, alpha, Phi, Phit, polyIout=b1_iout, knots=None):
+    """bhc/poissLAlpha.m: f, g, Hessian ray weight."""
+    R = Phi(alpha)
+    BL, sBL, ssBL = polyIout(knots, R, Ie)
+    f = _nll(BL, Imea)
+    g = Phit((1 - Imea / BL) * sBL)
+    hw = ((sBL / BL) ** 2) * Imea + ssBL * (1 - Imea / BL)
+    return f, g, hw
+
+
+def poiss_l_i(Imea, A, Ie):
+    """bhc/poissLI.m: f, g = A'(1-Imea./Ir), H = A' diag(Imea./Ir.^2) A."""
+    Ir = A @ np.asarray(Ie, np.float64)
+    ratio = Imea / Ir
+    g = A.T @ (1 - ratio)
+    f = _nll(Ir, Imea)
+    w = ratio / Ir
+    H = A.T @ (A * w[:, None])
+    return f, g, H
+
+
+def kappa_logspace(E, lo=0.03, hi=30.0):
+    return np.logspace(np.log10(lo), np.log10(hi), E)
+
+
+def spline_knots(kappa):
+    k = np.asarray(kappa, np.float64)
+    q = k[1] / k[0]
+    return np.concatenate([[k[0] / q], k, [k[-1] * q]])

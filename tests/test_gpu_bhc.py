@@ -1,0 +1,174 @@
+"""GPU parity of the beam-hardening model (bhc/Spline.m b1Iout, bhc/poissLAlpha.m,
+bhc/poissLI.m) against the FP64 oracle, plus finite-difference checks of the GPU
+gradients and the autograd binding.
+
+FP32 per-ray arithmetic and projections, FP64 reductions.  Bars: BL/sBL/ssBL 1e-5 of the
+basis scale; NLL and gradients 1e-5 relative; Hessian 1e-5 relative."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+GEO = (64, 91, 72, 72, 1.0, 1.0, 250.0)  # reduced 512^2/729-bin fan workload
+E = 20
+
+
+@pytest.fixture(scope="module")
+def problem(oracle_mod):
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.phantom import rasterise, ellipse_params
+    n, W, npj = GEO[0], GEO[1], GEO[2]
+    knots = B.spline_knots(B.kappa_logspace(E, 0.03, 30.0))
+    x = np.arange(E)
+    Ie = np.exp(-0.5 * ((x - 7.5) / 3.0) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    alpha_true = rasterise(ellipse_params(3, n, amp_lo=0.0, amp_hi=1.0), n)
+    cf = oracle_mod.conf(*GEO)
+    raw, _ = oracle_mod.forward(cf, alpha_true)
+    Ts = 3.0 / raw.max()
+    BL, _, _ = B.b1_iout_stable(knots, Ts * raw.ravel(), Ie)
+    rng = np.random.default_rng(33)
+    counts = rng.poisson(1e5 * BL)
+    Imea = counts / counts.max()
+    alpha = alpha_true * 0.8 + 0.05  # evaluation point away from the truth
+    return dict(knots=knots, Ie=Ie, Ts=Ts, Imea=Imea, alpha=alpha, cf=cf,
+                Imea_t=torch.from_numpy(Imea.astype(np.float32)).cuda(),
+                alpha_t=torch.from_numpy(alpha.astype(np.float32).ravel()).cuda())
+
+
+def test_b1_iout_device(problem):
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import Spline
+    s = np.concatenate([[0.0, 1e-7, 1e-4], np.logspace(-3, 1.3, 500), [-0.02]]).astype(np.float32)
+    sp = Spline("b1", problem["knots"])
+    st = torch.from_numpy(s).cuda()
+    BL, sBL, ssBL = [t.cpu().numpy() for t in sp.polyIout(st, problem["Ie"])]
+    r = B.b1_iout_stable(problem["knots"], s.astype(np.float64), problem["Ie"])
+    for got, ref in zip((BL, sBL, ssBL), r):
+        assert np.all(np.abs(got - ref) <= 1e-5 * np.abs(ref[0]) + 1e-5 * np.abs(ref))
+    A = sp.polyIout(st, None).cpu().numpy()
+    Ar, _, _ = B.b1_iout_stable(problem["knots"], s.astype(np.float64), None)
+    assert np.all(np.abs(A - Ar) <= 1e-5 * np.abs(Ar[0])[None, :] + 1e-5 * np.abs(Ar))
+    # the reference closed form + Taylor branch agrees within its own accuracy target
+    Rm = B.b1_iout(problem["knots"], s.astype(np.float64), problem["Ie"])
+    assert np.all(np.abs(BL - Rm[0]) <= 3e-6 * Rm[0][0] + 1e-5 * np.abs(Rm[0]))
+
+
+def _oracle_alpha(oracle_mod, P):
+    from oracle import bhc_oracle as B
+    cf, Ts = P["cf"], P["Ts"]
+    Phi = lambda a: Ts * oracle_mod.forward(cf, a.reshape(GEO[0], GEO[0]))[0].ravel()  # noqa: E731
+    Phit = lambda y: Ts * oracle_mod.adjoint(cf, y.reshape(GEO[2], GEO[1]))[0].ravel()  # noqa: E731
+    pol = lambda k, s, I: B.b1_iout_stable(k, s, I)  # noqa: E731
+    return B.poiss_l_alpha(P["Imea"], P["Ie"], P["alpha"].ravel(), Phi, Phit, pol, P["knots"])
+
+
+def test_poiss_l_alpha(problem, oracle_mod):
+    from imgrecsrc_amd.bhc import poissLAlpha
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    f, g, hw = poissLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"],
+                           want_hess=True)
+    f64, g64, hw64 = _oracle_alpha(oracle_mod, P)
+    ef, eg = abs(f.item() - f64) / abs(f64), rel_l2(g.cpu().numpy(), g64)
+    print(f"poissLAlpha: f rel {ef:.2e}, grad rel L2 {eg:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5
+    assert rel_l2(hw.cpu().numpy(), hw64) <= 1e-4
+
+
+def test_poiss_l_i(problem, oracle_mod):
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import poissLI
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    f, g, H = poissLI(P["Imea_t"], P["Ie"], proj, P["Ts"], P["knots"], alpha=P["alpha_t"])
+    raw, _ = oracle_mod.forward(P["cf"], P["alpha"])
+    A, _, _ = B.b1_iout_stable(P["knots"], P["Ts"] * raw.ravel(), None)
+    f64, g64, H64 = B.poiss_l_i(P["Imea"], A, P["Ie"])
+    ef = abs(f.item() - f64) / abs(f64)
+    eg, eh = rel_l2(g.cpu().numpy(), g64), rel_l2(H.cpu().numpy(), H64)
+    print(f"poissLI: f rel {ef:.2e}, g rel {eg:.2e}, H rel {eh:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5 and eh <= 1e-5
+    Hn = H.cpu().numpy()
+    assert np.array_equal(Hn, Hn.T)
+    # g . Ie = sum(Ir - Imea)
+    Ir = A @ P["Ie"]
+    assert g.cpu().numpy() @ P["Ie"] == pytest.approx(np.sum(Ir - P["Imea"]), rel=1e-5, abs=1e-6)
+
+
+def test_gradients_finite_differences(problem):
+    """Central differences (h = 1e-3) of the device NLL: along alpha directions against
+    the device alpha-gradient, and per Ie coordinate against the device Ie-gradient."""
+    import torch
+    from imgrecsrc_amd.bhc import poissLAlpha, poissLI
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    f, g = poissLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"])
+    rng = np.random.default_rng(7)
+    h = 1e-3
+    for _ in range(3):
+        v = torch.from_numpy(rng.standard_normal(GEO[0] ** 2).astype(np.float32)).cuda()
+        fp = poissLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"] + h * v, proj, P["Ts"], P["knots"],
+                         want_grad=False).item()
+        fm = poissLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"] - h * v, proj, P["Ts"], P["knots"],
+                         want_grad=False).item()
+        fd, gv = (fp - fm) / (2 * h), torch.dot(g.double(), v.double()).item()
+        assert abs(fd - gv) <= 2e-3 * abs(gv), (fd, gv)
+    _, gi, _ = poissLI(P["Imea_t"], P["Ie"], proj, P["Ts"], P["knots"], alpha=P["alpha_t"])
+    gi = gi.cpu().numpy()
+
+    def fie(e):
+        return poissLI(P["Imea_t"], P["Ie"] + e, proj, P["Ts"], P["knots"], alpha=P["alpha_t"])[0].item()
+
+    for j in range(0, E, 3):
+        e = np.zeros(E)
+        e[j] = h
+        d1 = (fie(e) - fie(-e)) / (2 * h)
+        d2 = (fie(e / 2) - fie(-e / 2)) / h
+        rich = (4 * d2 - d1) / 3  # Richardson: O(h^2) truncation -> O(h^4)
+        assert abs(rich - gi[j]) <= 1e-5 * np.abs(gi).max(), (j, rich, gi[j])
+
+
+def test_autograd_binding(problem):
+    import torch
+    from imgrecsrc_amd.autograd import backproject, poisson_nll, project
+    from imgrecsrc_amd.bhc import poissLAlpha, poissLI
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    a = P["alpha_t"].clone().requires_grad_(True)
+    ie = torch.tensor(P["Ie"], dtype=torch.float64, requires_grad=True)
+    f = poisson_nll(a, ie, proj, P["Imea_t"], P["knots"], P["Ts"])
+    f.backward()
+    f_ref, g_ref = poissLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"])
+    _, gi_ref, _ = poissLI(P["Imea_t"], P["Ie"], proj, P["Ts"], P["knots"], alpha=P["alpha_t"])
+    assert f.item() == f_ref.item()
+    assert torch.equal(a.grad, g_ref)
+    assert torch.allclose(ie.grad, gi_ref.cpu(), rtol=1e-12, atol=0)
+    # projection autograd: d<A x, y>/dx = A^T y
+    x = torch.randn(GEO[0] ** 2, device="cuda", requires_grad=True)
+    y = torch.randn(GEO[1] * GEO[2], device="cuda")
+    (project(x, proj) * y).sum().backward()
+    assert torch.equal(x.grad, proj.adjoint(y))
+    s = torch.randn(GEO[1] * GEO[2], device="cuda", requires_grad=True)
+    (backproject(s, proj) * x.detach()).sum().backward()
+    assert torch.equal(s.grad, proj.forward(x.detach()))
+
+
+def test_zero_counts(problem):
+    """Imea = 0 rays contribute BL to the NLL (limit of Imea log(BL/Imea)) and stay finite."""
+    from imgrecsrc_amd.bhc import poissLAlpha
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    im = P["Imea_t"].clone()
+    im[::5] = 0
+    f, g = poissLAlpha(im, P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"])
+    assert np.isfinite(f.item()) and bool(g.isfinite().all())

@@ -1,0 +1,191 @@
+"""GPU parity of the projector pair (libctprj through ctypes) against the reference's
+own cpuPrj (golden fixtures) and the FP64 oracle.
+
+Bars (FP32 projection): forward and adjoint of smooth data <= 1e-5 relative L2 vs the
+reference, with the error vs FP64 printed per case; adjoint of Gaussian noise <= 1e-5
+relative to |A^T||y|; FBP <= 1e-4; <Ax,y> = <x,A^T y> <= 1e-6 relative; A and A^T
+bitwise transposes of each other."""
+import numpy as np
+import pytest
+
+from conftest import golden_cases, load_golden, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    return torch.device("cuda", 0)
+
+
+def _proj(geo):
+    from imgrecsrc_amd.projector import Projector
+    return Projector([geo])
+
+
+def _run(P, x, op):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x, np.float32).ravel()).cuda()
+    out = getattr(P, op)(t)
+    torch.cuda.synchronize()
+    return out.cpu().numpy()
+
+
+@pytest.mark.parametrize("case", golden_cases())
+def test_golden_parity(torch_dev, oracle_mod, case):
+    geo, z = load_golden(case)
+    n, W, npj = geo[0], geo[1], geo[2]
+    P = _proj(geo)
+    cf = oracle_mod.conf(*geo)
+    s = _run(P, z["fwd_in"], "forward").reshape(npj, W)
+    s64, _ = oracle_mod.forward(cf, z["fwd_in"])
+    e_ref, e_64 = rel_l2(s, z["fwd_out"]), rel_l2(s, s64)
+    print(f"{case}: fwd vs reference {e_ref:.2e}, vs FP64 {e_64:.2e}")
+    assert e_ref <= 1e-5 and e_64 <= 1e-5
+    b = _run(P, z["bwd_in"], "adjoint").reshape(n, n)
+    b64, _ = oracle_mod.adjoint(cf, z["bwd_in"])
+    e_ref, e_64 = rel_l2(b, z["bwd_out"]), rel_l2(b, b64)
+    print(f"{case}: adj vs reference {e_ref:.2e}, vs FP64 {e_64:.2e}")
+    assert e_ref <= 1e-5 and e_64 <= 1e-5
+    if "bwdg_in" in z:
+        bg = _run(P, z["bwdg_in"], "adjoint").reshape(n, n)
+        bg64, _ = oracle_mod.adjoint(cf, z["bwdg_in"])
+        mag, _ = oracle_mod.adjoint(cf, np.abs(z["bwdg_in"]))
+        e_ref = np.linalg.norm(bg - z["bwdg_out"]) / np.linalg.norm(mag)
+        e_64 = np.linalg.norm(bg - bg64) / np.linalg.norm(mag)
+        print(f"{case}: adj(noise) vs reference {e_ref:.2e}, vs FP64 {e_64:.2e} (of |A^T||y|)")
+        assert e_ref <= 1e-5 and e_64 <= 2e-6
+    if "fbp_in" in z:
+        f = _run(P, z["fbp_in"], "fbp").reshape(n, n)
+        f64 = oracle_mod.fbp(cf, z["fbp_in"])
+        e_ref, e_64 = rel_l2(f, z["fbp_out"]), rel_l2(f, f64)
+        print(f"{case}: FBP vs reference {e_ref:.2e}, vs FP64 {e_64:.2e}")
+        assert e_ref <= 1e-4 and e_64 <= 1e-5
+    # reference quirks
+    if W % 2 == 0:
+        assert np.all(s[:, 0] == 0)
+    if n % 2 == 0:
+        assert np.all(b[0, :] == 0) and np.all(b[:, 0] == 0)
+
+
+TINY = [
+    (8, 13, 12, 24, 1.0, 1.0, 0.0),
+    (7, 10, 9, 18, 1.0, 1.0, 0.0),
+    (6, 9, 10, 20, 1.3, 0.7, 0.0),
+    (8, 12, 16, 16, 1.0, 1.0, 40.0),
+    (9, 15, 11, 22, 0.8, 1.0, 30.0),
+    (11, 16, 13, 13, 1.0, 0.9, 25.0),
+]
+
+
+@pytest.mark.parametrize("geo", TINY)
+def test_forward_adjoint_bitwise_transposes(torch_dev, oracle_mod, geo):
+    """Columns of A from forward(e_p) and rows from adjoint(e_r): every weight is one
+    term, so the two must agree bit for bit; and both within FP32 of the FP64 A."""
+    import torch
+    n, W, npj = geo[0], geo[1], geo[2]
+    P = _proj(geo)
+    eye_img = torch.eye(n * n, device="cuda")
+    eye_sino = torch.eye(npj * W, device="cuda")
+    Af = torch.stack([P.forward(eye_img[p].contiguous()) for p in range(n * n)], 1)
+    At = torch.stack([P.adjoint(eye_sino[r].contiguous()) for r in range(npj * W)], 0)
+    torch.cuda.synchronize()
+    Af, At = Af.cpu().numpy(), At.cpu().numpy()
+    assert np.array_equal(Af, At)
+    A64 = oracle_mod.dense(oracle_mod.conf(*geo, support_mode=1))
+    assert np.abs(Af - A64).max() <= 2e-6 * max(1.0, np.abs(A64).max())
+
+
+@pytest.mark.parametrize("geo", [
+    (256, 367, 180, 360, 1.0, 1.0, 0.0),          # 256^2 parallel workload
+    (1024, 1024, 360, 360, 1.0, 1.0, 4096.0),     # 1024^2 fan workload
+    (512, 729, 360, 360, 1.0, 1.0, 2000.0),       # 512^2 fan (beam hardening) workload
+    (511, 730, 200, 720, 1.0, 1.0, 0.0),          # odd image, even detector, partial arc
+])
+def test_adjoint_identity(torch_dev, geo):
+    import torch
+    from imgrecsrc_amd.phantom import gaussian
+    n, W, npj = geo[0], geo[1], geo[2]
+    P = _proj(geo)
+    x = torch.from_numpy(gaussian((n, n), 21)).cuda().reshape(-1)
+    y = torch.from_numpy(gaussian((npj, W), 22)).cuda().reshape(-1)
+    Ax = P.forward(x)
+    Aty = P.adjoint(y)
+    lhs = torch.dot(Ax.double(), y.double()).item()
+    rhs = torch.dot(x.double(), Aty.double()).item()
+    rel = abs(lhs - rhs) / abs(lhs)
+    print(f"{geo}: <Ax,y>={lhs:.9e} <x,A^T y>={rhs:.9e} rel={rel:.2e}")
+    assert rel <= 1e-6
+
+
+def test_batched_equals_single(torch_dev):
+    """A ragged, mixed parallel/fan plan gives every slice bit-identical results to a
+    one-slice plan of that slice."""
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom, gaussian
+    from imgrecsrc_amd.projector import Projector
+    geoms = [(128, 185, 90, 180, 1.0, 1.0, 0.0), (96, 140, 101, 101, 1.0, 1.0, 384.0),
+             (65, 96, 30, 120, 1.0, 1.0, 0.0), (200, 287, 77, 77, 1.0, 1.0, 800.0)]
+    B = Projector(geoms)
+    imgs = [torch.from_numpy(ellipse_phantom(g[0], 40 + i)).cuda() for i, g in enumerate(geoms)]
+    sinos = [torch.from_numpy(gaussian((g[2], g[1]), 50 + i)).cuda() for i, g in enumerate(geoms)]
+    sb = B.unpack_sinos(B.forward(B.pack_images(imgs)))
+    ib = B.unpack_images(B.adjoint(B.pack_sinos(sinos)))
+    for i, g in enumerate(geoms):
+        S = Projector([g])
+        assert torch.equal(sb[i].reshape(-1), S.forward(imgs[i].reshape(-1)))
+        assert torch.equal(ib[i].reshape(-1), S.adjoint(sinos[i].reshape(-1)))
+
+
+def test_nnz_matches_oracle(torch_dev, oracle_mod):
+    """Device count of non-zero FP32 weights vs the FP64 count: overlaps of a pixel corner
+    with a strip edge below ~1e-9 px^2 round to exactly 0 in FP32, so the FP32 count is
+    slightly lower (by < 1e-3)."""
+    for geo in [(64, 95, 36, 72, 1.0, 1.0, 0.0), (64, 95, 36, 72, 1.0, 1.0, 300.0)]:
+        P = _proj(geo)
+        g = P.nnz()
+        o = oracle_mod.nnz(oracle_mod.conf(*geo, support_mode=1))
+        assert o - 1e-3 * o <= g <= o, (g, o)
+
+
+def test_mass_conservation_parallel(torch_dev):
+    """Parallel beam, unit pitch: the strips of one view tile the plane, so every view
+    of a phantom inside the detector's reach sums to the image mass."""
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom
+    geo = (256, 367, 180, 360, 1.0, 1.0, 0.0)
+    P = _proj(geo)
+    img = ellipse_phantom(256, 1)
+    s = _run(P, img, "forward").reshape(180, 367).astype(np.float64)
+    # even n: row 0 and column 0 are outside the pixel grid the projector reads
+    assert np.allclose(s.sum(1), img[1:, 1:].sum(dtype=np.float64), rtol=2e-6)
+
+
+def test_plan_errors(torch_dev):
+    from imgrecsrc_amd._lib import CtprjError
+    from imgrecsrc_amd.projector import Projector
+    with pytest.raises(CtprjError, match="inside the image"):
+        Projector([(256, 367, 180, 360, 1.0, 1.0, 100.0)])
+    P = _proj((16, 23, 8, 16, 1.0, 1.0, 0.0))
+    import torch
+    with pytest.raises(ValueError):
+        P.forward(torch.zeros(15, device="cuda"))
+    with pytest.raises(TypeError):
+        P.forward(torch.zeros(256, device="cuda", dtype=torch.float64))
+
+
+def test_autotune_roundtrip(torch_dev, tmp_path):
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom
+    geo = (128, 185, 90, 180, 1.0, 1.0, 0.0)
+    P = _proj(geo)
+    x = torch.from_numpy(ellipse_phantom(128, 3)).cuda().reshape(-1)
+    base = P.forward(x).clone()
+    cache = str(tmp_path / "tune.txt")
+    t = P.autotune(reps=2, cache_path=cache)
+    assert t["fwd_block"] in (64, 128, 256) and t["adj_tile"] in (8, 16)
+    assert torch.equal(P.forward(x), base)   # shapes change scheduling, not results
+    Q = _proj(geo)
+    assert Q.autotune(reps=2, cache_path=cache) == t   # served from the cache file
