@@ -49,7 +49,7 @@ class Geom(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("fwd_block", ctypes.c_int), ("adj_tile", ctypes.c_int),
-                ("flags", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("adj_seg", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/ctprj.h
@@ -65,6 +65,7 @@ SIGNATURES = [
     ("ctprj_plan_destroy", _i, [_vp]),
     ("ctprj_plan_layout", _i, [_vp, _llp, _llp, _llp, _llp]),
     ("ctprj_plan_nnz", _i, [_vp, _llp, _vp]),
+    ("ctprj_plan_modes", _i, [_vp, ctypes.POINTER(ctypes.c_int)]),
     ("ctprj_plan_get_tune", _i, [_vp, ctypes.POINTER(Tune)]),
     ("ctprj_plan_set_tune", _i, [_vp, ctypes.POINTER(Tune)]),
     ("ctprj_plan_autotune", _i, [_vp, _i, ctypes.c_char_p, _vp]),

@@ -127,23 +127,34 @@ static void free_dev(void* p)
     if (p) (void)hipFree(p);
 }
 
+template <class T>
+static void free_ptr(T*& p)
+{
+    free_dev((void*)p);
+    p = nullptr;
+}
+
 Plan::~Plan()
 {
-    free_dev(d_desc);
-    free_dev(d_views);
-    free_dev(d_rays);
-    free_dev(d_fwd_items);
-    free_dev(d_adj_items);
-    free_dev(d_tr_items);
-    free_dev(d_fbp_items);
-    free_dev(d_fbp_kern);
-    free_dev(d_fbp_kern_off);
-    free_dev(d_fbp_prew);
-    free_dev(d_fbp_prew_off);
-    free_dev(d_imgT);
-    free_dev(d_sino_ws);
-    free_dev(d_red);
-    free_dev(d_red_out);
+    free_ptr(d_desc);
+    free_ptr(d_views);
+    free_ptr(d_rays);
+    free_ptr(d_fwd_items);
+    free_ptr(d_adj_items);
+    free_ptr(d_symf_items);
+    free_ptr(d_syma_items);
+    free_ptr(d_tr_items);
+    free_ptr(d_fbp_items);
+    free_ptr(d_border_ids);
+    free_ptr(d_zero_row);
+    free_ptr(d_fbp_kern);
+    free_ptr(d_fbp_kern_off);
+    free_ptr(d_fbp_prew);
+    free_ptr(d_fbp_prew_off);
+    free_ptr(d_imgT);
+    free_ptr(d_sino_ws);
+    free_ptr(d_red);
+    free_ptr(d_red_out);
 }
 
 template <class T>
@@ -185,73 +196,47 @@ static std::vector<Item> xcd_order(const std::vector<Item>& in)
     return out;
 }
 
-int Plan::build_fwd_items()
+// Canonical views of the 8-fold symmetric decomposition: a0 in [0, Q/2] whose images
+// (g a0, folded for a half-turn parallel scan) include a computed view.  Returns the
+// share of useful view evaluations (1 for a full or a half-turn scan).
+static double symmetric_views(const ctprj_geom& g, bool fold, std::vector<int>* canon)
 {
-    std::vector<FwdItem> items;
-    const int B = tune.fwd_block;
-    for (int s = 0; s < nslices; ++s) {
-        const ctprj_geom& g = geoms[s];
-        for (int a = 0; a < g.np; ++a)
-            for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
+    const int Q = g.prjFull / 4, P = 4 * Q;
+    long long slots = 0;
+    for (int a0 = 0; 2 * a0 <= Q; ++a0) {
+        const bool mirror = (a0 == 0) || (2 * a0 == Q);
+        bool any = false;
+        for (int e = 0; e < 8; ++e) {
+            if (mirror && e >= 4) continue;
+            int v = ((e >> 2) ? 2 * Q - a0 : a0) + (e & 3) * Q;
+            v %= P;
+            if (fold && v >= 2 * Q) v -= 2 * Q;
+            if (v < g.np) any = true;
+        }
+        if (!any) continue;
+        canon->push_back(a0);
+        slots += fold ? (mirror ? 2 : 4) : (mirror ? 4 : 8);
     }
-    items = xcd_order(items);
-    free_dev(d_fwd_items);
-    d_fwd_items = nullptr;
-    n_fwd_items = (int)items.size();
-    return upload(&d_fwd_items, items, "forward work list");
+    return slots > 0 ? (double)g.np / (double)slots : 0.0;
 }
 
-int Plan::build_adj_items()
+int Plan::build_tables()
 {
-    std::vector<AdjItem> items;
-    const int T = tune.adj_tile;
-    for (int s = 0; s < nslices; ++s) {
-        const int n = geoms[s].n;
-        for (int r = 0; r < n; r += T)
-            for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
-    }
-    items = xcd_order(items);
-    free_dev(d_adj_items);
-    d_adj_items = nullptr;
-    n_adj_items = (int)items.size();
-    return upload(&d_adj_items, items, "adjoint work list");
-}
-
-}  // namespace ctprj
-
-using namespace ctprj;
-
-struct ctprj_plan : public Plan {};
-
-extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_plan** out)
-{
-    if (!out) return fail(CTPRJ_EINVAL, "plan output pointer is NULL");
-    *out = nullptr;
-    if (!geoms || nslices < 1) return fail(CTPRJ_EINVAL, "need at least one slice");
-    for (int i = 0; i < nslices; ++i) {
-        int rc = validate_geom(geoms[i], i);
-        if (rc) return rc;
-    }
-    int dev = 0;
-    int rc = check_device(&dev);
-    if (rc) return rc;
-
-    ctprj_plan* p = new (std::nothrow) ctprj_plan();
-    if (!p) return fail(CTPRJ_ENOMEM, "host allocation");
-    p->device = dev;
-    p->nslices = nslices;
-    p->geoms.assign(geoms, geoms + nslices);
-
+    free_ptr(d_desc);
+    free_ptr(d_views);
+    free_ptr(d_rays);
+    free_ptr(d_border_ids);
     std::vector<SliceDesc> desc(nslices);
     std::vector<float4> views, bins;
     std::vector<long long> bin_off(nslices), ray_prefix(nslices + 1, 0);
-    std::vector<long long> kern_off(nslices), prew_off(nslices);
-    std::vector<float> kern, prew;
-    long long img = 0, sino = 0, nview = 0, nray = 0;
-    int max_n = 0, max_W = 0;
+    std::vector<int> border;
+    long long nview = 0, nray = 0;
+    slice_sym.assign(nslices, 0);
+    n_sym = 0;
     for (int s = 0; s < nslices; ++s) {
         const ctprj_geom& g = geoms[s];
         SliceDesc& D = desc[s];
+        D = SliceDesc{};
         D.n = g.n;
         D.W = g.prjWidth;
         D.np = g.np;
@@ -268,20 +253,38 @@ extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_pla
         D.hbd = 0.5f * (g.dSize * g.effectiveRate);
         D.er = g.effectiveRate;
         D.fbp_scale = kRefPi / (float)g.np / g.dSize / g.dSize;
-        D.img_off = img;
-        D.sino_off = sino;
+        D.img_off = img_offsets[s];
+        D.sino_off = sino_offsets[s];
         D.view_off = nview;
         D.ray_off = nray;
-        p->img_offsets.push_back(img);
-        p->sino_offsets.push_back(sino);
-        img += (long long)g.n * g.n;
-        sino += (long long)g.np * g.prjWidth;
-        for (int a = 0; a < g.np; ++a) {
-            float c, si;
-            view_trig(a, g.prjFull, &c, &si);
-            views.push_back(make_float4(c, si, 0.0f, 0.0f));
+        std::vector<int> canon;
+        bool sym = (tune.flags & 1) && g.prjFull % 4 == 0 && g.prjFull >= 4 && g.np <= g.prjFull;
+        const bool fold = sym && !D.fan && g.np <= g.prjFull / 2;
+        if (sym && symmetric_views(g, fold, &canon) < 0.5) sym = false;
+        if (sym) {
+            D.sym = 1;
+            D.fold = fold;
+            D.Q = g.prjFull / 4;
+            D.ncanon = (int)canon.size();
+            for (int a0 : canon) {
+                float c, si;
+                view_trig(a0, g.prjFull, &c, &si);
+                float tag;
+                std::memcpy(&tag, &a0, sizeof(tag));
+                views.push_back(make_float4(c, si, tag, 0.0f));
+            }
+            slice_sym[s] = 1;
+            ++n_sym;
+            if (g.n % 2 == 0) border.push_back(s);
+        } else {
+            for (int a = 0; a < g.np; ++a) {
+                float c, si;
+                view_trig(a, g.prjFull, &c, &si);
+                views.push_back(make_float4(c, si, 0.0f, 0.0f));
+            }
         }
-        nview += g.np;
+        const int ntab = sym ? D.ncanon : g.np;
+        nview += ntab;
         bin_off[s] = (long long)bins.size();
         if (D.fan) {
             for (int b = 0; b < g.prjWidth; ++b) {
@@ -289,13 +292,135 @@ extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_pla
                 fan_bin(b, g, q);
                 bins.push_back(make_float4(q[0], q[1], q[2], q[3]));
             }
-            nray += (long long)g.np * g.prjWidth;
+            nray += (long long)ntab * g.prjWidth;
         } else {
-            nray += g.np;
+            nray += ntab;
         }
         ray_prefix[s + 1] = nray;
+    }
+    n_border = (int)border.size();
+    float4* d_bins = nullptr;
+    long long *d_bin_off = nullptr, *d_ray_prefix = nullptr;
+    int rc;
+    if ((rc = upload(&d_desc, desc, "slice table")) || (rc = upload(&d_views, views, "view table")) ||
+        (rc = upload(&d_bins, bins, "bin table")) || (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
+        (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
+        (rc = upload(&d_border_ids, border, "border list")) ||
+        (rc = dev_alloc((void**)&d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
+        free_dev(d_bins);
+        free_dev(d_bin_off);
+        free_dev(d_ray_prefix);
+        return rc;
+    }
+    launch_build_rays(d_desc, nslices, d_views, d_bins, d_bin_off, d_ray_prefix, d_rays, nray, 0);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    free_dev(d_bins);
+    free_dev(d_bin_off);
+    free_dev(d_ray_prefix);
+    if (e != hipSuccess) return hip_fail(e, "building ray table");
+    if ((rc = build_fwd_items())) return rc;
+    return build_adj_items();
+}
+
+int Plan::build_fwd_items()
+{
+    std::vector<FwdItem> items, sym;
+    const int B = tune.fwd_block;
+    for (int s = 0; s < nslices; ++s) {
+        const ctprj_geom& g = geoms[s];
+        if (slice_sym[s]) {
+            const int pC = g.prjWidth / 2, bhi = pC + (g.prjWidth - 1) / 2;
+            const bool fold = g.d <= 0.0f && g.np <= g.prjFull / 2;
+            const int bmax = fold ? pC : bhi;
+            std::vector<int> canon;
+            symmetric_views(g, fold, &canon);
+            for (int i = 0; i < (int)canon.size(); ++i)
+                for (int b0 = 0; b0 <= bmax; b0 += 64) sym.push_back(FwdItem{s, i, b0, 0});
+        } else {
+            for (int a = 0; a < g.np; ++a)
+                for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
+        }
+    }
+    items = xcd_order(items);
+    sym = xcd_order(sym);
+    free_ptr(d_fwd_items);
+    free_ptr(d_symf_items);
+    n_fwd_items = (int)items.size();
+    n_symf_items = (int)sym.size();
+    int rc = upload(&d_fwd_items, items, "forward work list");
+    if (rc) return rc;
+    return upload(&d_symf_items, sym, "symmetric forward work list");
+}
+
+int Plan::build_adj_items()
+{
+    std::vector<AdjItem> items, sym;
+    const int T = tune.adj_tile;
+    for (int s = 0; s < nslices; ++s) {
+        const int n = geoms[s].n;
+        if (slice_sym[s]) {
+            const int H = (n - 1) / 2, nt = (H + 1 + 7) / 8;
+            for (int ty = 0; ty < nt; ++ty)
+                for (int tx = ty; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
+        } else {
+            for (int r = 0; r < n; r += T)
+                for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
+        }
+    }
+    items = xcd_order(items);
+    sym = xcd_order(sym);
+    free_ptr(d_adj_items);
+    free_ptr(d_syma_items);
+    n_adj_items = (int)items.size();
+    n_syma_items = (int)sym.size();
+    int rc = upload(&d_adj_items, items, "adjoint work list");
+    if (rc) return rc;
+    return upload(&d_syma_items, sym, "symmetric adjoint work list");
+}
+
+static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out);
+
+}  // namespace ctprj
+
+using namespace ctprj;
+
+struct ctprj_plan : public Plan {};
+
+namespace ctprj {
+
+static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
+{
+    *out = nullptr;
+    if (!geoms || nslices < 1) return fail(CTPRJ_EINVAL, "need at least one slice");
+    for (int i = 0; i < nslices; ++i) {
+        int rc = validate_geom(geoms[i], i);
+        if (rc) return rc;
+    }
+    int dev = 0;
+    int rc = check_device(&dev);
+    if (rc) return rc;
+    ctprj_plan* p = new (std::nothrow) ctprj_plan();
+    if (!p) return fail(CTPRJ_ENOMEM, "host allocation");
+    p->device = dev;
+    p->nslices = nslices;
+    p->geoms.assign(geoms, geoms + nslices);
+    p->tune.fwd_block = 256;
+    p->tune.adj_tile = 16;
+    p->tune.flags = flags;
+    p->tune.adj_seg = 4;
+    std::vector<long long> kern_off(nslices), prew_off(nslices);
+    std::vector<float> kern, prew;
+    long long img = 0, sino = 0;
+    int max_W = 0;
+    for (int s = 0; s < nslices; ++s) {
+        const ctprj_geom& g = geoms[s];
+        p->img_offsets.push_back(img);
+        p->sino_offsets.push_back(sino);
+        img += (long long)g.n * g.n;
+        sino += (long long)g.np * g.prjWidth;
         // FBP: filter taps (2W) and fan pre-weights (W)
-        const int W = g.prjWidth;
+        const int W = g.prjWidth, pC = W / 2;
         const long long L = 2LL * W;
         kern_off[s] = (long long)kern.size();
         auto ramlak = [&](long long k) -> double {
@@ -310,22 +435,17 @@ extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_pla
         prew_off[s] = (long long)prew.size();
         for (int b = 0; b < W; ++b) {
             double w = 1.0;
-            if (D.fan) {
-                const double t = (double)(b - D.pC) * g.dSize;
+            if (g.d > 0.0f) {
+                const double t = (double)(b - pC) * g.dSize;
                 w = (double)g.d / std::sqrt((double)g.d * g.d + t * t);
             }
             prew.push_back((float)w);
         }
-        max_n = std::max(max_n, g.n);
         max_W = std::max(max_W, W);
     }
     p->img_elems = img;
     p->sino_elems = sino;
     p->max_W = max_W;
-    p->tune.fwd_block = 64;
-    p->tune.adj_tile = 16;
-    p->tune.flags = 0;
-
     std::vector<AdjItem> tr;
     for (int s = 0; s < nslices; ++s)
         for (int r = 0; r < geoms[s].n; r += 32)
@@ -335,44 +455,33 @@ extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_pla
         for (int a = 0; a < geoms[s].np; ++a) fbpi.push_back(make_int2(s, a));
     p->n_tr_items = (int)tr.size();
     p->n_fbp_items = (int)fbpi.size();
-
-    float4* d_bins = nullptr;
-    long long *d_bin_off = nullptr, *d_ray_prefix = nullptr;
-    if ((rc = upload(&p->d_desc, desc, "slice table")) ||
-        (rc = upload(&p->d_views, views, "view table")) ||
-        (rc = upload(&d_bins, bins, "bin table")) ||
-        (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
-        (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
-        (rc = upload(&p->d_tr_items, tr, "transpose work list")) ||
+    std::vector<float> zeros((size_t)max_W + 1, 0.0f);
+    if ((rc = upload(&p->d_tr_items, tr, "transpose work list")) ||
         (rc = upload(&p->d_fbp_items, fbpi, "fbp work list")) ||
         (rc = upload(&p->d_fbp_kern, kern, "fbp taps")) ||
         (rc = upload(&p->d_fbp_kern_off, kern_off, "fbp tap offsets")) ||
         (rc = upload(&p->d_fbp_prew, prew, "fbp preweights")) ||
         (rc = upload(&p->d_fbp_prew_off, prew_off, "fbp preweight offsets")) ||
-        (rc = dev_alloc((void**)&p->d_rays, sizeof(Ray) * (size_t)nray, "ray table")) ||
+        (rc = upload(&p->d_zero_row, zeros, "zero row")) ||
         (rc = dev_alloc((void**)&p->d_imgT, sizeof(float) * (size_t)img, "transposed image")) ||
         (rc = dev_alloc((void**)&p->d_sino_ws, sizeof(float) * (size_t)sino, "sinogram workspace")) ||
-        (rc = p->build_fwd_items()) || (rc = p->build_adj_items())) {
-        free_dev(d_bins);
-        free_dev(d_bin_off);
-        free_dev(d_ray_prefix);
-        delete p;
-        return rc;
-    }
-    launch_build_rays(p->d_desc, nslices, p->d_views, d_bins, d_bin_off, d_ray_prefix, p->d_rays,
-                      nray, 0);
-    hipError_t e = hipGetLastError();
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    free_dev(d_bins);
-    free_dev(d_bin_off);
-    free_dev(d_ray_prefix);
-    if (e != hipSuccess) {
-        rc = hip_fail(e, "building ray table");
+        (rc = p->build_tables())) {
         delete p;
         return rc;
     }
     *out = p;
     return 0;
+}
+
+}  // namespace ctprj
+
+extern "C" int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_plan** out)
+{
+    if (!out) return fail(CTPRJ_EINVAL, "plan output pointer is NULL");
+    Plan* p = nullptr;
+    int rc = plan_new(geoms, nslices, 1, &p);
+    *out = static_cast<ctprj_plan*>(p);
+    return rc;
 }
 
 extern "C" int ctprj_plan_destroy(ctprj_plan* plan)
@@ -401,17 +510,28 @@ namespace ctprj {
 
 int Plan::launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count)
 {
+    if (count && n_sym) return fail(CTPRJ_EINVAL, "internal: count needs the general kernels");
     launch_transpose(n_tr_items, d_desc, d_tr_items, d_img, d_imgT, st);
-    launch_forward_kernel(count, tune.fwd_block, n_fwd_items, d_desc, d_fwd_items, d_views,
-                          d_rays, d_img, d_imgT, d_sino, st);
+    if (n_fwd_items)
+        launch_forward_kernel(count, tune.fwd_block, n_fwd_items, d_desc, d_fwd_items, d_views,
+                              d_rays, d_img, d_imgT, d_sino, st);
+    if (n_symf_items)
+        launch_forward_sym(tune.fwd_block / 64, n_symf_items, d_desc, d_symf_items, d_views,
+                           d_rays, d_img, d_imgT, d_sino, st);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
 }
 
 int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp)
 {
-    launch_adjoint_kernel(tune.adj_tile, fbp, n_adj_items, d_desc, d_adj_items, d_views, d_rays,
-                          d_sino, d_img, st);
+    if (n_adj_items)
+        launch_adjoint_kernel(tune.adj_tile, fbp, n_adj_items, d_desc, d_adj_items, d_views,
+                              d_rays, d_sino, d_img, st);
+    if (n_syma_items) {
+        launch_adjoint_sym(fbp, tune.adj_seg, n_syma_items, d_desc, d_syma_items, d_views, d_rays,
+                           d_sino, d_zero_row, d_img, st);
+        launch_zero_border(n_border, d_desc, d_border_ids, d_img, st);
+    }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "adjoint launch");
 }
@@ -447,28 +567,45 @@ extern "C" int ctprj_fbp(ctprj_plan* p, const float* d_sino, float* d_img, void*
     return p->launch_fbp(d_sino, d_img, (hipStream_t)stream);
 }
 
+// nnz(A): the general kernels count the non-zero FP32 weights of every ray (a plan
+// without the symmetric kernels is made for the count when needed).
 extern "C" int ctprj_plan_nnz(ctprj_plan* p, long long* nnz, void* stream)
 {
     if (!p || !nnz) return fail(CTPRJ_EINVAL, "ctprj_plan_nnz: NULL argument");
     hipStream_t st = (hipStream_t)stream;
+    Plan* q = p;
+    Plan* tmp = nullptr;
+    if (p->n_sym) {
+        int rc = plan_new(p->geoms.data(), p->nslices, 0, &tmp);
+        if (rc) return rc;
+        q = tmp;
+    }
     float* d_img = nullptr;
-    int rc = dev_alloc((void**)&d_img, sizeof(float) * (size_t)p->img_elems, "nnz image");
-    if (rc) return rc;
-    hipError_t e = hipMemsetAsync(d_img, 0, sizeof(float) * (size_t)p->img_elems, st);
-    if (e != hipSuccess) { free_dev(d_img); return hip_fail(e, "hipMemsetAsync"); }
-    rc = p->launch_forward(d_img, p->d_sino_ws, st, true);
-    std::vector<float> counts((size_t)p->sino_elems);
+    int rc = dev_alloc((void**)&d_img, sizeof(float) * (size_t)q->img_elems, "nnz image");
+    if (rc) { delete static_cast<ctprj_plan*>(tmp); return rc; }
+    hipError_t e = hipMemsetAsync(d_img, 0, sizeof(float) * (size_t)q->img_elems, st);
+    if (e != hipSuccess) rc = hip_fail(e, "hipMemsetAsync");
+    if (!rc) rc = q->launch_forward(d_img, q->d_sino_ws, st, true);
+    std::vector<float> counts((size_t)q->sino_elems);
     if (!rc) {
-        e = hipMemcpyAsync(counts.data(), p->d_sino_ws, sizeof(float) * counts.size(),
+        e = hipMemcpyAsync(counts.data(), q->d_sino_ws, sizeof(float) * counts.size(),
                            hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = hip_fail(e, "nnz readback");
     }
     free_dev(d_img);
+    delete static_cast<ctprj_plan*>(tmp);
     if (rc) return rc;
     long long total = 0;
     for (float c : counts) total += (long long)c;
     *nnz = total;
+    return 0;
+}
+
+extern "C" int ctprj_plan_modes(const ctprj_plan* p, int* modes)
+{
+    if (!p || !modes) return fail(CTPRJ_EINVAL, "NULL argument");
+    for (int s = 0; s < p->nslices; ++s) modes[s] = p->slice_sym[s];
     return 0;
 }
 
@@ -485,9 +622,14 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     if (t->fwd_block != 64 && t->fwd_block != 128 && t->fwd_block != 256)
         return fail(CTPRJ_EINVAL, "fwd_block must be 64, 128 or 256");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
+    if (t->flags & ~1) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
+    if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
+        return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
+    const bool tables = (t->flags != p->tune.flags);
     const bool fwd = t->fwd_block != p->tune.fwd_block, adj = t->adj_tile != p->tune.adj_tile;
     p->tune = *t;
     int rc = 0;
+    if (tables) return p->build_tables();
     if (fwd && (rc = p->build_fwd_items())) return rc;
     if (adj && (rc = p->build_adj_items())) return rc;
     return 0;
@@ -503,6 +645,8 @@ static std::string tune_key(const ctprj_plan* p, const char* arch)
     return os.str();
 }
 
+// Times the forward work-group sizes (general: threads per group; symmetric: waves per
+// ray run) and the adjoint tile edges on this plan's geometry; keeps tune.flags.
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -510,7 +654,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     hipStream_t st = (hipStream_t)stream;
     char arch[64] = "gfx950";
     ctprj_device_arch(arch, sizeof(arch));
-    const std::string key = tune_key(p, arch);
+    const std::string key = tune_key(p, arch) + "|f" + std::to_string(p->tune.flags);
     if (cache_path) {
         std::ifstream f(cache_path);
         std::string line;
@@ -518,7 +662,8 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
             const size_t sp = line.rfind(' ');
             if (sp == std::string::npos || line.substr(0, sp) != key) continue;
             ctprj_tune t{};
-            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d", &t.fwd_block, &t.adj_tile, &t.flags) == 3)
+            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d,%d", &t.fwd_block, &t.adj_tile,
+                            &t.flags, &t.adj_seg) == 4)
                 return ctprj_plan_set_tune(p, &t);
         }
     }
@@ -553,7 +698,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
         }
         if (ms < best_f) { best_f = ms; best.fwd_block = blocks[bi]; }
     }
-    for (int ti = 0; ti < 2 && !rc; ++ti) {
+    for (int ti = 0; ti < 2 && !rc && p->n_adj_items; ++ti) {
         ctprj_tune t = p->tune;
         t.adj_tile = tiles[ti];
         if ((rc = ctprj_plan_set_tune(p, &t))) break;
@@ -569,6 +714,24 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
         }
         if (ms < best_a) { best_a = ms; best.adj_tile = tiles[ti]; }
     }
+    const int segs[4] = {1, 2, 4, 8};
+    float best_s = 1e30f;
+    for (int si = 0; si < 4 && !rc && p->n_syma_items; ++si) {
+        ctprj_tune t = p->tune;
+        t.adj_seg = segs[si];
+        if ((rc = ctprj_plan_set_tune(p, &t))) break;
+        float ms = 0;
+        for (int r = -1; r < reps && !rc; ++r) {
+            (void)hipEventRecord(e0, st);
+            rc = p->launch_adjoint(d_sino, d_img, st, false);
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float m = 0;
+            (void)hipEventElapsedTime(&m, e0, e1);
+            if (r >= 0) ms += m;
+        }
+        if (ms < best_s) { best_s = ms; best.adj_seg = segs[si]; }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     free_dev(d_img);
@@ -577,12 +740,14 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     rc = ctprj_plan_set_tune(p, &best);
     if (!rc && cache_path) {
         std::ofstream f(cache_path, std::ios::app);
-        f << key << " " << best.fwd_block << "," << best.adj_tile << "," << best.flags << "\n";
+        f << key << " " << best.fwd_block << "," << best.adj_tile << "," << best.flags << ","
+          << best.adj_seg << "\n";
     }
     return rc;
 }
 
 // ------------------------------------------------- reference-compatible API ---
+
 
 static std::mutex g_ref_mu;
 static ctprj_plan* g_ref_plan = nullptr;

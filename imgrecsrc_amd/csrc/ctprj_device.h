@@ -16,6 +16,8 @@ constexpr float kSupportMargin = 0.02f;
 // One slice of a plan.  Offsets are element offsets into the packed arrays.
 struct SliceDesc {
     int n, W, np, fan;     // image edge, bins per view, views, 1 = fan beam
+    int sym, fold;         // 8-fold symmetric kernels; parallel beam folded over 180 deg
+    int Q, ncanon;         // prjFull/4; canonical views held in the view / ray tables
     int H;                 // (n-1)/2: pixel coordinates run over [-H, H]
     int half;              // n/2: coordinate -> index offset
     int pC;                // W/2: index of the bin at t = 0
@@ -55,12 +57,15 @@ __device__ __forceinline__ float gcum(float u, float b1, float b2, float s, floa
 
 // Area of overlap of the unit pixel centred `dist` from the ray line with the strip of
 // half width hb (prj/utils.h:12-92 computes the same area by a 20-way case split).
+// Evaluated at |dist| like the reference (the footprint is symmetric), which makes the
+// float result of the pixel reflected through the ray identical bit for bit.
 __device__ __forceinline__ float strip_weight(float dist, float hb, const Ray& r)
 {
+    const float ad = fabsf(dist);
     const float b2 = r.b1 + r.s;
     const float b1x2 = r.b1 + r.b1;
-    const float gp = gcum(hb - dist, r.b1, b2, r.s, b1x2, r.k);
-    const float gm = gcum(-hb - dist, r.b1, b2, r.s, b1x2, r.k);
+    const float gp = gcum(hb - ad, r.b1, b2, r.s, b1x2, r.k);
+    const float gm = gcum(-hb - ad, r.b1, b2, r.s, b1x2, r.k);
     return r.h * (gp - gm);
 }
 
@@ -76,6 +81,49 @@ __device__ __forceinline__ float fan_depth_scale(float x, float y, float cosT, f
                                                  float invd)
 {
     return fmaf(fmaf(x, sinT, -(y * cosT)), invd, 1.0f);
+}
+
+// Candidate range along a row (or column) of the walk: the pixels whose footprint can
+// meet the strip, |dist| < b2 + hb(x, y), solved as two linear inequalities in the
+// inner coordinate u for the walk coordinate v:
+//   u in [min(U(v)*iu, L(v)*il), max(...)],  U(v) = U0 - v*U1,  L(v) = L0 - v*L1.
+struct RowBounds {
+    float U0, U1, L0, L1, iu, il;
+};
+
+__device__ __forceinline__ RowBounds row_bounds(const Ray& r, float cT, float sT, float invd,
+                                                bool rowmode)
+{
+    RowBounds rb;
+    const float b2 = r.b1 + r.s;
+    const float m = kSupportMargin;
+    const float al = r.hbc * sT * invd;  // d(hb)/dx
+    const float be = r.hbc * cT * invd;  // -d(hb)/dy
+    rb.U0 = r.dc + b2 + r.hbc + m;
+    rb.L0 = r.dc - b2 - r.hbc - m;
+    if (rowmode) {  // u = x, v = y
+        rb.U1 = r.sinR + be;
+        rb.L1 = r.sinR - be;
+        rb.iu = 1.0f / (r.cosR - al);
+        rb.il = 1.0f / (r.cosR + al);
+    } else {        // u = y, v = x
+        rb.U1 = r.cosR - al;
+        rb.L1 = r.cosR + al;
+        rb.iu = 1.0f / (r.sinR + be);
+        rb.il = 1.0f / (r.sinR - be);
+    }
+    return rb;
+}
+
+__device__ __forceinline__ void row_range(const RowBounds& rb, float v, int H, int& lo, int& hi)
+{
+    const float ua = fmaf(-v, rb.U1, rb.U0) * rb.iu;
+    const float ub = fmaf(-v, rb.L1, rb.L0) * rb.il;
+    const float fH = (float)H;
+    lo = (int)ceilf(fmaxf(fminf(ua, ub), -fH - 1.0f));
+    hi = (int)floorf(fminf(fmaxf(ua, ub), fH + 1.0f));
+    lo = max(lo, -H);
+    hi = min(hi, H);
 }
 
 // Compensated accumulation of a partial sum (Kahan): keeps the long sums over rows /

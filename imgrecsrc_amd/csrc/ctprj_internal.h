@@ -26,30 +26,37 @@ struct Plan {
     int nslices = 0;
     std::vector<ctprj_geom> geoms;
     std::vector<long long> img_offsets, sino_offsets;
+    std::vector<int> slice_sym;  // per slice: 1 = 8-fold symmetric kernels
     long long img_elems = 0, sino_elems = 0;
     int max_W = 0;
-    ctprj_tune tune{64, 16, 0};
+    ctprj_tune tune{64, 16, 1, 4};
 
     SliceDesc* d_desc = nullptr;
     float4* d_views = nullptr;
     Ray* d_rays = nullptr;
-    FwdItem* d_fwd_items = nullptr;
-    AdjItem* d_adj_items = nullptr;
+    FwdItem* d_fwd_items = nullptr;   // general forward
+    AdjItem* d_adj_items = nullptr;   // general adjoint
+    FwdItem* d_symf_items = nullptr;  // symmetric forward
+    AdjItem* d_syma_items = nullptr;  // symmetric adjoint
     AdjItem* d_tr_items = nullptr;
     int2* d_fbp_items = nullptr;
+    int* d_border_ids = nullptr;      // symmetric slices of even n
+    float* d_zero_row = nullptr;      // max_W zeros: reads of views beyond np
     float* d_fbp_kern = nullptr;
     long long* d_fbp_kern_off = nullptr;
     float* d_fbp_prew = nullptr;
     long long* d_fbp_prew_off = nullptr;
-    int n_fwd_items = 0, n_adj_items = 0, n_tr_items = 0, n_fbp_items = 0;
+    int n_fwd_items = 0, n_adj_items = 0, n_symf_items = 0, n_syma_items = 0;
+    int n_tr_items = 0, n_fbp_items = 0, n_border = 0, n_sym = 0;
 
-    float* d_imgT = nullptr;     // transposed image (forward, steep views)
+    float* d_imgT = nullptr;     // transposed image (forward, steep views / symmetric images)
     float* d_sino_ws = nullptr;  // sinogram workspace (FBP filter output, BHC ray values)
     double* d_red = nullptr;     // reduction partials (beam-hardening)
     double* d_red_out = nullptr; // reduced sums
     long long red_cap = 0;       // doubles held by d_red
 
     ~Plan();
+    int build_tables();          // geometry tables for the current tune.flags
     int build_fwd_items();
     int build_adj_items();
     int launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count);
@@ -74,5 +81,14 @@ void launch_fbp_filter(int nitems, size_t shmem, const SliceDesc* desc, const in
                        const long long* prew_off, const float* sino, float* out,
                        hipStream_t st);
 void launch_scale(float* p, long long n, float s, hipStream_t st);
+// 8-fold symmetric kernels (ctprj_sym.hip)
+void launch_forward_sym(int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
+                        const float4* views, const Ray* rays, const float* img,
+                        const float* imgT, float* sino, hipStream_t st);
+void launch_adjoint_sym(bool fbp, int seg, int nitems, const SliceDesc* desc,
+                        const AdjItem* items, const float4* views, const Ray* rays,
+                        const float* sino, const float* zero_row, float* img, hipStream_t st);
+void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
+                        hipStream_t st);
 
 }  // namespace ctprj

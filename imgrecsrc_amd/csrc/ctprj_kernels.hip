@@ -87,49 +87,6 @@ __global__ __launch_bounds__(256) void k_transpose(const SliceDesc* __restrict__
 
 // ---------------------------------------------------------------------- forward ---
 
-// Candidate range along a row (or column) of the walk: the pixels whose footprint can
-// meet the strip, |dist| < b2 + hb(x, y), solved as two linear inequalities in the
-// inner coordinate u for the walk coordinate v:
-//   u in [min(U(v)*iu, L(v)*il), max(...)],  U(v) = U0 - v*U1,  L(v) = L0 - v*L1.
-struct RowBounds {
-    float U0, U1, L0, L1, iu, il;
-};
-
-__device__ __forceinline__ RowBounds row_bounds(const Ray& r, float cT, float sT, float invd,
-                                                bool rowmode)
-{
-    RowBounds rb;
-    const float b2 = r.b1 + r.s;
-    const float m = kSupportMargin;
-    const float al = r.hbc * sT * invd;  // d(hb)/dx
-    const float be = r.hbc * cT * invd;  // -d(hb)/dy
-    rb.U0 = r.dc + b2 + r.hbc + m;
-    rb.L0 = r.dc - b2 - r.hbc - m;
-    if (rowmode) {  // u = x, v = y
-        rb.U1 = r.sinR + be;
-        rb.L1 = r.sinR - be;
-        rb.iu = 1.0f / (r.cosR - al);
-        rb.il = 1.0f / (r.cosR + al);
-    } else {        // u = y, v = x
-        rb.U1 = r.cosR - al;
-        rb.L1 = r.cosR + al;
-        rb.iu = 1.0f / (r.sinR + be);
-        rb.il = 1.0f / (r.sinR - be);
-    }
-    return rb;
-}
-
-__device__ __forceinline__ void row_range(const RowBounds& rb, float v, int H, int& lo, int& hi)
-{
-    const float ua = fmaf(-v, rb.U1, rb.U0) * rb.iu;
-    const float ub = fmaf(-v, rb.L1, rb.L0) * rb.il;
-    const float fH = (float)H;
-    lo = (int)ceilf(fmaxf(fminf(ua, ub), -fH - 1.0f));
-    hi = (int)floorf(fminf(fmaxf(ua, ub), fH + 1.0f));
-    lo = max(lo, -H);
-    hi = min(hi, H);
-}
-
 template <bool FAN, bool COUNT>
 __device__ __forceinline__ float forward_ray(const SliceDesc& S, const Ray& r, float cT, float sT,
                                              const float* __restrict__ img,

@@ -46,7 +46,7 @@ def _ptr(t: torch.Tensor):
 class Projector:
     """sino = A img, img = A^T sino, FBP -- for every slice of the plan at once."""
 
-    def __init__(self, geoms: Iterable, device=None):
+    def __init__(self, geoms: Iterable, device=None, symmetric: bool = True):
         geoms = [make_geom(g) for g in (geoms if not isinstance(geoms, (dict, Geom)) else [geoms])]
         if not geoms:
             raise ValueError("need at least one slice geometry")
@@ -66,6 +66,8 @@ class Projector:
         self.img_elems, self.sino_elems = ie.value, se.value
         self.img_offsets = list(io)
         self.sino_offsets = list(so)
+        if not symmetric:
+            self.set_tune(flags=0)
 
     # ------------------------------------------------------------------ lifetime
     def close(self):
@@ -145,16 +147,23 @@ class Projector:
         check(lib().ctprj_plan_nnz(self.handle, ctypes.byref(v), _stream_ptr(stream)))
         return v.value
 
+    def modes(self):
+        """Per slice: 1 = 8-fold symmetric kernels, 0 = general kernels."""
+        m = (ctypes.c_int * self.nslices)()
+        check(lib().ctprj_plan_modes(self.handle, m))
+        return list(m)
+
     # ------------------------------------------------------------------ tuning
     def get_tune(self) -> dict:
         t = Tune()
         check(lib().ctprj_plan_get_tune(self.handle, ctypes.byref(t)))
-        return {"fwd_block": t.fwd_block, "adj_tile": t.adj_tile, "flags": t.flags}
+        return {"fwd_block": t.fwd_block, "adj_tile": t.adj_tile, "flags": t.flags,
+                "adj_seg": t.adj_seg}
 
-    def set_tune(self, fwd_block=None, adj_tile=None, flags=None):
+    def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None):
         cur = self.get_tune()
         t = Tune(fwd_block or cur["fwd_block"], adj_tile or cur["adj_tile"],
-                 cur["flags"] if flags is None else flags)
+                 cur["flags"] if flags is None else flags, adj_seg or cur["adj_seg"])
         check(lib().ctprj_plan_set_tune(self.handle, ctypes.byref(t)))
 
     def autotune(self, reps: int = 3, cache_path: str | None = None, stream=None) -> dict:
