@@ -47,6 +47,12 @@ class Geom(ctypes.Structure):
                 self.effectiveRate, self.d)
 
 
+class PnpgInfo(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_double), ("t", ctypes.c_double), ("theta", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("trials", ctypes.c_int), ("rejects", ctypes.c_int),
+                ("restarts", ctypes.c_int), ("last_accepted", ctypes.c_int)]
+
+
 class Tune(ctypes.Structure):
     _fields_ = [("fwd_block", ctypes.c_int), ("adj_tile", ctypes.c_int),
                 ("flags", ctypes.c_int), ("adj_seg", ctypes.c_int)]
@@ -76,6 +82,11 @@ SIGNATURES = [
     ("ctprj_bhc_grad_alpha", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_alpha_rays", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_grad_ie", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
+    ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
+                               ctypes.POINTER(_vp)]),
+    ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),
+    ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),
+    ("ctprj_pnpg_destroy", _i, [_vp]),
     ("ctprj_last_error", ctypes.c_char_p, []),
     ("ctprj_device_arch", _i, [ctypes.c_char_p, _i]),
     ("ctprj_version", ctypes.c_char_p, []),

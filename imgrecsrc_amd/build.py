@@ -18,7 +18,8 @@ LIBDIR = os.path.join(PKG, "lib")
 LIB = os.path.join(LIBDIR, "libctprj.so")
 ROOT = os.path.dirname(PKG)
 
-SOURCES = ["ctprj_kernels.hip", "ctprj_sym.hip", "bhc_kernels.hip", "ctprj_api.hip", "bhc_api.hip"]
+SOURCES = ["ctprj_kernels.hip", "ctprj_sym.hip", "bhc_kernels.hip", "ctprj_api.hip", "bhc_api.hip",
+           "bhc_pnpg.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -1,0 +1,70 @@
+"""On-device PNPG alpha step (npg/PNPG.m:50-167, non-negativity prox) over a projector
+plan and the polychromatic Poisson objective of bhc/poissLAlpha.m.
+
+``PNPG.step(alpha)`` enqueues one line-search trial with no host synchronisation, so a
+run of trials can be captured once in a HIP graph (``torch.cuda.CUDAGraph``) and
+replayed; ``stats()`` reads the device scalars.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import PnpgInfo, check, dbl_array, lib
+from .bhc import _knots_ie, poissLAlpha
+from .projector import Projector, _ptr, _stream_ptr
+
+
+def bb_step(proj: Projector, Imea, Ie, alpha, Ts, knots, rel=1e-5) -> float:
+    """Barzilai-Borwein estimate of the inverse step (npg/Methods.m:100-107):
+    t = |<g1 - g2, d>| / ||d||^2 with d = rel * g1/||g1||, g2 = grad(alpha - d)."""
+    _, g1 = poissLAlpha(Imea, Ie, alpha, proj, Ts, knots)
+    d = rel * g1 / torch.linalg.vector_norm(g1.double()).float()
+    _, g2 = poissLAlpha(Imea, Ie, alpha - d, proj, Ts, knots)
+    num = torch.dot((g1 - g2).double(), d.double()).abs().item()
+    den = torch.dot(d.double(), d.double()).item()
+    t = num / den if den > 0 else 1.0
+    return t if np.isfinite(t) and t > 0 else 1.0
+
+
+class PNPG:
+    def __init__(self, proj: Projector, Imea: torch.Tensor, knots, Ie, Ts: float,
+                 alpha0: torch.Tensor, t0: float | None = None):
+        self.proj = proj
+        self.Imea = Imea.reshape(-1).contiguous()
+        self.knots = np.asarray(knots, np.float64)
+        self.Ie = np.asarray(Ie, np.float64)
+        self.Ts = float(Ts)
+        a0 = alpha0.reshape(-1).contiguous()
+        if t0 is None:
+            t0 = bb_step(proj, self.Imea, self.Ie, a0, self.Ts, self.knots)
+        self.t0 = float(t0)
+        E, kp, self._kh, ip, self._ih = _knots_ie(self.knots, self.Ie)
+        h = ctypes.c_void_p()
+        check(lib().ctprj_pnpg_create(proj.handle, _ptr(self.Imea), kp, E, ip, self.Ts, _ptr(a0),
+                                      ctypes.c_double(self.t0), ctypes.byref(h)))
+        self._h = h
+
+    def step(self, alpha: torch.Tensor, stream=None):
+        check(lib().ctprj_pnpg_step(self._h, _ptr(alpha), _stream_ptr(stream)))
+
+    def stats(self, stream=None) -> dict:
+        info = PnpgInfo()
+        check(lib().ctprj_pnpg_stats(self._h, ctypes.byref(info), _stream_ptr(stream)))
+        return {f: getattr(info, f) for f, _ in PnpgInfo._fields_}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().ctprj_pnpg_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["PNPG", "bb_step"]

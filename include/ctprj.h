@@ -171,6 +171,37 @@ int ctprj_bhc_grad_ie(ctprj_plan* plan, const float* d_alpha, const float* d_sra
                       float Ts, double* d_f, double* d_g, double* d_H, void* stream);
 
 /* ------------------------------------------------------------------------------
+ * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
+ * poissLAlpha objective): no reference counterpart as a C call -- the reference runs
+ * this loop in MATLAB (bhc/beamhardenSpline.m:340-384 -> npg/NPG.m/PNPG.m).
+ * ------------------------------------------------------------------------------ */
+typedef struct ctprj_pnpg ctprj_pnpg;
+typedef struct ctprj_pnpg_info {
+    double cost;       /* objective at the current iterate */
+    double t;          /* inverse step size */
+    double theta;      /* momentum parameter */
+    int iterations;    /* accepted iterations */
+    int trials;        /* line-search trials (ctprj_pnpg_step calls) */
+    int rejects;       /* trials rejected by the majorization test */
+    int restarts;      /* momentum restarts */
+    int last_accepted; /* 1 if the last trial was accepted */
+} ctprj_pnpg_info;
+
+/* Solver state over `plan` (all slices share one objective): Imea (device, packed like
+ * sinograms), spectrum model (knots, E, Ie as in ctprj_bhc_grad_alpha), Ts, the start
+ * point alpha0 (device image; also the first preAlpha) and the initial inverse step t0. */
+int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const double* knots, int E,
+                      const double* Ie, float Ts, const float* d_alpha0, double t0,
+                      ctprj_pnpg** out);
+/* One line-search trial on d_alpha (updated in place when the trial is accepted).
+ * Enqueues a fixed kernel sequence on `stream` with no host synchronisation and no
+ * allocation: calls may be captured in a HIP graph and replayed. */
+int ctprj_pnpg_step(ctprj_pnpg* solver, float* d_alpha, void* stream);
+/* Copies the device scalars to `info` (synchronises `stream`). */
+int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
+int ctprj_pnpg_destroy(ctprj_pnpg* solver);
+
+/* ------------------------------------------------------------------------------
  * Library state.
  * ------------------------------------------------------------------------------ */
 const char* ctprj_last_error(void);
