@@ -163,6 +163,63 @@ def cpu_baseline(geo, img, sino, reps=3):
     return float(np.median(ts)), kind, threads, cores
 
 
+def pnpg_leg(nslices, trials, dev, stream):
+    """PNPG alpha step on a stack of 512^2 parallel slices (720 views over 180 degrees,
+    729 bins), polychromatic Poisson data with I0 = 1e5, E = 20: `trials` line-search
+    trials captured once in a HIP graph and replayed; each trial is one gradient
+    evaluation (forward + FP64 NLL + adjoint) and one cost evaluation (forward)."""
+    import numpy as np
+    import torch
+
+    from imgrecsrc_amd.bhc import Spline, kappa_logspace, normalize_ie, spline_knots
+    from imgrecsrc_amd.phantom import ellipse_params, rasterise
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+
+    log(f"pnpg leg: {nslices} slices x {trials} trials")
+    geo = PAR512
+    P = Projector([geo] * nslices)
+    n = geo[0]
+    base = [torch.from_numpy(rasterise(ellipse_params(500 + k, n), n).astype(np.float32)).to(dev)
+            for k in range(8)]
+    truth = torch.cat([torch.roll(base[i % 8], shifts=(i // 8) % 32, dims=0).reshape(-1)
+                       for i in range(nslices)])
+    knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
+    Ie = normalize_ie(np.exp(-0.5 * ((np.arange(20) - 7.5) / 3.0) ** 2) + 0.05, knots)
+    raw = P.forward(truth)
+    Ts = float(3.0 / raw.max().item())
+    BL = Spline("b1", knots).polyIout(raw * Ts, Ie, want=1)[0]
+    g = torch.Generator(device=dev).manual_seed(6)
+    counts = torch.poisson(1e5 * BL, generator=g)
+    Imea = (counts / counts.max()).float()
+    del raw, BL, counts
+    alpha = torch.full_like(truth, 0.02)
+    solver = PNPG(P, Imea, knots, Ie, Ts, alpha)
+    solver.step(alpha)  # warm-up trial outside the graph
+    torch.cuda.synchronize()
+    graph = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    with torch.cuda.graph(graph, stream=cap):
+        for _ in range(trials):
+            solver.step(alpha)
+    torch.cuda.synchronize()
+    c0 = solver.stats()["cost"]
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    graph.replay()
+    e1.record(stream)
+    e1.synchronize()
+    ms = e0.elapsed_time(e1)
+    st = solver.stats()
+    nnz = Projector([geo]).nnz() * nslices
+    return {"workload": f"{nslices} x 512^2 parallel, 720 views x 729 bins, E=20, I0=1e5",
+            "trials": trials, "ms_per_trial": round(ms / trials, 3),
+            "GUPS": round(3 * nnz * trials / (ms * 1e-3) / 1e9, 1),
+            "cost_before": c0, "cost_after": st["cost"], "iterations": st["iterations"],
+            "rejects": st["rejects"], "restarts": st["restarts"], "t": st["t"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,6 +230,10 @@ def main():
     ap.add_argument("--no-extras", action="store_true", help="skip the bhc and batched legs")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (PMC scaling reference)")
+    ap.add_argument("--pnpg-slices", type=int, default=512,
+                    help="slices of the 512^2 parallel stack in the PNPG leg (0: skip)")
+    ap.add_argument("--pnpg-trials", type=int, default=8,
+                    help="graph-captured PNPG line-search trials to time")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "TAG"),
                     help="reduce two rocprofv3 --pmc csv runs to profiles/TAG_pmc.json")
     ap.add_argument("--pmc-command", default="", help="command recorded in the summary")
@@ -335,6 +396,9 @@ def main():
                           "batched_GUPS": round(2 * tot_nnz / t_b / 1e6, 1),
                           "plain_GUPS": round(2 * tot_nnz / t_p / 1e6, 1),
                           "speedup": round(t_p / t_b, 3)}
+
+    if not args.no_extras and args.pnpg_slices > 0:
+        out["pnpg"] = pnpg_leg(args.pnpg_slices, args.pnpg_trials, dev, stream)
 
     if not args.no_cpu_baseline:
         log("cpu baseline (reference cpuPrj)")

@@ -96,12 +96,13 @@ __device__ __forceinline__ float forward_ray(const SliceDesc& S, const Ray& r, f
     const RowBounds rb = row_bounds(r, cT, sT, FAN ? S.invd : 0.0f, rowmode);
     const int H = S.H, n = S.n, half = S.half;
     const float* base = (rowmode ? img : imgT) + S.img_off;
+    // plain sums over blocks of 4 rows, compensated across blocks
     KahanAcc acc;
+    float part = 0.0f;
     for (int v = -H; v <= H; ++v) {
         int lo, hi;
         row_range(rb, (float)v, H, lo, hi);
         const float* row = base + (long long)(v + half) * n + half;
-        float part = 0.0f;
         for (int u = lo; u <= hi; ++u) {
             const float x = rowmode ? (float)u : (float)v;
             const float y = rowmode ? (float)v : (float)u;
@@ -111,7 +112,10 @@ __device__ __forceinline__ float forward_ray(const SliceDesc& S, const Ray& r, f
             if (COUNT) part += (w != 0.0f) ? 1.0f : 0.0f;
             else part = fmaf(w, row[u], part);
         }
-        acc.add(part);
+        if (((v + H) & 3) == 3 || v == H) {
+            acc.add(part);
+            part = 0.0f;
+        }
     }
     return acc.s;
 }
@@ -158,7 +162,9 @@ __device__ __forceinline__ float adjoint_pixel(const SliceDesc& S, int x, int y,
 {
     const float fx = (float)x, fy = (float)y;
     const float* srow = sino + S.sino_off;
+    // plain sums over blocks of 4 views, compensated across blocks
     KahanAcc acc;
+    float vsum = 0.0f;
     for (int a = 0; a < S.np; ++a, srow += S.W) {
         const float4 T = views[S.view_off + a];
         const float cT = T.x, sT = T.y;
@@ -208,7 +214,11 @@ __device__ __forceinline__ float adjoint_pixel(const SliceDesc& S, int x, int y,
                 part = fmaf(w, srow[b], part);
             }
         }
-        acc.add(part);
+        vsum += part;
+        if ((a & 3) == 3 || a == S.np - 1) {
+            acc.add(vsum);
+            vsum = 0.0f;
+        }
     }
     return acc.s;
 }

@@ -81,6 +81,11 @@ __device__ __forceinline__ void pixel_bins(const SliceDesc& S, const Ray* rays_v
 
 // ------------------------------------------------------------------ forward ---
 
+// Rows are summed in plain FP32 over blocks of kRowBlock rows, and the block sums are
+// added with compensation: the long sum over rows stays at FP32 rounding of the result
+// while the compensation costs a quarter of a per-row update.
+constexpr int kRowBlock = 4;
+
 template <bool FAN, int NACC>
 __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, float cT, float sT,
                                              const float* __restrict__ img,
@@ -91,6 +96,10 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
     const int H = S.H, n = S.n, half = S.half;
     const float* I = img + S.img_off;
     const float* IT = imgT + S.img_off;
+    float part[NACC];
+#pragma unroll
+    for (int e = 0; e < NACC; ++e) part[e] = 0.0f;
+    int inblock = 0;
     for (int v = vlo; v <= vhi; ++v) {
         int lo, hi;
         row_range(rb, (float)v, H, lo, hi);
@@ -98,9 +107,6 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
         const float* rm = I + (long long)(half - v) * n + half;     // img row -v
         const float* tp = IT + (long long)(v + half) * n + half;    // imgT row  v
         const float* tm = IT + (long long)(half - v) * n + half;    // imgT row -v
-        float part[NACC];
-#pragma unroll
-        for (int e = 0; e < NACC; ++e) part[e] = 0.0f;
         const float fy = (float)v;
         for (int u = lo; u <= hi; ++u) {
             const float fx = (float)u;
@@ -118,8 +124,14 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
                 part[7] = fmaf(w, tm[-u], part[7]);  // R3M   (-y, -x)
             }
         }
+        if (++inblock == kRowBlock || v == vhi) {
 #pragma unroll
-        for (int e = 0; e < NACC; ++e) acc[e].add(part[e]);
+            for (int e = 0; e < NACC; ++e) {
+                acc[e].add(part[e]);
+                part[e] = 0.0f;
+            }
+            inblock = 0;
+        }
     }
 }
 
@@ -220,7 +232,7 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
                                              const float* __restrict__ sino,
                                              const float* __restrict__ zero_row, float cT,
                                              float sT, int a0, int k2, int m2, int x0, int y0,
-                                             KahanAcc* acc)
+                                             float* vpart)
 {
     // p = g'^-1 q0; (R^k M)^-1 = R^k M, (R^k)^-1 = R^-k
     int px, py;
@@ -266,7 +278,7 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
         if ((HSET == 1 && h >= 4) || (HSET == 2 && h < 4)) continue;
-        acc[h].add(FBP && FAN ? part[h] * fw : part[h]);
+        vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
     }
 }
 
@@ -283,8 +295,11 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
         const int a0 = __float_as_int(T.z);
         const bool mirror = (a0 == 0) || (2 * a0 == S.Q);
         const Ray* rview = FAN ? rays + S.ray_off + (long long)i * S.W : rays + S.ray_off + i;
+        float vpart[8];  // this view's sums; compensated once per view
+#pragma unroll
+        for (int h = 0; h < 8; ++h) vpart[h] = 0.0f;
 #define CTPRJ_ELEM(HS, K, M) \
-    adj_sym_elem<FAN, FBP, HS>(S, rview, sino, zero_row, T.x, T.y, a0, K, M, x0, y0, acc)
+    adj_sym_elem<FAN, FBP, HS>(S, rview, sino, zero_row, T.x, T.y, a0, K, M, x0, y0, vpart)
         if (mirror) {
             // rotations g' -> rotation orbit pixels, reflections g' -> reflected ones
             const int nk = S.fold ? 2 : 4;
@@ -301,6 +316,8 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
             for (int k = 0; k < 4; ++k) CTPRJ_ELEM(0, k, 1);
         }
 #undef CTPRJ_ELEM
+#pragma unroll
+        for (int h = 0; h < 8; ++h) acc[h].add(vpart[h]);
     }
 }
 
