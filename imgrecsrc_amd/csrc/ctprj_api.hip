@@ -93,8 +93,10 @@ int validate_geom(const ctprj_geom& g, int idx)
             return fail(CTPRJ_EINVAL, "slice %d: detector half width %.1f too wide for d=%g", idx,
                         tmax, (double)g.d);
     }
+    // the kernels address one slice's image and sinogram with 32-bit byte offsets
     const double sino = (double)g.np * g.prjWidth, img = (double)g.n * g.n;
-    if (sino > 2e9 || img > 2e9) return fail(CTPRJ_EINVAL, "slice %d: too large", idx);
+    if (4.0 * sino >= 2147483648.0 || 4.0 * img >= 2147483648.0)
+        return fail(CTPRJ_EINVAL, "slice %d: image or sinogram exceeds 2 GiB", idx);
     return 0;
 }
 
@@ -196,28 +198,59 @@ static std::vector<Item> xcd_order(const std::vector<Item>& in)
     return out;
 }
 
-// Canonical views of the 8-fold symmetric decomposition: a0 in [0, Q/2] whose images
-// (g a0, folded for a half-turn parallel scan) include a computed view.  Returns the
-// share of useful view evaluations (1 for a full or a half-turn scan).
-static double symmetric_views(const ctprj_geom& g, bool fold, std::vector<int>* canon)
+// Canonical views of the symmetric decomposition under group `grp`: a0 in [0, P/q]
+// whose images (g a0, folded for a half-turn parallel scan) include a computed view.
+// Returns the share of useful view evaluations (1 for a full or a half-turn scan).
+static double symmetric_views(const ctprj_geom& g, int grp, bool fold, std::vector<int>* canon)
 {
-    const int Q = g.prjFull / 4, P = 4 * Q;
+    const GroupMasks G = group_masks(grp);
+    const int P = g.prjFull;
     long long slots = 0;
-    for (int a0 = 0; 2 * a0 <= Q; ++a0) {
-        const bool mirror = (a0 == 0) || (2 * a0 == Q);
+    for (int a0 = 0; G.q * a0 <= P; ++a0) {
+        const bool mirror = (a0 == 0) || (G.q * a0 == P);
+        const unsigned els = mirror ? G.rot : G.all;
         bool any = false;
+        int count = 0;
         for (int e = 0; e < 8; ++e) {
-            if (mirror && e >= 4) continue;
-            int v = ((e >> 2) ? 2 * Q - a0 : a0) + (e & 3) * Q;
+            if (!(els >> e & 1u)) continue;
+            const int k = e & 3, m = e >> 2;
+            int v = ((m ? -4 * a0 : 4 * a0) + ((k + 2 * m) & 3) * P) / 4;
             v %= P;
-            if (fold && v >= 2 * Q) v -= 2 * Q;
+            if (v < 0) v += P;
+            if (fold && 2 * v >= P) v -= P / 2;
             if (v < g.np) any = true;
+            ++count;
         }
         if (!any) continue;
         canon->push_back(a0);
-        slots += fold ? (mirror ? 2 : 4) : (mirror ? 4 : 8);
+        slots += fold ? count / 2 : count;
     }
     return slots > 0 ? (double)g.np / (double)slots : 0.0;
+}
+
+// Symmetry group of a slice under `flags` (bit 0: symmetric kernels on; bit 1: skip D4;
+// bit 2: skip V4): the largest group whose view maps are integral and whose canonical
+// evaluations are at least half useful, else kGrpNone.
+static int choose_group(const ctprj_geom& g, int flags, bool* fold, std::vector<int>* canon)
+{
+    canon->clear();
+    *fold = false;
+    if (!(flags & 1) || g.prjFull < 1 || g.np > g.prjFull) return kGrpNone;
+    const bool fan = g.d > 0.0f;
+    const int P = g.prjFull;
+    const int cands[3] = {kGrpD4, kGrpV4, kGrpZ2};
+    for (int grp : cands) {
+        if (grp == kGrpD4 && (P % 4 || (flags & 2))) continue;
+        if (grp == kGrpV4 && (P % 2 || (flags & 4))) continue;
+        const bool fo = !fan && grp != kGrpZ2 && 2 * g.np <= P;
+        std::vector<int> c;
+        if (symmetric_views(g, grp, fo, &c) >= 0.5) {
+            *fold = fo;
+            *canon = c;
+            return grp;
+        }
+    }
+    return kGrpNone;
 }
 
 int Plan::build_tables()
@@ -258,13 +291,13 @@ int Plan::build_tables()
         D.view_off = nview;
         D.ray_off = nray;
         std::vector<int> canon;
-        bool sym = (tune.flags & 1) && g.prjFull % 4 == 0 && g.prjFull >= 4 && g.np <= g.prjFull;
-        const bool fold = sym && !D.fan && g.np <= g.prjFull / 2;
-        if (sym && symmetric_views(g, fold, &canon) < 0.5) sym = false;
+        bool fold = false;
+        const int grp = choose_group(g, tune.flags, &fold, &canon);
+        const bool sym = grp != kGrpNone;
         if (sym) {
-            D.sym = 1;
+            D.sym = grp;
             D.fold = fold;
-            D.Q = g.prjFull / 4;
+            D.P = g.prjFull;
             D.ncanon = (int)canon.size();
             for (int a0 : canon) {
                 float c, si;
@@ -273,7 +306,7 @@ int Plan::build_tables()
                 std::memcpy(&tag, &a0, sizeof(tag));
                 views.push_back(make_float4(c, si, tag, 0.0f));
             }
-            slice_sym[s] = 1;
+            slice_sym[s] = grp;
             ++n_sym;
             if (g.n % 2 == 0) border.push_back(s);
         } else {
@@ -299,6 +332,15 @@ int Plan::build_tables()
         ray_prefix[s + 1] = nray;
     }
     n_border = (int)border.size();
+    // the transposed image serves the general forward's steep views, D4's odd rotations
+    // and the column walks of V4 / Z2 canonical views beyond 45 degrees
+    std::vector<AdjItem> tr;
+    for (int s = 0; s < nslices; ++s) {
+        for (int r = 0; r < geoms[s].n; r += 32)
+            for (int c = 0; c < geoms[s].n; c += 32) tr.push_back(AdjItem{s, r, c, 0});
+    }
+    free_ptr(d_tr_items);
+    n_tr_items = (int)tr.size();
     float4* d_bins = nullptr;
     long long *d_bin_off = nullptr, *d_ray_prefix = nullptr;
     int rc;
@@ -306,6 +348,7 @@ int Plan::build_tables()
         (rc = upload(&d_bins, bins, "bin table")) || (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
         (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
         (rc = upload(&d_border_ids, border, "border list")) ||
+        (rc = upload(&d_tr_items, tr, "transpose work list")) ||
         (rc = dev_alloc((void**)&d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
         free_dev(d_bins);
         free_dev(d_bin_off);
@@ -331,10 +374,10 @@ int Plan::build_fwd_items()
         const ctprj_geom& g = geoms[s];
         if (slice_sym[s]) {
             const int pC = g.prjWidth / 2, bhi = pC + (g.prjWidth - 1) / 2;
-            const bool fold = g.d <= 0.0f && g.np <= g.prjFull / 2;
-            const int bmax = fold ? pC : bhi;
             std::vector<int> canon;
-            symmetric_views(g, fold, &canon);
+            bool fold = false;
+            choose_group(g, tune.flags, &fold, &canon);
+            const int bmax = fold ? pC : bhi;
             for (int i = 0; i < (int)canon.size(); ++i)
                 for (int b0 = 0; b0 <= bmax; b0 += 64) sym.push_back(FwdItem{s, i, b0, 0});
         } else {
@@ -360,9 +403,18 @@ int Plan::build_adj_items()
     for (int s = 0; s < nslices; ++s) {
         const int n = geoms[s].n;
         if (slice_sym[s]) {
+            // 8x8 tiles of the group's fundamental domain of orbit base pixels
             const int H = (n - 1) / 2, nt = (H + 1 + 7) / 8;
-            for (int ty = 0; ty < nt; ++ty)
-                for (int tx = ty; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
+            if (slice_sym[s] == kGrpD4) {
+                for (int ty = 0; ty < nt; ++ty)
+                    for (int tx = ty; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
+            } else if (slice_sym[s] == kGrpV4) {
+                for (int ty = 0; ty < nt; ++ty)
+                    for (int tx = 0; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
+            } else {
+                for (int y = -H; y <= H; y += 8)
+                    for (int tx = 0; tx < nt; ++tx) sym.push_back(AdjItem{s, y, tx * 8, 0});
+            }
         } else {
             for (int r = 0; r < n; r += T)
                 for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
@@ -446,18 +498,12 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
     p->img_elems = img;
     p->sino_elems = sino;
     p->max_W = max_W;
-    std::vector<AdjItem> tr;
-    for (int s = 0; s < nslices; ++s)
-        for (int r = 0; r < geoms[s].n; r += 32)
-            for (int c = 0; c < geoms[s].n; c += 32) tr.push_back(AdjItem{s, r, c, 0});
     std::vector<int2> fbpi;
     for (int s = 0; s < nslices; ++s)
         for (int a = 0; a < geoms[s].np; ++a) fbpi.push_back(make_int2(s, a));
-    p->n_tr_items = (int)tr.size();
     p->n_fbp_items = (int)fbpi.size();
     std::vector<float> zeros((size_t)max_W + 1, 0.0f);
-    if ((rc = upload(&p->d_tr_items, tr, "transpose work list")) ||
-        (rc = upload(&p->d_fbp_items, fbpi, "fbp work list")) ||
+    if ((rc = upload(&p->d_fbp_items, fbpi, "fbp work list")) ||
         (rc = upload(&p->d_fbp_kern, kern, "fbp taps")) ||
         (rc = upload(&p->d_fbp_kern_off, kern_off, "fbp tap offsets")) ||
         (rc = upload(&p->d_fbp_prew, prew, "fbp preweights")) ||
@@ -622,7 +668,7 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     if (t->fwd_block != 64 && t->fwd_block != 128 && t->fwd_block != 256)
         return fail(CTPRJ_EINVAL, "fwd_block must be 64, 128 or 256");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
-    if (t->flags & ~1) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
+    if (t->flags & ~7) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
     const bool tables = (t->flags != p->tune.flags);

@@ -13,11 +13,44 @@ namespace ctprj {
 // weight itself is exactly 0 outside the footprint, so a margin never changes a result.
 constexpr float kSupportMargin = 0.02f;
 
+// Word 3 of a raw (stride 0) buffer resource on the gfx9 family, gfx950 included: 32-bit
+// data format, identity component selects.  Loads beyond num_records return 0.
+constexpr int kBufferWord3 = 0x00020000;
+
+// Symmetry group of a slice's view set used by the symmetric kernels (SliceDesc::sym).
+// R rotates the pixel grid by 90 degrees, M reflects it in the x axis; element
+// e = k + 4m is R^k M^m.  The group is the largest whose view maps are integral:
+//   D4 (all eight elements) when prjFull % 4 == 0,
+//   V4 = {1, R^2, M, R^2 M} when prjFull is even,
+//   Z2 = {1, R^2 M} (theta -> -theta) for any prjFull.
+enum : int { kGrpNone = 0, kGrpD4 = 1, kGrpV4 = 2, kGrpZ2 = 3 };
+
+// Element sets of a group as bit masks over e, and its canonical view range.
+struct GroupMasks {
+    unsigned all;        // the group's elements
+    unsigned rot;        // its rotations
+    unsigned fold;       // adjoint elements of a half-turn parallel scan, off a mirror view
+    unsigned mfold_rot;  // ... on a mirror view, paired with rotation orbit pixels
+    unsigned mfold_ref;  // ... on a mirror view, paired with reflected orbit pixels
+    int q;               // canonical views 0 <= a0 <= prjFull/q; mirror views a0 = 0, prjFull/q
+    int order;           // number of elements
+};
+
+__host__ __device__ constexpr GroupMasks group_masks(int grp)
+{
+    // D4 fold set {1, R, M, R^3 M} maps (0, 45) degrees into [0, 180); V4's {1, M} maps
+    // (0, 90) into [0, 180).  Z2 never folds (prjFull odd).
+    return grp == kGrpD4   ? GroupMasks{0xFFu, 0x0Fu, 0x93u, 0x03u, 0x30u, 8, 8}
+           : grp == kGrpV4 ? GroupMasks{0x55u, 0x05u, 0x11u, 0x01u, 0x10u, 4, 4}
+           : grp == kGrpZ2 ? GroupMasks{0x41u, 0x01u, 0x00u, 0x00u, 0x00u, 2, 2}
+                           : GroupMasks{0x01u, 0x01u, 0x00u, 0x00u, 0x00u, 1, 1};
+}
+
 // One slice of a plan.  Offsets are element offsets into the packed arrays.
 struct SliceDesc {
     int n, W, np, fan;     // image edge, bins per view, views, 1 = fan beam
-    int sym, fold;         // 8-fold symmetric kernels; parallel beam folded over 180 deg
-    int Q, ncanon;         // prjFull/4; canonical views held in the view / ray tables
+    int sym, fold;         // symmetry group (kGrp*); parallel beam folded over 180 deg
+    int P, ncanon;         // prjFull; canonical views held in the view / ray tables
     int H;                 // (n-1)/2: pixel coordinates run over [-H, H]
     int half;              // n/2: coordinate -> index offset
     int pC;                // W/2: index of the bin at t = 0

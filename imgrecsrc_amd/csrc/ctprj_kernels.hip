@@ -314,7 +314,8 @@ void launch_build_rays(const SliceDesc* desc, int nslices, const float4* views,
 void launch_transpose(int nitems, const SliceDesc* desc, const AdjItem* items, const float* in,
                       float* out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_transpose, dim3(nitems), dim3(256), 0, st, desc, items, in, out);
+    if (nitems > 0)
+        hipLaunchKernelGGL(k_transpose, dim3(nitems), dim3(256), 0, st, desc, items, in, out);
 }
 
 void launch_forward_kernel(bool count, int block, int nitems, const SliceDesc* desc,

@@ -1,20 +1,22 @@
-// 8-fold symmetric projector kernels for gfx950.
+// Symmetric projector kernels for gfx950.
 //
-// When prjFull is a multiple of 4 the view set is closed under the symmetry group D4 of
-// the pixel grid (rotations by 90 degrees and reflections): with Q = prjFull/4,
-//   R  (rotate +90):          pixel (x, y) -> (-y, x),  ray (a, b) -> (a + Q, b)
-//   M  (reflect in the x axis): pixel (x, y) -> (x, -y), ray (a, b) -> (2Q - a, 2pC - b)
-// (views mod 4Q; prj/cpuPrj.c:401-460 writes the same eight images).  Element
-// e = k + 4m is R^k M^m.  Every weight is evaluated once, for a canonical view
-// a0 in [0, Q/2] (0..45 degrees), and reused for the eight images:
-//   A[(g a0, g b), g p] = W(a0, b, p)  for g in D4,
+// The view set of a scan over a full (or half) turn is closed under a symmetry group of
+// the pixel grid.  With P = prjFull,
+//   R  (rotate +90):          pixel (x, y) -> (-y, x),  ray (a, b) -> (a + P/4, b)
+//   M  (reflect in the x axis): pixel (x, y) -> (x, -y), ray (a, b) -> (P/2 - a, 2pC - b)
+// (views mod P; prj/cpuPrj.c:401-460 writes the same eight images for P % 4 == 0).
+// Element e = k + 4m is R^k M^m.  A slice uses the largest group whose view maps are
+// integral: D4 (all eight) for P % 4 == 0, V4 = {1, R^2, M, R^2 M} for even P, and
+// Z2 = {1, R^2 M} (theta -> -theta) otherwise.  Every weight is evaluated once, for a
+// canonical view a0 in [0, P/q] (q = 8, 4, 2), and reused for the group's images:
+//   A[(g a0, g b), g p] = W(a0, b, p)  for g in the group,
 // which fixes the float value of every matrix entry, so forward and adjoint stay
-// bitwise transposes.  Canonical views on a mirror line (a0 = 0, a0 = Q/2) use the four
+// bitwise transposes.  Canonical views on a mirror line (a0 = 0, a0 = P/q) use the
 // rotations only, otherwise each view would be counted twice.
-// Parallel beam over half a turn (np <= 2Q, "fold"): ray (a + 2Q, b) is ray
-// (a, 2pC - b); the adjoint uses the half group S = {R0, R1, M, R3 M} (which maps
-// (0, 45) into [0, 180)) and folds, the forward takes half the detector with all eight
-// images.  |dist| in the weight makes W(a0, b, p) == W(a0, 2pC - b, R^2 p) exactly.
+// Parallel beam over half a turn (np <= P/2, "fold"; D4 and V4): ray (a + P/2, b) is ray
+// (a, 2pC - b); the adjoint uses a half set of elements that maps the canonical range
+// into [0, 180) and folds, the forward takes half the detector with all the images.
+// |dist| in the weight makes W(a0, b, p) == W(a0, 2pC - b, R^2 p) exactly.
 #include "ctprj_internal.h"
 
 namespace ctprj {
@@ -32,22 +34,30 @@ __device__ __forceinline__ void apply_px(int k, int m, int x, int y, int& ox, in
 }
 
 // (k, m) action on (canonical view a0, bin b) -> (view, sign, offset): bin = off + sign*b.
-// Views beyond np read a zero row (valid = false).
+// 4 v = (+-4 a0) + (k + 2m) P, a multiple of 4 for every element of the slice's group.
+// Views beyond np read a zero row (the caller checks v < np).
 __device__ __forceinline__ int map_view(const SliceDesc& S, int k, int m, int a0, int& sign,
                                         int& off)
 {
-    const int P = 4 * S.Q;
-    int v = (m ? 2 * S.Q - a0 : a0) + (k & 3) * S.Q;
-    v %= P;
+    const int P = S.P;
+    // a0 <= P/2, so v starts in [-P/2, 7P/2): reduce mod P without a division
+    int v = ((m ? -4 * a0 : 4 * a0) + ((k + 2 * m) & 3) * P) >> 2;
     if (v < 0) v += P;
+    if (v >= 2 * P) v -= 2 * P;
+    if (v >= P) v -= P;
     sign = m ? -1 : 1;
     off = m ? 2 * S.pC : 0;
-    if (S.fold && v >= 2 * S.Q) {
-        v -= 2 * S.Q;
+    if (S.fold && 2 * v >= P) {
+        v -= P >> 1;
         sign = -sign;
         off = 2 * S.pC - off;
     }
     return v;
+}
+
+__device__ __forceinline__ bool mirror_view(const SliceDesc& S, int a0)
+{
+    return a0 == 0 || group_masks(S.sym).q * a0 == S.P;
 }
 
 // Candidate bins of pixel (x, y) for a view (shared with the general adjoint's rule).
@@ -86,47 +96,74 @@ __device__ __forceinline__ void pixel_bins(const SliceDesc& S, const Ray* rays_v
 // while the compensation costs a quarter of a per-row update.
 constexpr int kRowBlock = 4;
 
-template <bool FAN, int NACC>
+// Raw buffer view of one slice's array: loads take a wave-uniform byte offset (a row) in
+// an SGPR and a 32-bit per-lane byte offset (a column), so an address costs one integer
+// add per lane instead of a 64-bit pointer update.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slice_rsrc(const float* base, int elems)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, elems * 4,
+                                             kBufferWord3);
+}
+
+__device__ __forceinline__ float ldb(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// One ray of canonical view a0 against every image g p of the pixels p it meets, for
+// the elements g in MASK.  Rows mode walks image rows v = y with u = x inside; columns
+// mode (canonical views steeper than 45 degrees) walks v = x with u = y inside.  Each
+// image pixel g (x, y) is read from the image or its transpose at row +-v, column +-u,
+// whichever makes u the contiguous index.
+template <bool FAN, unsigned MASK, bool COLS>
 __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, float cT, float sT,
                                              const float* __restrict__ img,
                                              const float* __restrict__ imgT, int vlo, int vhi,
                                              KahanAcc* acc)
 {
-    const RowBounds rb = row_bounds(r, cT, sT, FAN ? S.invd : 0.0f, true);
+    const RowBounds rb = row_bounds(r, cT, sT, FAN ? S.invd : 0.0f, !COLS);
     const int H = S.H, n = S.n, half = S.half;
-    const float* I = img + S.img_off;
-    const float* IT = imgT + S.img_off;
-    float part[NACC];
+    const __amdgpu_buffer_rsrc_t RI = slice_rsrc(img + S.img_off, n * n);
+    const __amdgpu_buffer_rsrc_t RT = slice_rsrc(imgT + S.img_off, n * n);
+    // rows mode: g (u, v) of element e at [RI|RT](row +-v, col +-u); columns mode swaps
+    // the two arrays' roles
+    const __amdgpu_buffer_rsrc_t RA = COLS ? RT : RI;
+    const __amdgpu_buffer_rsrc_t RB = COLS ? RI : RT;
+    float part[8];
 #pragma unroll
-    for (int e = 0; e < NACC; ++e) part[e] = 0.0f;
+    for (int e = 0; e < 8; ++e) part[e] = 0.0f;
     int inblock = 0;
     for (int v = vlo; v <= vhi; ++v) {
         int lo, hi;
         row_range(rb, (float)v, H, lo, hi);
-        const float* rp = I + (long long)(v + half) * n + half;     // img row  v
-        const float* rm = I + (long long)(half - v) * n + half;     // img row -v
-        const float* tp = IT + (long long)(v + half) * n + half;    // imgT row  v
-        const float* tm = IT + (long long)(half - v) * n + half;    // imgT row -v
-        const float fy = (float)v;
-        for (int u = lo; u <= hi; ++u) {
-            const float fx = (float)u;
+        const int op = (v + half) * n * 4;  // byte offset of row  v
+        const int om = (half - v) * n * 4;  // byte offset of row -v
+        const float fv = (float)v;
+        int cp = (half + lo) * 4, cm = (half - lo) * 4;  // columns +u, -u
+        for (int u = lo; u <= hi; ++u, cp += 4, cm -= 4) {
+            const float fu = (float)u;
+            const float fx = COLS ? fv : fu, fy = COLS ? fu : fv;
             const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
             const float hb = FAN ? r.hbc * fan_depth_scale(fx, fy, cT, sT, S.invd) : r.hbc;
             const float w = strip_weight(dist, hb, r);
-            part[0] = fmaf(w, rp[u], part[0]);   // R0   ( x,  y)
-            part[1] = fmaf(w, tm[u], part[1]);   // R1   (-y,  x)
-            part[2] = fmaf(w, rm[-u], part[2]);  // R2   (-x, -y)
-            part[3] = fmaf(w, tp[-u], part[3]);  // R3   ( y, -x)
-            if (NACC == 8) {
-                part[4] = fmaf(w, rm[u], part[4]);   // M     ( x, -y)
-                part[5] = fmaf(w, tp[u], part[5]);   // R1M   ( y,  x)
-                part[6] = fmaf(w, rp[-u], part[6]);  // R2M   (-x,  y)
-                part[7] = fmaf(w, tm[-u], part[7]);  // R3M   (-y, -x)
-            }
+            // rows: e0 ( x, y) A[v][u]   e1 (-y, x) B[-v][u]  e2 (-x,-y) A[-v][-u]
+            //       e3 ( y,-x) B[v][-u]  e4 ( x,-y) A[-v][u]  e5 ( y, x) B[v][u]
+            //       e6 (-x, y) A[v][-u]  e7 (-y,-x) B[-v][-u]          (A = img, B = imgT)
+            // cols: e0 A[v][u]  e1 B[v][-u]  e2 A[-v][-u]  e3 B[-v][u]
+            //       e4 A[v][-u] e5 B[v][u]   e6 A[-v][u]   e7 B[-v][-u]  (A = imgT, B = img)
+            if (MASK & 0x01) part[0] = fmaf(w, ldb(RA, cp, op), part[0]);
+            if (MASK & 0x02) part[1] = fmaf(w, COLS ? ldb(RB, cm, op) : ldb(RB, cp, om), part[1]);
+            if (MASK & 0x04) part[2] = fmaf(w, ldb(RA, cm, om), part[2]);
+            if (MASK & 0x08) part[3] = fmaf(w, COLS ? ldb(RB, cp, om) : ldb(RB, cm, op), part[3]);
+            if (MASK & 0x10) part[4] = fmaf(w, COLS ? ldb(RA, cm, op) : ldb(RA, cp, om), part[4]);
+            if (MASK & 0x20) part[5] = fmaf(w, ldb(RB, cp, op), part[5]);
+            if (MASK & 0x40) part[6] = fmaf(w, COLS ? ldb(RA, cp, om) : ldb(RA, cm, op), part[6]);
+            if (MASK & 0x80) part[7] = fmaf(w, ldb(RB, cm, om), part[7]);
         }
         if (++inblock == kRowBlock || v == vhi) {
 #pragma unroll
-            for (int e = 0; e < NACC; ++e) {
+            for (int e = 0; e < 8; ++e) {
+                if (!(MASK >> e & 1u)) continue;
                 acc[e].add(part[e]);
                 part[e] = 0.0f;
             }
@@ -135,19 +172,22 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
     }
 }
 
-// One canonical ray run: NACC images (8, or the 4 rotations on a mirror line).
-template <bool FAN, int NACC, int SEG>
+// One canonical ray run over the images in MASK (the group, or its rotations on a
+// mirror view).
+template <bool FAN, unsigned MASK, bool COLS, int SEG>
 __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& it, float cT,
                                             float sT, int a0, const Ray* __restrict__ rays,
                                             const float* __restrict__ img,
                                             const float* __restrict__ imgT,
                                             float* __restrict__ sino, float (*red)[8][64])
 {
-    const int lane = (int)threadIdx.x & 63, seg = (int)threadIdx.x >> 6;
+    // the wave index is uniform over the wave: the row loop then runs on scalar registers
+    const int lane = (int)threadIdx.x & 63;
+    const int seg = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
     const int b = it.bin0 + lane;
     const int bmax = S.fold ? S.pC : S.bhi;
     const bool active = b >= S.blo && b <= bmax;
-    KahanAcc acc[NACC];
+    KahanAcc acc[8];
     if (active) {
         Ray r;
         if (FAN) r = rays[S.ray_off + (long long)it.view * S.W + b];
@@ -157,15 +197,17 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
         }
         const int nrow = 2 * S.H + 1, per = (nrow + SEG - 1) / SEG;
         const int vlo = -S.H + seg * per, vhi = min(S.H, vlo + per - 1);
-        fwd_sym_rows<FAN, NACC>(S, r, cT, sT, img, imgT, vlo, vhi, acc);
+        fwd_sym_rows<FAN, MASK, COLS>(S, r, cT, sT, img, imgT, vlo, vhi, acc);
     }
     if (SEG > 1) {
 #pragma unroll
-        for (int e = 0; e < NACC; ++e) red[seg][e][lane] = acc[e].s;
+        for (int e = 0; e < 8; ++e)
+            if (MASK >> e & 1u) red[seg][e][lane] = acc[e].s;
         __syncthreads();
         if (seg != 0) return;
 #pragma unroll
-        for (int e = 0; e < NACC; ++e) {
+        for (int e = 0; e < 8; ++e) {
+            if (!(MASK >> e & 1u)) continue;
             float t = red[0][e][lane];
 #pragma unroll
             for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
@@ -175,7 +217,8 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
     float* out = sino + S.sino_off;
     if (b < S.blo) {  // even detector: bin 0 carries no ray in any view
 #pragma unroll
-        for (int e = 0; e < NACC; ++e) {
+        for (int e = 0; e < 8; ++e) {
+            if (!(MASK >> e & 1u)) continue;
             int sg, off;
             const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
             if (v < S.np) out[(long long)v * S.W] = 0.0f;
@@ -184,11 +227,43 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
     }
     if (!active) return;
 #pragma unroll
-    for (int e = 0; e < NACC; ++e) {
+    for (int e = 0; e < 8; ++e) {
+        if (!(MASK >> e & 1u)) continue;
         int sg, off;
         const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
         if (v < S.np) out[(long long)v * S.W + off + sg * b] = acc[e].s;
     }
+}
+
+template <bool FAN, int SEG>
+__device__ __forceinline__ void fwd_sym_dispatch(const SliceDesc& S, const FwdItem& it, float cT,
+                                                 float sT, int a0, bool mirror,
+                                                 const Ray* __restrict__ rays,
+                                                 const float* __restrict__ img,
+                                                 const float* __restrict__ imgT,
+                                                 float* __restrict__ sino, float (*red)[8][64])
+{
+    // D4's canonical views lie in [0, 45] degrees and always walk rows; V4's reach 90 and
+    // Z2's 180 degrees, and the steep ones walk columns
+    const bool cols = fabsf(cT) < fabsf(sT);
+#define CTPRJ_RUN(MASK, COLS) \
+    fwd_sym_run<FAN, MASK, COLS, SEG>(S, it, cT, sT, a0, rays, img, imgT, sino, red)
+#define CTPRJ_GROUP(G, BOTH)                                                    \
+    do {                                                                        \
+        constexpr GroupMasks GM = group_masks(G);                               \
+        if (BOTH && cols) {                                                     \
+            if (mirror) CTPRJ_RUN(GM.rot, true);                                \
+            else CTPRJ_RUN(GM.all, true);                                       \
+        } else {                                                                \
+            if (mirror) CTPRJ_RUN(GM.rot, false);                               \
+            else CTPRJ_RUN(GM.all, false);                                      \
+        }                                                                       \
+    } while (0)
+    if (S.sym == kGrpD4) CTPRJ_GROUP(kGrpD4, false);
+    else if (S.sym == kGrpV4) CTPRJ_GROUP(kGrpV4, true);
+    else CTPRJ_GROUP(kGrpZ2, true);
+#undef CTPRJ_GROUP
+#undef CTPRJ_RUN
 }
 
 // One work item: (slice, canonical view index, first bin of 64).  SEG waves share the
@@ -208,31 +283,25 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
     const SliceDesc S = slices[it.slice];
     const float4 T = views[S.view_off + it.view];
     const int a0 = __float_as_int(T.z);
-    const bool mirror = (a0 == 0) || (2 * a0 == S.Q);
-    // (fan, mirror) are uniform over the work-group: the barrier inside is reached by all
-    if (S.fan) {
-        if (mirror) fwd_sym_run<true, 4, SEG>(S, it, T.x, T.y, a0, rays, img, imgT, sino, red);
-        else fwd_sym_run<true, 8, SEG>(S, it, T.x, T.y, a0, rays, img, imgT, sino, red);
-    } else {
-        if (mirror) fwd_sym_run<false, 4, SEG>(S, it, T.x, T.y, a0, rays, img, imgT, sino, red);
-        else fwd_sym_run<false, 8, SEG>(S, it, T.x, T.y, a0, rays, img, imgT, sino, red);
-    }
+    const bool mirror = mirror_view(S, a0);
+    // (fan, group, mirror) are uniform over the work-group: the barrier inside is
+    // reached by all of its waves
+    if (S.fan) fwd_sym_dispatch<true, SEG>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
+    else fwd_sym_dispatch<false, SEG>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
 }
 
 // ------------------------------------------------------------------ adjoint ---
 
 // Orbit gather for one canonical view and one group element g' = (k2, m2): the weight
-// W(a0, b, g'^-1 q0) is shared by the orbit pixels h q0 in HSET, which read view
-// h g' (a0, b).  HSET 0: all eight h; 1: rotations; 2: reflections.  On a mirror view
-// (a0 = 0 or Q/2) rotations g' pair with rotation h and reflections with reflection h,
-// so that h g' is always a rotation: every entry then takes the forward's key
-// (a0, b, R^-k q) and the two directions evaluate the same float weight.
-template <bool FAN, bool FBP, int HSET>
+// W(a0, b, g'^-1 q0) is shared by the orbit pixels h q0 for h in HMASK, which read view
+// h g' (a0, b).  On a mirror view rotations g' pair with rotation h and reflections
+// with reflection h, so that h g' is always a rotation: every entry then takes the
+// forward's key (a0, b, R^-k q) and the two directions evaluate the same float weight.
+template <bool FAN, bool FBP, unsigned HMASK>
 __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __restrict__ rview,
-                                             const float* __restrict__ sino,
-                                             const float* __restrict__ zero_row, float cT,
-                                             float sT, int a0, int k2, int m2, int x0, int y0,
-                                             float* vpart)
+                                             __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
+                                             float cT, float sT, int a0, int k2, int m2, int x0,
+                                             int y0, float* vpart)
 {
     // p = g'^-1 q0; (R^k M)^-1 = R^k M, (R^k)^-1 = R^-k
     int px, py;
@@ -242,21 +311,28 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
     int blo, bhi;
     pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
     if (blo > bhi) return;
-    const float* base[8];
-    int sgn[8];
+    // view rows of the elements h g' (wave-uniform): the sinogram row, or the zero row for
+    // views beyond np; bins read at b or, for reflections, at 2pC - b
+    __amdgpu_buffer_rsrc_t rs[8];
+    int soff[8];
+    bool flip[8];
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {  // element h g'
+    for (int h = 0; h < 8; ++h) {
+        if (!(HMASK >> h & 1u)) continue;
         const int k1 = h & 3, m1 = h >> 2;
         const int k = k1 + (m1 ? -k2 : k2), m = m1 ^ m2;
-        int off;
-        const int v = map_view(S, k, m, a0, sgn[h], off);
-        base[h] = (v < S.np) ? sino + S.sino_off + (long long)v * S.W + off : zero_row;
-        if (v >= S.np) sgn[h] = 1;
+        int sg, off;
+        const int v = map_view(S, k, m, a0, sg, off);
+        const bool ok = v < S.np;
+        rs[h] = ok ? RS : RZ;
+        soff[h] = ok ? v * S.W * 4 : 0;
+        flip[h] = ok && sg < 0;
     }
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     float part[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) part[h] = 0.0f;
+    const int two_pc = 2 * S.pC;
     for (int b = blo; b <= bhi; ++b) {
         Ray r;
         if (FAN) r = rview[b];
@@ -266,64 +342,128 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
         }
         const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
         const float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+        const int ob = b * 4, of = (two_pc - b) * 4;
 #pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            if ((HSET == 1 && h >= 4) || (HSET == 2 && h < 4)) continue;
-            part[h] = fmaf(w, base[h][sgn[h] * b], part[h]);
-        }
+        for (int h = 0; h < 8; ++h)
+            if (HMASK >> h & 1u) part[h] = fmaf(w, ldb(rs[h], flip[h] ? of : ob, soff[h]), part[h]);
     }
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258); qe of g'^-1 q0 at a0 is the depth of
     // every orbit pixel at its own view
     const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
 #pragma unroll
-    for (int h = 0; h < 8; ++h) {
-        if ((HSET == 1 && h >= 4) || (HSET == 2 && h < 4)) continue;
-        vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
-    }
+    for (int h = 0; h < 8; ++h)
+        if (HMASK >> h & 1u) vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
 }
 
-// Canonical views i = i0, i0 + step, ... for the orbit of (x0, y0).
-template <bool FAN, bool FBP>
+// Canonical views i = i0, i0 + step, ... for the orbit of (x0, y0) under group GRP.
+template <bool FAN, bool FBP, int GRP>
 __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* __restrict__ views,
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ sino,
                                               const float* __restrict__ zero_row, int x0, int y0,
                                               int i0, int step, KahanAcc* acc)
 {
+    constexpr GroupMasks G = group_masks(GRP);
+    constexpr unsigned REF = G.all & ~G.rot;
+    const __amdgpu_buffer_rsrc_t RS = slice_rsrc(sino + S.sino_off, S.np * S.W);
+    const __amdgpu_buffer_rsrc_t RZ = slice_rsrc(zero_row, S.W);
     for (int i = i0; i < S.ncanon; i += step) {
         const float4 T = views[S.view_off + i];
         const int a0 = __float_as_int(T.z);
-        const bool mirror = (a0 == 0) || (2 * a0 == S.Q);
         const Ray* rview = FAN ? rays + S.ray_off + (long long)i * S.W : rays + S.ray_off + i;
         float vpart[8];  // this view's sums; compensated once per view
 #pragma unroll
         for (int h = 0; h < 8; ++h) vpart[h] = 0.0f;
-#define CTPRJ_ELEM(HS, K, M) \
-    adj_sym_elem<FAN, FBP, HS>(S, rview, sino, zero_row, T.x, T.y, a0, K, M, x0, y0, vpart)
-        if (mirror) {
+        if (a0 == 0 || G.q * a0 == S.P) {
             // rotations g' -> rotation orbit pixels, reflections g' -> reflected ones
-            const int nk = S.fold ? 2 : 4;
-            for (int k = 0; k < nk; ++k) CTPRJ_ELEM(1, k, 0);
-            for (int k = 0; k < nk; ++k) CTPRJ_ELEM(2, k, 1);
-        } else if (S.fold) {
-            // half-turn parallel scan: S = {R0, R1, M, R3 M} maps (0, 45) into [0, 180)
-            CTPRJ_ELEM(0, 0, 0);
-            CTPRJ_ELEM(0, 1, 0);
-            CTPRJ_ELEM(0, 0, 1);
-            CTPRJ_ELEM(0, 3, 1);
+            const unsigned grot = S.fold ? G.mfold_rot : G.rot;
+            const unsigned gref = S.fold ? G.mfold_ref : REF;
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e)
+                if (grot >> e & 1u)
+                    adj_sym_elem<FAN, FBP, G.rot>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                                                  e >> 2, x0, y0, vpart);
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e)
+                if (gref >> e & 1u)
+                    adj_sym_elem<FAN, FBP, REF>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                                                e >> 2, x0, y0, vpart);
         } else {
-            for (int k = 0; k < 4; ++k) CTPRJ_ELEM(0, k, 0);
-            for (int k = 0; k < 4; ++k) CTPRJ_ELEM(0, k, 1);
+            // a half-turn parallel scan takes the half set that maps the canonical range
+            // into [0, 180) and reads the rest folded
+            const unsigned gset = S.fold ? G.fold : G.all;
+#pragma unroll 1
+            for (int e = 0; e < 8; ++e)
+                if (gset >> e & 1u)
+                    adj_sym_elem<FAN, FBP, G.all>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                                                  e >> 2, x0, y0, vpart);
         }
-#undef CTPRJ_ELEM
 #pragma unroll
-        for (int h = 0; h < 8; ++h) acc[h].add(vpart[h]);
+        for (int h = 0; h < 8; ++h)
+            if (G.all >> h & 1u) acc[h].add(vpart[h]);
     }
 }
 
-// One work item: an 8x8 tile of the octant 0 <= y0 <= x0 <= H; each thread owns the
-// orbit of (x0, y0) and writes its distinct pixels.  SEG waves split the canonical views
-// (wave w takes i = w, w + SEG, ...); wave 0 sums their partials in order.
+// Orbit base pixels of a work item: an 8x8 tile of the fundamental domain of the group,
+//   D4: 0 <= y0 <= x0 <= H,  V4: 0 <= x0, y0 <= H,  Z2: 0 <= x0 <= H, -H <= y0 <= H
+// (tiles start inside it; the upper ends are checked here).
+template <bool FBP, int SEG, int GRP>
+__device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& it,
+                                             const float4* __restrict__ views,
+                                             const Ray* __restrict__ rays,
+                                             const float* __restrict__ sino,
+                                             const float* __restrict__ zero_row,
+                                             float* __restrict__ img, float (*red)[8][64])
+{
+    constexpr GroupMasks G = group_masks(GRP);
+    // the wave index is uniform over the wave: the view loop then runs on scalar registers
+    const int lane = (int)threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+    const int x0 = it.col0 + (lane & 7);
+    const int y0 = it.row0 + (lane >> 3);
+    const bool valid = x0 <= S.H && y0 <= (GRP == kGrpD4 ? x0 : S.H);
+    KahanAcc acc[8];
+    if (valid) {
+        if (S.fan) adj_sym_pixel<true, FBP, GRP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
+        else adj_sym_pixel<false, FBP, GRP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
+    }
+    if (SEG > 1) {
+#pragma unroll
+        for (int h = 0; h < 8; ++h)
+            if (G.all >> h & 1u) red[wave][h][lane] = acc[h].s;
+        __syncthreads();
+        if (wave != 0) return;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            if (!(G.all >> h & 1u)) continue;
+            float t = red[0][h][lane];
+#pragma unroll
+            for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
+            acc[h].s = t;
+        }
+    }
+    if (!valid) return;
+    float* out = img + S.img_off;
+    const int n = S.n, half = S.half;
+    // each distinct orbit pixel h q0 once, from the first h (in element order) that
+    // reaches it
+    int ox[8], oy[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        if (!(G.all >> h & 1u)) continue;
+        apply_px(h & 3, h >> 2, x0, y0, ox[h], oy[h]);
+        bool fresh = true;
+#pragma unroll
+        for (int g = 0; g < h; ++g)
+            if ((G.all >> g & 1u) && ox[g] == ox[h] && oy[g] == oy[h]) fresh = false;
+        if (!fresh) continue;
+        const float val = FBP ? acc[h].s * S.fbp_scale : acc[h].s;
+        out[(long long)(oy[h] + half) * n + ox[h] + half] = val;
+    }
+}
+
+// SEG waves split the canonical views (wave w takes i = w, w + SEG, ...); wave 0 sums
+// their partials in order.
 template <bool FBP, int SEG>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
@@ -337,43 +477,10 @@ __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __res
     const AdjItem it = items[blockIdx.x];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
-    const int lane = (int)threadIdx.x & 63, wave = (int)threadIdx.x >> 6;
-    const int x0 = it.col0 + (lane & 7);
-    const int y0 = it.row0 + (lane >> 3);
-    const bool valid = y0 <= x0 && x0 <= S.H;
-    KahanAcc acc[8];
-    if (valid) {
-        if (S.fan) adj_sym_pixel<true, FBP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
-        else adj_sym_pixel<false, FBP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
-    }
-    if (SEG > 1) {
-#pragma unroll
-        for (int h = 0; h < 8; ++h) red[wave][h][lane] = acc[h].s;
-        __syncthreads();
-        if (wave != 0) return;
-#pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            float t = red[0][h][lane];
-#pragma unroll
-            for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
-            acc[h].s = t;
-        }
-    }
-    if (!valid) return;
-    if (FBP)
-#pragma unroll
-        for (int h = 0; h < 8; ++h) acc[h].s *= S.fbp_scale;
-    float* out = img + S.img_off;
-    const int n = S.n, half = S.half;
-    const int nrot = (x0 == 0 && y0 == 0) ? 1 : 4;
-    const bool refl = y0 > 0 && y0 < x0;
-#pragma unroll
-    for (int h = 0; h < 8; ++h) {
-        if (h < 4 ? h >= nrot : !refl) continue;
-        int x, y;
-        apply_px(h & 3, h >> 2, x0, y0, x, y);
-        out[(long long)(y + half) * n + x + half] = acc[h].s;
-    }
+    // the group is uniform over the work-group: the barrier inside is reached by all
+    if (S.sym == kGrpD4) adj_sym_tile<FBP, SEG, kGrpD4>(S, it, views, rays, sino, zero_row, img, red);
+    else if (S.sym == kGrpV4) adj_sym_tile<FBP, SEG, kGrpV4>(S, it, views, rays, sino, zero_row, img, red);
+    else adj_sym_tile<FBP, SEG, kGrpZ2>(S, it, views, rays, sino, zero_row, img, red);
 }
 
 // Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the

@@ -97,9 +97,11 @@ int ctprj_plan_layout(const ctprj_plan* plan, long long* img_elems, long long* s
  * updates of one application), computed on the device. */
 int ctprj_plan_nnz(ctprj_plan* plan, long long* nnz, void* stream);
 
-/* Per slice: 1 when it runs on the 8-fold symmetric kernels (prjFull a multiple of 4,
- * tune flag bit0 set, and at least half of each symmetric evaluation lands on computed
- * views), 0 on the general kernels.  `modes` holds nslices ints. */
+/* Per slice, the kernels it runs on: 0 general; symmetric, with the symmetry group of
+ * its view set: 1 = 8-fold (prjFull % 4 == 0), 2 = 4-fold (prjFull even),
+ * 3 = 2-fold (theta -> -theta, any prjFull).  A slice takes the first group that the
+ * tune flags allow and whose symmetric evaluations land at least half on computed views.
+ * `modes` holds nslices ints. */
 int ctprj_plan_modes(const ctprj_plan* plan, int* modes);
 
 /* Kernel shape knobs (the autotuner's search space). */
@@ -107,9 +109,10 @@ typedef struct ctprj_tune {
     int fwd_block;   /* forward work-group: general kernels 64/128/256 threads (one ray
                         each); symmetric kernels fwd_block/64 waves splitting each ray's rows */
     int adj_tile;    /* general adjoint pixel tile edge: 8 or 16 */
-    int flags;       /* bit0: use the 8-fold symmetric kernels where the geometry allows */
-    int adj_seg;     /* symmetric adjoint: waves per 8x8 octant tile splitting the views:
-                        1, 2, 4 or 8 */
+    int flags;       /* bit0: use the symmetric kernels where the geometry allows;
+                        bit1: not the 8-fold group; bit2: not the 4-fold group */
+    int adj_seg;     /* symmetric adjoint: waves per 8x8 tile of orbit base pixels
+                        splitting the views: 1, 2, 4 or 8 */
 } ctprj_tune;
 int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
 int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);

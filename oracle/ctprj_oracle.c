@@ -108,17 +108,56 @@ typedef struct {
     double cosT, sinT;
 } view_t;
 
+static double ref_theta(const orc_conf* c, int a)
+{
+    /* prj/cpuPrj.c:342 -- float theta, then cos()/sin() of that float in double */
+    float thf = (float)(a * 2) * REF_PI_F / (float)c->prjFull;
+    return (double)thf;
+}
+
+/* Canonical view a0 and group element R^k M^m with view a = g a0 (views mod P):
+ *   R^k   maps a0 -> a0 + k P/4,   M maps a0 -> P/2 - a0,
+ * rotations preferred where both reach a (mirror views).  Group by angle_mode:
+ * 2 = all eight (P % 4 == 0), 3 = {1, R^2, M, R^2 M} (P even), 4 = {1, R^2 M}. */
+static void canonical_of(const orc_conf* c, int a, int* a0, int* k, int* m)
+{
+    const int P = c->prjFull;
+    if (c->angle_mode == 2) {
+        const int Q = P / 4, j = a / Q, r = a % Q;
+        if (8 * r <= P) { *a0 = r; *k = j; *m = 0; }
+        else { *a0 = Q - r; *k = (j + 3) & 3; *m = 1; }
+    } else if (c->angle_mode == 3) {
+        const int H = P / 2, j = a / H, r = a % H;
+        if (4 * r <= P) { *a0 = r; *k = 2 * j; *m = 0; }
+        else { *a0 = H - r; *k = 2 * j; *m = 1; }
+    } else {
+        if (2 * a <= P) { *a0 = a; *k = 0; *m = 0; }
+        else { *a0 = P - a; *k = 2; *m = 1; }
+    }
+}
+
 static view_t view_of(const orc_conf* c, int a)
 {
     view_t v;
-    double th;
-    if (c->angle_mode == 1) {
-        /* prj/cpuPrj.c:342 -- float theta, then cos()/sin() of that float in double */
-        float thf = (float)(a * 2) * REF_PI_F / (float)c->prjFull;
-        th = (double)thf;
-    } else {
-        th = (double)a * 2.0 * ORC_PI / (double)c->prjFull;
+    if (c->angle_mode >= 2) {
+        /* the ray normal (cos, sin) of the canonical view, mapped by g: M takes theta to
+         * pi - theta, R to theta + pi/2 (exact in double) */
+        int a0, k, m;
+        canonical_of(c, a, &a0, &k, &m);
+        const double th = ref_theta(c, a0);
+        double cs = cos(th), sn = sin(th);
+        if (m) cs = -cs;
+        for (int i = 0; i < (k & 3); i++) {
+            const double t = cs;
+            cs = -sn;
+            sn = t;
+        }
+        v.cosT = cs;
+        v.sinT = sn;
+        return v;
     }
+    const double th = (c->angle_mode == 1) ? ref_theta(c, a)
+                                           : (double)a * 2.0 * ORC_PI / (double)c->prjFull;
     v.cosT = cos(th);
     v.sinT = sin(th);
     return v;

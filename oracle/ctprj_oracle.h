@@ -37,7 +37,12 @@ typedef struct orc_conf {
     double effectiveRate; /* beam width = dSize*effectiveRate; outputs are divided by it */
     double d;           /* source-to-rotation-centre distance; <= 0 means parallel beam */
     int angle_mode;     /* 0: theta in double; 1: the reference's float theta
-                           (thetaIdx*2*PI/prjFull with PI=3.14159265359f, prj/cpuPrj.c:342) */
+                           (thetaIdx*2*PI/prjFull with PI=3.14159265359f, prj/cpuPrj.c:342);
+                           2, 3, 4: symmetric angles -- view a takes the float theta of
+                           its canonical view a0 under the 8-fold (2), 4-fold (3) or
+                           2-fold (4) symmetry of the pixel grid, rotated / reflected
+                           exactly (prj/cpuPrj.c:383-460 reuses the canonical view's
+                           weights for all eight images in the same way) */
     int support_mode;   /* 0: reference support -- a ray touches a pixel iff the ray's
                               detector strip meets the pixel (prj/cpuPrj.c:365-369,530-536,
                               82-105,219-244);

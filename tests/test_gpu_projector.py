@@ -9,6 +9,7 @@ import numpy as np
 import pytest
 
 from conftest import golden_cases, load_golden, rel_l2
+from test_sym_decomposition import choose_group
 
 pytestmark = pytest.mark.gpu
 
@@ -20,9 +21,23 @@ def torch_dev():
     return torch.device("cuda", 0)
 
 
-def _proj(geo, symmetric=True):
+def _proj(geo, flags=1):
+    """flags: 0 general kernels, 1 symmetric where possible, 3 / 7 with the 8-fold
+    (and 4-fold) group switched off."""
     from imgrecsrc_amd.projector import Projector
-    return Projector([geo], symmetric=symmetric)
+    P = Projector([geo], symmetric=bool(flags & 1))
+    if flags not in (0, 1):
+        P.set_tune(flags=flags)
+    return P
+
+
+def _expected_mode(geo, flags):
+    return choose_group(geo[3], geo[2], geo[6] > 0, flags)[0]
+
+
+# FP64 oracle angle mode matching a slice's kernels: per-view float angles on the
+# general kernels, the canonical view's angle mapped by the group on the symmetric ones
+ANGLE_MODE = {0: 1, 1: 2, 2: 3, 3: 4}
 
 
 def _run(P, x, op):
@@ -70,29 +85,6 @@ def test_golden_parity(torch_dev, oracle_mod, case):
         assert np.all(b[0, :] == 0) and np.all(b[:, 0] == 0)
 
 
-def _sym_eligible(geo):
-    """prjFull % 4 == 0 and at least half of the symmetric evaluations are used."""
-    n, W, npj, pf = geo[:4]
-    fan = geo[6] > 0
-    if pf % 4 or npj > pf:
-        return False
-    Q = pf // 4
-    fold = (not fan) and npj <= 2 * Q
-    slots = 0
-    for a0 in range(Q // 2 + 1):
-        mirror = a0 == 0 or 2 * a0 == Q
-        vs = []
-        for e in range(4 if mirror else 8):
-            v = ((2 * Q - a0) if e >> 2 else a0) + (e & 3) * Q
-            v %= 4 * Q
-            if fold and v >= 2 * Q:
-                v -= 2 * Q
-            vs.append(v)
-        if min(vs) < npj:
-            slots += (2 if mirror else 4) if fold else (4 if mirror else 8)
-    return slots > 0 and npj / slots >= 0.5
-
-
 TINY = [
     (8, 13, 12, 24, 1.0, 1.0, 0.0),
     (7, 10, 9, 18, 1.0, 1.0, 0.0),
@@ -104,19 +96,33 @@ TINY = [
     (9, 14, 20, 40, 1.0, 1.0, 0.0),        # parallel, half turn (folded)
     (10, 15, 28, 28, 1.0, 1.0, 0.0),       # parallel over a full turn
     (7, 12, 9, 24, 1.0, 1.0, 0.0),         # parallel, partial arc
+    (9, 14, 18, 18, 1.0, 1.0, 30.0),       # fan, prjFull = 2 mod 4: 4-fold
+    (8, 13, 9, 18, 1.0, 1.0, 0.0),         # parallel half turn, 4-fold folded
+    (10, 15, 22, 22, 1.1, 0.9, 0.0),       # parallel full turn, 4-fold
+    (7, 12, 13, 13, 1.0, 1.0, 25.0),       # fan, odd prjFull: 2-fold
+    (7, 10, 11, 11, 1.0, 1.0, 0.0),        # parallel, odd prjFull: 2-fold
+    (9, 13, 7, 13, 0.9, 1.1, 30.0),        # fan arc, odd prjFull: 2-fold, half used
+    (8, 11, 6, 24, 1.0, 1.0, 20.0),        # fan quarter arc: 2-fold
 ]
 
+# every geometry on the general and the automatically chosen kernels, and two with the
+# larger groups switched off (4-fold and 2-fold on prjFull % 4 == 0, folded and not)
+TRANSPOSE_CASES = ([(g, f) for g in TINY for f in (1, 0)] +
+                   [((12, 17, 24, 24, 1.0, 1.0, 30.0), f) for f in (3, 7)] +
+                   [((9, 14, 20, 40, 1.0, 1.0, 0.0), f) for f in (3, 7)])
 
-@pytest.mark.parametrize("symmetric", [True, False])
-@pytest.mark.parametrize("geo", TINY)
-def test_forward_adjoint_bitwise_transposes(torch_dev, oracle_mod, geo, symmetric):
+
+@pytest.mark.parametrize("geo,flags", TRANSPOSE_CASES)
+def test_forward_adjoint_bitwise_transposes(torch_dev, oracle_mod, geo, flags):
     """Columns of A from forward(e_p) and rows from adjoint(e_r): every weight is one
-    term, so the two must agree bit for bit; and both within FP32 of the FP64 A.
-    Both kernel families (8-fold symmetric where prjFull % 4 == 0, and general)."""
+    term, so the two must agree bit for bit; and both within FP32 of the FP64 A at the
+    angles the kernels use (per view on the general kernels, the canonical view's angle
+    mapped by the symmetry group on the symmetric ones)."""
     import torch
     n, W, npj = geo[0], geo[1], geo[2]
-    P = _proj(geo, symmetric)
-    assert P.modes() == [int(symmetric and geo[3] % 4 == 0 and _sym_eligible(geo))]
+    P = _proj(geo, flags)
+    mode = _expected_mode(geo, flags)
+    assert P.modes() == [mode]
     eye_img = torch.eye(n * n, device="cuda")
     eye_sino = torch.eye(npj * W, device="cuda")
     Af = torch.stack([P.forward(eye_img[p].contiguous()) for p in range(n * n)], 1)
@@ -124,8 +130,10 @@ def test_forward_adjoint_bitwise_transposes(torch_dev, oracle_mod, geo, symmetri
     torch.cuda.synchronize()
     Af, At = Af.cpu().numpy(), At.cpu().numpy()
     assert np.array_equal(Af, At)
-    A64 = oracle_mod.dense(oracle_mod.conf(*geo, support_mode=1))
-    assert np.abs(Af - A64).max() <= 2e-6 * max(1.0, np.abs(A64).max())
+    A64 = oracle_mod.dense(oracle_mod.conf(*geo, angle_mode=ANGLE_MODE[mode], support_mode=1))
+    err = np.abs(Af - A64).max()
+    print(f"{geo} flags={flags} mode={mode}: max |A - A64| = {err:.2e}")
+    assert err <= 2e-6 * max(1.0, np.abs(A64).max())
 
 
 @pytest.mark.parametrize("geo", [
@@ -150,19 +158,24 @@ def test_adjoint_identity(torch_dev, geo):
     assert rel <= 1e-6
 
 
-@pytest.mark.parametrize("geo", [
-    (256, 367, 180, 360, 1.0, 1.0, 0.0),
-    (257, 365, 360, 360, 1.0, 1.0, 1100.0),
-    (200, 290, 90, 360, 1.0, 1.0, 0.0),     # quarter turn: half the evaluations used
+@pytest.mark.parametrize("geo,flags,mode", [
+    ((256, 367, 180, 360, 1.0, 1.0, 0.0), 1, 1),
+    ((257, 365, 360, 360, 1.0, 1.0, 1100.0), 1, 1),
+    ((200, 290, 90, 360, 1.0, 1.0, 0.0), 1, 1),     # quarter turn: half the evaluations used
+    ((256, 367, 181, 362, 1.0, 1.0, 0.0), 1, 2),    # parallel half turn, 4-fold folded
+    ((300, 427, 298, 298, 1.0, 1.0, 1200.0), 1, 2),  # fan full turn, 4-fold
+    ((255, 363, 361, 361, 1.0, 1.0, 1000.0), 1, 3),  # fan, odd prjFull: 2-fold
+    ((257, 365, 360, 360, 1.0, 1.0, 1100.0), 3, 2),  # 8-fold geometry on the 4-fold group
+    ((257, 365, 360, 360, 1.0, 1.0, 1100.0), 7, 3),  # ... and on the 2-fold group
 ])
-def test_symmetric_matches_general(torch_dev, geo):
-    """The 8-fold symmetric kernels (weights at the canonical view) and the general
-    kernels (weights at each view) agree to FP32 rounding of the view angles."""
+def test_symmetric_matches_general(torch_dev, geo, flags, mode):
+    """The symmetric kernels (weights at the canonical view) and the general kernels
+    (weights at each view) agree to FP32 rounding of the view angles."""
     import torch
     from imgrecsrc_amd.phantom import ellipse_phantom, gaussian
     n, W, npj = geo[:3]
-    S, G = _proj(geo, True), _proj(geo, False)
-    assert S.modes() == [1] and G.modes() == [0]
+    S, G = _proj(geo, flags), _proj(geo, 0)
+    assert S.modes() == [mode] and G.modes() == [0]
     x = torch.from_numpy(ellipse_phantom(n, 9)).cuda().reshape(-1)
     y = S.forward(x)
     e_f = rel_l2(y.cpu().numpy(), G.forward(x).cpu().numpy())
