@@ -179,6 +179,7 @@ def pnpg_leg(nslices, trials, dev, stream):
     log(f"pnpg leg: {nslices} slices x {trials} trials")
     geo = PAR512
     P = Projector([geo] * nslices)
+    P.autotune(reps=2)
     n = geo[0]
     base = [torch.from_numpy(rasterise(ellipse_params(500 + k, n), n).astype(np.float32)).to(dev)
             for k in range(8)]
@@ -262,6 +263,8 @@ def main():
     n, W, npj = geo[0], geo[1], geo[2]
     log(f"workload {args.workload} {geo}")
     P = Projector([geo])
+    tune = P.autotune(reps=5)
+    log(f"autotuned shapes {tune}")
     img_np = ellipse_phantom(n, 2)
     img = torch.from_numpy(img_np).to(dev).reshape(-1)
     sino = torch.empty(P.sino_elems, dtype=torch.float32, device=dev)
@@ -299,7 +302,7 @@ def main():
     reps = max(5, min(args.steps, 50))
     t_fwd = timed(lambda: P.forward(img, out=sino), reps, stream)
     t_adj = timed(lambda: P.adjoint(sino, out=back), reps, stream)
-    sfx = "_sym" if P.modes() == [1] else ""
+    sfx = "_sym" if P.modes()[0] != 0 else ""
     # the forward call also runs the 32x32 transpose of the image (a few microseconds)
     dom_name, dom_ms = (f"k_adjoint{sfx}", t_adj) if t_adj >= t_fwd else (f"k_forward{sfx}", t_fwd)
     alg_bytes = 4.0 * nnz
@@ -327,7 +330,8 @@ def main():
         "data": "synthetic (seeded ellipse phantom, FP32)",
         "config": {"workload": f"{args.workload}: n={n} prjWidth={W} np={npj} "
                                f"prjFull={geo[3]} d={geo[6]}", "nnz": nnz,
-                   "parallelism": "single GPU"},
+                   "parallelism": "single GPU", "symmetry_group": P.modes()[0],
+                   "tune": tune},
         "adjoint_test_rel": adj_rel,
         "roofline": roofline,
     }
@@ -367,6 +371,7 @@ def main():
         log("batched leg")
         geoms = ragged_geoms()
         PBat = Projector(geoms)
+        PBat.autotune(reps=3)
         imgs = [torch.from_numpy(ellipse_phantom(gq[0], 100 + i)).to(dev).reshape(-1)
                 for i, gq in enumerate(geoms)]
         bimg = PBat.pack_images(imgs)
@@ -374,6 +379,12 @@ def main():
         bback = torch.empty(PBat.img_elems, dtype=torch.float32, device=dev)
         tot_nnz = PBat.nnz()
         singles = [Projector([gq]) for gq in geoms]
+        for s1 in singles:
+            s1.autotune(reps=3)
+        # the plain path on the general (per-view) kernels, one slice per launch
+        generals = [Projector([gq], symmetric=False) for gq in geoms]
+        for s1 in generals:
+            s1.autotune(reps=3)
         s_out = [torch.empty(s.sino_elems, dtype=torch.float32, device=dev) for s in singles]
         i_out = [torch.empty(s.img_elems, dtype=torch.float32, device=dev) for s in singles]
 
@@ -386,16 +397,28 @@ def main():
                 s.forward(imgs[k], out=s_out[k])
                 s.adjoint(s_out[k], out=i_out[k])
 
+        def plain_general():
+            for k, s in enumerate(generals):
+                s.forward(imgs[k], out=s_out[k])
+                s.adjoint(s_out[k], out=i_out[k])
+
         batched()
         plain()
+        plain_general()
         t_b = timed(batched, 3, stream)
         t_p = timed(plain, 3, stream)
+        t_g = timed(plain_general, 3, stream)
+        groups = PBat.modes()
         out["batched"] = {"workload": "64 ragged slices, n in {128..1024}, views 90..720, "
                                       "mixed parallel/fan", "nnz": tot_nnz,
                           "batched_ms": round(t_b, 3), "plain_ms": round(t_p, 3),
                           "batched_GUPS": round(2 * tot_nnz / t_b / 1e6, 1),
                           "plain_GUPS": round(2 * tot_nnz / t_p / 1e6, 1),
-                          "speedup": round(t_p / t_b, 3)}
+                          "speedup": round(t_p / t_b, 3),
+                          "plain_general_ms": round(t_g, 3),
+                          "plain_general_GUPS": round(2 * tot_nnz / t_g / 1e6, 1),
+                          "speedup_vs_general": round(t_g / t_b, 3),
+                          "slices_per_group": {str(k): groups.count(k) for k in range(4)}}
 
     if not args.no_extras and args.pnpg_slices > 0:
         out["pnpg"] = pnpg_leg(args.pnpg_slices, args.pnpg_trials, dev, stream)

@@ -460,7 +460,8 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
     p->tune.fwd_block = 256;
     p->tune.adj_tile = 16;
     p->tune.flags = flags;
-    p->tune.adj_seg = 4;
+    p->tune.adj_seg = 8;
+    p->tune.fwd_seg = 8;
     std::vector<long long> kern_off(nslices), prew_off(nslices);
     std::vector<float> kern, prew;
     long long img = 0, sino = 0;
@@ -562,7 +563,7 @@ int Plan::launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool
         launch_forward_kernel(count, tune.fwd_block, n_fwd_items, d_desc, d_fwd_items, d_views,
                               d_rays, d_img, d_imgT, d_sino, st);
     if (n_symf_items)
-        launch_forward_sym(tune.fwd_block / 64, n_symf_items, d_desc, d_symf_items, d_views,
+        launch_forward_sym(tune.fwd_seg, n_symf_items, d_desc, d_symf_items, d_views,
                            d_rays, d_img, d_imgT, d_sino, st);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
@@ -667,6 +668,9 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     if (!p || !t) return fail(CTPRJ_EINVAL, "NULL argument");
     if (t->fwd_block != 64 && t->fwd_block != 128 && t->fwd_block != 256)
         return fail(CTPRJ_EINVAL, "fwd_block must be 64, 128 or 256");
+    if (t->fwd_seg != 1 && t->fwd_seg != 2 && t->fwd_seg != 4 && t->fwd_seg != 8 &&
+        t->fwd_seg != 16)
+        return fail(CTPRJ_EINVAL, "fwd_seg must be 1, 2, 4, 8 or 16");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
     if (t->flags & ~7) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
@@ -691,8 +695,9 @@ static std::string tune_key(const ctprj_plan* p, const char* arch)
     return os.str();
 }
 
-// Times the forward work-group sizes (general: threads per group; symmetric: waves per
-// ray run) and the adjoint tile edges on this plan's geometry; keeps tune.flags.
+// Times the kernel shapes on this plan's geometry: general forward threads per group,
+// symmetric forward waves per ray run, general adjoint tile edge, symmetric adjoint waves
+// per tile; keeps tune.flags.
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -708,8 +713,8 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
             const size_t sp = line.rfind(' ');
             if (sp == std::string::npos || line.substr(0, sp) != key) continue;
             ctprj_tune t{};
-            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d,%d", &t.fwd_block, &t.adj_tile,
-                            &t.flags, &t.adj_seg) == 4)
+            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d,%d,%d", &t.fwd_block, &t.adj_tile,
+                            &t.flags, &t.adj_seg, &t.fwd_seg) == 5)
                 return ctprj_plan_set_tune(p, &t);
         }
     }
@@ -726,58 +731,43 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     (void)hipEventCreate(&e0);
     (void)hipEventCreate(&e1);
     ctprj_tune best = p->tune;
-    float best_f = 1e30f, best_a = 1e30f;
+    // average time of `reps` launches of one direction after one untimed launch
+    auto time_dir = [&](bool fwd, float* out) -> int {
+        float ms = 0;
+        for (int r = -1; r < reps; ++r) {
+            (void)hipEventRecord(e0, st);
+            const int rc2 = fwd ? p->launch_forward(d_img, d_sino, st, false)
+                                : p->launch_adjoint(d_sino, d_img, st, false);
+            if (rc2) return rc2;
+            (void)hipEventRecord(e1, st);
+            (void)hipEventSynchronize(e1);
+            float m = 0;
+            (void)hipEventElapsedTime(&m, e0, e1);
+            if (r >= 0) ms += m;
+        }
+        *out = ms;
+        return 0;
+    };
+    // one knob at a time, the others held at the best found so far
+    auto search = [&](int ctprj_tune::*knob, const int* vals, int nv, bool fwd) -> int {
+        float best_ms = 1e30f;
+        for (int i = 0; i < nv; ++i) {
+            ctprj_tune t = best;
+            t.*knob = vals[i];
+            int rc2 = ctprj_plan_set_tune(p, &t);
+            float ms = 0;
+            if (!rc2) rc2 = time_dir(fwd, &ms);
+            if (rc2) return rc2;
+            if (ms < best_ms) { best_ms = ms; best.*knob = vals[i]; }
+        }
+        return ctprj_plan_set_tune(p, &best);
+    };
     const int blocks[3] = {64, 128, 256}, tiles[2] = {8, 16};
-    for (int bi = 0; bi < 3 && !rc; ++bi) {
-        ctprj_tune t = p->tune;
-        t.fwd_block = blocks[bi];
-        if ((rc = ctprj_plan_set_tune(p, &t))) break;
-        float ms = 0;
-        for (int r = -1; r < reps && !rc; ++r) {
-            (void)hipEventRecord(e0, st);
-            rc = p->launch_forward(d_img, d_sino, st, false);
-            (void)hipEventRecord(e1, st);
-            (void)hipEventSynchronize(e1);
-            float m = 0;
-            (void)hipEventElapsedTime(&m, e0, e1);
-            if (r >= 0) ms += m;
-        }
-        if (ms < best_f) { best_f = ms; best.fwd_block = blocks[bi]; }
-    }
-    for (int ti = 0; ti < 2 && !rc && p->n_adj_items; ++ti) {
-        ctprj_tune t = p->tune;
-        t.adj_tile = tiles[ti];
-        if ((rc = ctprj_plan_set_tune(p, &t))) break;
-        float ms = 0;
-        for (int r = -1; r < reps && !rc; ++r) {
-            (void)hipEventRecord(e0, st);
-            rc = p->launch_adjoint(d_sino, d_img, st, false);
-            (void)hipEventRecord(e1, st);
-            (void)hipEventSynchronize(e1);
-            float m = 0;
-            (void)hipEventElapsedTime(&m, e0, e1);
-            if (r >= 0) ms += m;
-        }
-        if (ms < best_a) { best_a = ms; best.adj_tile = tiles[ti]; }
-    }
-    const int segs[4] = {1, 2, 4, 8};
-    float best_s = 1e30f;
-    for (int si = 0; si < 4 && !rc && p->n_syma_items; ++si) {
-        ctprj_tune t = p->tune;
-        t.adj_seg = segs[si];
-        if ((rc = ctprj_plan_set_tune(p, &t))) break;
-        float ms = 0;
-        for (int r = -1; r < reps && !rc; ++r) {
-            (void)hipEventRecord(e0, st);
-            rc = p->launch_adjoint(d_sino, d_img, st, false);
-            (void)hipEventRecord(e1, st);
-            (void)hipEventSynchronize(e1);
-            float m = 0;
-            (void)hipEventElapsedTime(&m, e0, e1);
-            if (r >= 0) ms += m;
-        }
-        if (ms < best_s) { best_s = ms; best.adj_seg = segs[si]; }
-    }
+    const int fsegs[5] = {1, 2, 4, 8, 16}, asegs[4] = {1, 2, 4, 8};
+    if (!rc && p->n_fwd_items) rc = search(&ctprj_tune::fwd_block, blocks, 3, true);
+    if (!rc && p->n_symf_items) rc = search(&ctprj_tune::fwd_seg, fsegs, 5, true);
+    if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);
+    if (!rc && p->n_syma_items) rc = search(&ctprj_tune::adj_seg, asegs, 4, false);
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     free_dev(d_img);
@@ -787,7 +777,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     if (!rc && cache_path) {
         std::ofstream f(cache_path, std::ios::app);
         f << key << " " << best.fwd_block << "," << best.adj_tile << "," << best.flags << ","
-          << best.adj_seg << "\n";
+          << best.adj_seg << "," << best.fwd_seg << "\n";
     }
     return rc;
 }

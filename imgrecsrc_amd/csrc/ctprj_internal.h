@@ -29,7 +29,7 @@ struct Plan {
     std::vector<int> slice_sym;  // per slice: 1 = 8-fold symmetric kernels
     long long img_elems = 0, sino_elems = 0;
     int max_W = 0;
-    ctprj_tune tune{64, 16, 1, 4};
+    ctprj_tune tune{64, 16, 1, 8, 8};
 
     SliceDesc* d_desc = nullptr;
     float4* d_views = nullptr;

@@ -209,7 +209,7 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
         for (int e = 0; e < 8; ++e) {
             if (!(MASK >> e & 1u)) continue;
             float t = red[0][e][lane];
-#pragma unroll
+#pragma unroll 1
             for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
             acc[e].s = t;
         }
@@ -437,7 +437,7 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
         for (int h = 0; h < 8; ++h) {
             if (!(G.all >> h & 1u)) continue;
             float t = red[0][h][lane];
-#pragma unroll
+#pragma unroll 1
             for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
             acc[h].s = t;
         }
@@ -502,15 +502,17 @@ void launch_forward_sym(int seg, int nitems, const SliceDesc* desc, const FwdIte
                         const float4* views, const Ray* rays, const float* img,
                         const float* imgT, float* sino, hipStream_t st)
 {
-    if (seg == 1)
-        hipLaunchKernelGGL(k_forward_sym<1>, dim3(nitems), dim3(64), 0, st, desc, items, views,
-                           rays, img, imgT, sino);
-    else if (seg == 2)
-        hipLaunchKernelGGL(k_forward_sym<2>, dim3(nitems), dim3(128), 0, st, desc, items, views,
-                           rays, img, imgT, sino);
-    else
-        hipLaunchKernelGGL(k_forward_sym<4>, dim3(nitems), dim3(256), 0, st, desc, items, views,
-                           rays, img, imgT, sino);
+#define CTPRJ_LAUNCH(SEG)                                                                   \
+    hipLaunchKernelGGL(k_forward_sym<SEG>, dim3(nitems), dim3(64 * SEG), 0, st, desc, items, \
+                       views, rays, img, imgT, sino)
+    switch (seg) {
+        case 1: CTPRJ_LAUNCH(1); break;
+        case 2: CTPRJ_LAUNCH(2); break;
+        case 4: CTPRJ_LAUNCH(4); break;
+        case 8: CTPRJ_LAUNCH(8); break;
+        default: CTPRJ_LAUNCH(16); break;
+    }
+#undef CTPRJ_LAUNCH
 }
 
 template <bool FBP>

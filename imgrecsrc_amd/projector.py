@@ -148,7 +148,8 @@ class Projector:
         return v.value
 
     def modes(self):
-        """Per slice: 1 = 8-fold symmetric kernels, 0 = general kernels."""
+        """Per slice: 0 general kernels; symmetric kernels under the 8-fold (1), 4-fold (2)
+        or 2-fold (3) symmetry group of its view set."""
         m = (ctypes.c_int * self.nslices)()
         check(lib().ctprj_plan_modes(self.handle, m))
         return list(m)
@@ -158,12 +159,13 @@ class Projector:
         t = Tune()
         check(lib().ctprj_plan_get_tune(self.handle, ctypes.byref(t)))
         return {"fwd_block": t.fwd_block, "adj_tile": t.adj_tile, "flags": t.flags,
-                "adj_seg": t.adj_seg}
+                "adj_seg": t.adj_seg, "fwd_seg": t.fwd_seg}
 
-    def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None):
+    def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None, fwd_seg=None):
         cur = self.get_tune()
         t = Tune(fwd_block or cur["fwd_block"], adj_tile or cur["adj_tile"],
-                 cur["flags"] if flags is None else flags, adj_seg or cur["adj_seg"])
+                 cur["flags"] if flags is None else flags, adj_seg or cur["adj_seg"],
+                 fwd_seg or cur["fwd_seg"])
         check(lib().ctprj_plan_set_tune(self.handle, ctypes.byref(t)))
 
     def autotune(self, reps: int = 3, cache_path: str | None = None, stream=None) -> dict:

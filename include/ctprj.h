@@ -106,13 +106,14 @@ int ctprj_plan_modes(const ctprj_plan* plan, int* modes);
 
 /* Kernel shape knobs (the autotuner's search space). */
 typedef struct ctprj_tune {
-    int fwd_block;   /* forward work-group: general kernels 64/128/256 threads (one ray
-                        each); symmetric kernels fwd_block/64 waves splitting each ray's rows */
+    int fwd_block;   /* general forward work-group: 64, 128 or 256 threads, one ray each */
     int adj_tile;    /* general adjoint pixel tile edge: 8 or 16 */
     int flags;       /* bit0: use the symmetric kernels where the geometry allows;
                         bit1: not the 8-fold group; bit2: not the 4-fold group */
     int adj_seg;     /* symmetric adjoint: waves per 8x8 tile of orbit base pixels
                         splitting the views: 1, 2, 4 or 8 */
+    int fwd_seg;     /* symmetric forward: waves per 64 canonical rays splitting the
+                        image rows: 1, 2, 4, 8 or 16 */
 } ctprj_tune;
 int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
 int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);
