@@ -230,12 +230,13 @@ static double symmetric_views(const ctprj_geom& g, int grp, bool fold, std::vect
 
 // Symmetry group of a slice under `flags` (bit 0: symmetric kernels on; bit 1: skip D4;
 // bit 2: skip V4): the largest group whose view maps are integral and whose canonical
-// evaluations are at least half useful, else kGrpNone.
+// evaluations are at least half useful, else kGrpNone (also for images under 5x5).
 static int choose_group(const ctprj_geom& g, int flags, bool* fold, std::vector<int>* canon)
 {
     canon->clear();
     *fold = false;
-    if (!(flags & 1) || g.prjFull < 1 || g.np > g.prjFull) return kGrpNone;
+    // the symmetric forward reads 4-pixel windows inside each row: n >= 5
+    if (!(flags & 1) || g.n < 5 || g.prjFull < 1 || g.np > g.prjFull) return kGrpNone;
     const bool fan = g.d > 0.0f;
     const int P = g.prjFull;
     const int cands[3] = {kGrpD4, kGrpV4, kGrpZ2};

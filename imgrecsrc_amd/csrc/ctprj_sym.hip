@@ -110,11 +110,27 @@ __device__ __forceinline__ float ldb(__amdgpu_buffer_rsrc_t r, int voff, int sof
     return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
 }
 
+// Four consecutive floats at a 4-byte aligned offset (one 16-byte load per lane).
+struct Quad {
+    float q[4];
+};
+
+__device__ __forceinline__ Quad ldb4(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    return Quad{{__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                 __uint_as_float(v[3])}};
+}
+
 // One ray of canonical view a0 against every image g p of the pixels p it meets, for
 // the elements g in MASK.  Rows mode walks image rows v = y with u = x inside; columns
 // mode (canonical views steeper than 45 degrees) walks v = x with u = y inside.  Each
 // image pixel g (x, y) is read from the image or its transpose at row +-v, column +-u,
-// whichever makes u the contiguous index.
+// whichever makes u the contiguous index.  The pixels [lo, hi] a ray meets in one row
+// are read as 4-pixel windows [s, s+3] (one 16-byte load per image instead of one load
+// per pixel and image); windows are clamped into [-H, H] so they never leave the row
+// (n >= 5), and pixels outside [lo, hi] or already taken by the previous window are
+// skipped.
 template <bool FAN, unsigned MASK, bool COLS>
 __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, float cT, float sT,
                                              const float* __restrict__ img,
@@ -139,26 +155,40 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
         const int op = (v + half) * n * 4;  // byte offset of row  v
         const int om = (half - v) * n * 4;  // byte offset of row -v
         const float fv = (float)v;
-        int cp = (half + lo) * 4, cm = (half - lo) * 4;  // columns +u, -u
-        for (int u = lo; u <= hi; ++u, cp += 4, cm -= 4) {
-            const float fu = (float)u;
-            const float fx = COLS ? fv : fu, fy = COLS ? fu : fv;
-            const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
-            const float hb = FAN ? r.hbc * fan_depth_scale(fx, fy, cT, sT, S.invd) : r.hbc;
-            const float w = strip_weight(dist, hb, r);
+        for (int s0 = lo; s0 <= hi; s0 += 4) {
+            const int s = min(s0, H - 3);
+            const int cp = (half + s) * 4;      // window [s, s+3]
+            const int cm = (half - s - 3) * 4;  // window [-(s+3), -s]
             // rows: e0 ( x, y) A[v][u]   e1 (-y, x) B[-v][u]  e2 (-x,-y) A[-v][-u]
             //       e3 ( y,-x) B[v][-u]  e4 ( x,-y) A[-v][u]  e5 ( y, x) B[v][u]
             //       e6 (-x, y) A[v][-u]  e7 (-y,-x) B[-v][-u]          (A = img, B = imgT)
             // cols: e0 A[v][u]  e1 B[v][-u]  e2 A[-v][-u]  e3 B[-v][u]
             //       e4 A[v][-u] e5 B[v][u]   e6 A[-v][u]   e7 B[-v][-u]  (A = imgT, B = img)
-            if (MASK & 0x01) part[0] = fmaf(w, ldb(RA, cp, op), part[0]);
-            if (MASK & 0x02) part[1] = fmaf(w, COLS ? ldb(RB, cm, op) : ldb(RB, cp, om), part[1]);
-            if (MASK & 0x04) part[2] = fmaf(w, ldb(RA, cm, om), part[2]);
-            if (MASK & 0x08) part[3] = fmaf(w, COLS ? ldb(RB, cp, om) : ldb(RB, cm, op), part[3]);
-            if (MASK & 0x10) part[4] = fmaf(w, COLS ? ldb(RA, cm, op) : ldb(RA, cp, om), part[4]);
-            if (MASK & 0x20) part[5] = fmaf(w, ldb(RB, cp, op), part[5]);
-            if (MASK & 0x40) part[6] = fmaf(w, COLS ? ldb(RA, cp, om) : ldb(RA, cm, op), part[6]);
-            if (MASK & 0x80) part[7] = fmaf(w, ldb(RB, cm, om), part[7]);
+            Quad w8[8];
+            if (MASK & 0x01) w8[0] = ldb4(RA, cp, op);
+            if (MASK & 0x02) w8[1] = COLS ? ldb4(RB, cm, op) : ldb4(RB, cp, om);
+            if (MASK & 0x04) w8[2] = ldb4(RA, cm, om);
+            if (MASK & 0x08) w8[3] = COLS ? ldb4(RB, cp, om) : ldb4(RB, cm, op);
+            if (MASK & 0x10) w8[4] = COLS ? ldb4(RA, cm, op) : ldb4(RA, cp, om);
+            if (MASK & 0x20) w8[5] = ldb4(RB, cp, op);
+            if (MASK & 0x40) w8[6] = COLS ? ldb4(RA, cp, om) : ldb4(RA, cm, op);
+            if (MASK & 0x80) w8[7] = ldb4(RB, cm, om);
+            // which images read column +u (window index j) and which -u (index 3 - j)
+            constexpr unsigned PLUS = COLS ? 0x69u : 0x33u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int u = s + j;
+                if (u < s0 || u > hi) continue;
+                const float fu = (float)u;
+                const float fx = COLS ? fv : fu, fy = COLS ? fu : fv;
+                const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
+                const float hb = FAN ? r.hbc * fan_depth_scale(fx, fy, cT, sT, S.invd) : r.hbc;
+                const float w = strip_weight(dist, hb, r);
+#pragma unroll
+                for (int e = 0; e < 8; ++e)
+                    if (MASK >> e & 1u)
+                        part[e] = fmaf(w, w8[e].q[(PLUS >> e & 1u) ? j : 3 - j], part[e]);
+            }
         }
         if (++inblock == kRowBlock || v == vhi) {
 #pragma unroll

@@ -32,7 +32,7 @@ def _proj(geo, flags=1):
 
 
 def _expected_mode(geo, flags):
-    return choose_group(geo[3], geo[2], geo[6] > 0, flags)[0]
+    return choose_group(geo[3], geo[2], geo[6] > 0, flags, n=geo[0])[0]
 
 
 # FP64 oracle angle mode matching a slice's kernels: per-view float angles on the

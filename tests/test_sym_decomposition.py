@@ -62,8 +62,10 @@ def symmetric_views(P, np_, grp, fold):
     return canon, (np_ / slots if slots else 0.0)
 
 
-def choose_group(P, np_, fan, flags=1):
-    if not flags & 1 or np_ > P:
+def choose_group(P, np_, fan, flags=1, n=None):
+    """The host's choice (ctprj_api.hip: choose_group); images under 5x5 (n given) stay
+    on the general kernels."""
+    if not flags & 1 or np_ > P or (n is not None and n < 5):
         return 0, False, []
     for grp in (D4, V4, Z2):
         if grp == D4 and (P % 4 or flags & 2): continue
@@ -217,3 +219,4 @@ def test_group_choice():
     # more views than prjFull, or the symmetric kernels switched off
     assert choose_group(360, 361, True)[0] == 0
     assert choose_group(360, 360, True, flags=0)[0] == 0
+    assert choose_group(8, 8, True, n=4)[0] == 0 and choose_group(8, 8, True, n=5)[0] == D4
