@@ -149,6 +149,7 @@ Plan::~Plan()
     free_ptr(d_fbp_items);
     free_ptr(d_border_ids);
     free_ptr(d_zero_row);
+    free_ptr(d_symtab);
     free_ptr(d_fbp_kern);
     free_ptr(d_fbp_kern_off);
     free_ptr(d_fbp_prew);
@@ -235,8 +236,10 @@ static int choose_group(const ctprj_geom& g, int flags, bool* fold, std::vector<
 {
     canon->clear();
     *fold = false;
-    // the symmetric forward reads 4-pixel windows inside each row: n >= 5
-    if (!(flags & 1) || g.n < 5 || g.prjFull < 1 || g.np > g.prjFull) return kGrpNone;
+    // the symmetric kernels read 4-wide windows inside image rows (n >= 5) and sinogram
+    // rows (prjWidth >= 4)
+    if (!(flags & 1) || g.n < 5 || g.prjWidth < 4 || g.prjFull < 1 || g.np > g.prjFull)
+        return kGrpNone;
     const bool fan = g.d > 0.0f;
     const int P = g.prjFull;
     const int cands[3] = {kGrpD4, kGrpV4, kGrpZ2};
@@ -254,16 +257,33 @@ static int choose_group(const ctprj_geom& g, int flags, bool* fold, std::vector<
     return kGrpNone;
 }
 
+// View of element (k, m) applied to canonical view a0 (the kernels' map_view), and
+// whether its bins are reflected.
+static int host_map_view(int P, bool fold, int k, int m, int a0, bool* flip)
+{
+    int v = ((m ? -4 * a0 : 4 * a0) + ((k + 2 * m) & 3) * P) / 4;
+    v %= P;
+    if (v < 0) v += P;
+    bool f = m != 0;
+    if (fold && 2 * v >= P) {
+        v -= P / 2;
+        f = !f;
+    }
+    *flip = f;
+    return v;
+}
+
 int Plan::build_tables()
 {
     free_ptr(d_desc);
+    free_ptr(d_symtab);
     free_ptr(d_views);
     free_ptr(d_rays);
     free_ptr(d_border_ids);
     std::vector<SliceDesc> desc(nslices);
     std::vector<float4> views, bins;
     std::vector<long long> bin_off(nslices), ray_prefix(nslices + 1, 0);
-    std::vector<int> border;
+    std::vector<int> border, symtab;
     long long nview = 0, nray = 0;
     slice_sym.assign(nslices, 0);
     n_sym = 0;
@@ -300,6 +320,22 @@ int Plan::build_tables()
             D.fold = fold;
             D.P = g.prjFull;
             D.ncanon = (int)canon.size();
+            D.tab_off = (long long)symtab.size();
+            const unsigned all = group_masks(grp).all;
+            for (int a0 : canon)
+                for (int e2 = 0; e2 < 8; ++e2)
+                    for (int h = 0; h < 8; ++h) {
+                        if (!(all >> e2 & 1u) || !(all >> h & 1u)) {
+                            symtab.push_back(kRowNone);
+                            continue;
+                        }
+                        const int k1 = h & 3, m1 = h >> 2, k2 = e2 & 3, m2 = e2 >> 2;
+                        bool flip = false;
+                        const int v = host_map_view(g.prjFull, fold, k1 + (m1 ? -k2 : k2),
+                                                    m1 ^ m2, a0, &flip);
+                        symtab.push_back(v < g.np ? v * g.prjWidth * 4 + (flip ? kRowFlip : 0)
+                                                  : kRowNone);
+                    }
             for (int a0 : canon) {
                 float c, si;
                 view_trig(a0, g.prjFull, &c, &si);
@@ -349,6 +385,7 @@ int Plan::build_tables()
         (rc = upload(&d_bins, bins, "bin table")) || (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
         (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
         (rc = upload(&d_border_ids, border, "border list")) ||
+        (rc = upload(&d_symtab, symtab, "symmetric view-row table")) ||
         (rc = upload(&d_tr_items, tr, "transpose work list")) ||
         (rc = dev_alloc((void**)&d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
         free_dev(d_bins);
@@ -367,9 +404,23 @@ int Plan::build_tables()
     return build_adj_items();
 }
 
+// Concatenates per-group lists (each XCD-ordered on its own) and records the segments.
+template <class Item>
+static std::vector<Item> group_segments(std::vector<Item> (&by)[4], int* start, int* count)
+{
+    std::vector<Item> out;
+    for (int g = 1; g < 4; ++g) {
+        std::vector<Item> seg = by[g].empty() ? by[g] : xcd_order(by[g]);
+        start[g] = (int)out.size();
+        count[g] = (int)seg.size();
+        out.insert(out.end(), seg.begin(), seg.end());
+    }
+    return out;
+}
+
 int Plan::build_fwd_items()
 {
-    std::vector<FwdItem> items, sym;
+    std::vector<FwdItem> items, by[4];
     const int B = tune.fwd_block;
     for (int s = 0; s < nslices; ++s) {
         const ctprj_geom& g = geoms[s];
@@ -380,14 +431,15 @@ int Plan::build_fwd_items()
             choose_group(g, tune.flags, &fold, &canon);
             const int bmax = fold ? pC : bhi;
             for (int i = 0; i < (int)canon.size(); ++i)
-                for (int b0 = 0; b0 <= bmax; b0 += 64) sym.push_back(FwdItem{s, i, b0, 0});
+                for (int b0 = 0; b0 <= bmax; b0 += 64)
+                    by[slice_sym[s]].push_back(FwdItem{s, i, b0, 0});
         } else {
             for (int a = 0; a < g.np; ++a)
                 for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
         }
     }
     items = xcd_order(items);
-    sym = xcd_order(sym);
+    std::vector<FwdItem> sym = group_segments(by, symf_start, symf_count);
     free_ptr(d_fwd_items);
     free_ptr(d_symf_items);
     n_fwd_items = (int)items.size();
@@ -399,22 +451,23 @@ int Plan::build_fwd_items()
 
 int Plan::build_adj_items()
 {
-    std::vector<AdjItem> items, sym;
+    std::vector<AdjItem> items, by[4];
     const int T = tune.adj_tile;
     for (int s = 0; s < nslices; ++s) {
         const int n = geoms[s].n;
-        if (slice_sym[s]) {
+        const int grp = slice_sym[s];
+        if (grp) {
             // 8x8 tiles of the group's fundamental domain of orbit base pixels
             const int H = (n - 1) / 2, nt = (H + 1 + 7) / 8;
-            if (slice_sym[s] == kGrpD4) {
+            if (grp == kGrpD4) {
                 for (int ty = 0; ty < nt; ++ty)
-                    for (int tx = ty; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
-            } else if (slice_sym[s] == kGrpV4) {
+                    for (int tx = ty; tx < nt; ++tx) by[grp].push_back(AdjItem{s, ty * 8, tx * 8, 0});
+            } else if (grp == kGrpV4) {
                 for (int ty = 0; ty < nt; ++ty)
-                    for (int tx = 0; tx < nt; ++tx) sym.push_back(AdjItem{s, ty * 8, tx * 8, 0});
+                    for (int tx = 0; tx < nt; ++tx) by[grp].push_back(AdjItem{s, ty * 8, tx * 8, 0});
             } else {
                 for (int y = -H; y <= H; y += 8)
-                    for (int tx = 0; tx < nt; ++tx) sym.push_back(AdjItem{s, y, tx * 8, 0});
+                    for (int tx = 0; tx < nt; ++tx) by[grp].push_back(AdjItem{s, y, tx * 8, 0});
             }
         } else {
             for (int r = 0; r < n; r += T)
@@ -422,7 +475,7 @@ int Plan::build_adj_items()
         }
     }
     items = xcd_order(items);
-    sym = xcd_order(sym);
+    std::vector<AdjItem> sym = group_segments(by, syma_start, syma_count);
     free_ptr(d_adj_items);
     free_ptr(d_syma_items);
     n_adj_items = (int)items.size();
@@ -563,9 +616,10 @@ int Plan::launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool
     if (n_fwd_items)
         launch_forward_kernel(count, tune.fwd_block, n_fwd_items, d_desc, d_fwd_items, d_views,
                               d_rays, d_img, d_imgT, d_sino, st);
-    if (n_symf_items)
-        launch_forward_sym(tune.fwd_seg, n_symf_items, d_desc, d_symf_items, d_views,
-                           d_rays, d_img, d_imgT, d_sino, st);
+    for (int g = 1; g < 4; ++g)
+        if (symf_count[g])
+            launch_forward_sym(g, tune.fwd_seg, symf_count[g], d_desc, d_symf_items + symf_start[g],
+                               d_views, d_rays, d_img, d_imgT, d_sino, st);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
 }
@@ -576,8 +630,11 @@ int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool
         launch_adjoint_kernel(tune.adj_tile, fbp, n_adj_items, d_desc, d_adj_items, d_views,
                               d_rays, d_sino, d_img, st);
     if (n_syma_items) {
-        launch_adjoint_sym(fbp, tune.adj_seg, n_syma_items, d_desc, d_syma_items, d_views, d_rays,
-                           d_sino, d_zero_row, d_img, st);
+        for (int g = 1; g < 4; ++g)
+            if (syma_count[g])
+                launch_adjoint_sym(fbp, g, tune.adj_seg, syma_count[g], d_desc,
+                                   d_syma_items + syma_start[g], d_views, d_rays, d_sino,
+                                   d_zero_row, d_symtab, d_img, st);
         launch_zero_border(n_border, d_desc, d_border_ids, d_img, st);
     }
     hipError_t e = hipGetLastError();

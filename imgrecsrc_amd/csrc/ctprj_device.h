@@ -25,6 +25,12 @@ constexpr int kBufferWord3 = 0x00020000;
 //   Z2 = {1, R^2 M} (theta -> -theta) for any prjFull.
 enum : int { kGrpNone = 0, kGrpD4 = 1, kGrpV4 = 2, kGrpZ2 = 3 };
 
+// View-row table of the symmetric adjoint: for canonical view i, element g' = e2 and
+// orbit element h, entry [i][e2][h] locates the sinogram row of view h g' a0 as
+// (byte offset of the row) | kRowFlip (bins read at 2pC - b) or kRowNone (view beyond np,
+// or h g' outside the slice's group).
+constexpr int kRowFlip = 1, kRowNone = 2;
+
 // Element sets of a group as bit masks over e, and its canonical view range.
 struct GroupMasks {
     unsigned all;        // the group's elements
@@ -61,6 +67,7 @@ struct SliceDesc {
     float er;              // effectiveRate (outputs are divided by it)
     float fbp_scale;       // PI/np/dSize^2 (prj/cpuPrj.c:161,304)
     long long img_off, sino_off, view_off, ray_off;
+    long long tab_off;     // symmetric slices: first entry of the view-row table
 };
 
 // Per-ray footprint constants.  Parallel beam: one entry per view (dc unused, the bin

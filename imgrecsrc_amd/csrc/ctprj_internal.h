@@ -26,7 +26,7 @@ struct Plan {
     int nslices = 0;
     std::vector<ctprj_geom> geoms;
     std::vector<long long> img_offsets, sino_offsets;
-    std::vector<int> slice_sym;  // per slice: 1 = 8-fold symmetric kernels
+    std::vector<int> slice_sym;  // per slice: symmetry group (kGrp*), 0 = general kernels
     long long img_elems = 0, sino_elems = 0;
     int max_W = 0;
     ctprj_tune tune{64, 16, 1, 8, 8};
@@ -42,11 +42,16 @@ struct Plan {
     int2* d_fbp_items = nullptr;
     int* d_border_ids = nullptr;      // symmetric slices of even n
     float* d_zero_row = nullptr;      // max_W zeros: reads of views beyond np
+    int* d_symtab = nullptr;          // view-row table of the symmetric adjoint
     float* d_fbp_kern = nullptr;
     long long* d_fbp_kern_off = nullptr;
     float* d_fbp_prew = nullptr;
     long long* d_fbp_prew_off = nullptr;
     int n_fwd_items = 0, n_adj_items = 0, n_symf_items = 0, n_syma_items = 0;
+    // symmetric work lists are grouped by symmetry group: [start, start + count) of
+    // group g (1..3) in d_symf_items / d_syma_items
+    int symf_start[4] = {0, 0, 0, 0}, symf_count[4] = {0, 0, 0, 0};
+    int syma_start[4] = {0, 0, 0, 0}, syma_count[4] = {0, 0, 0, 0};
     int n_tr_items = 0, n_fbp_items = 0, n_border = 0, n_sym = 0;
 
     float* d_imgT = nullptr;     // transposed image (forward, steep views / symmetric images)
@@ -81,13 +86,14 @@ void launch_fbp_filter(int nitems, size_t shmem, const SliceDesc* desc, const in
                        const long long* prew_off, const float* sino, float* out,
                        hipStream_t st);
 void launch_scale(float* p, long long n, float s, hipStream_t st);
-// 8-fold symmetric kernels (ctprj_sym.hip)
-void launch_forward_sym(int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
+// symmetric kernels (ctprj_sym.hip), one launch per symmetry group
+void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
                         const float4* views, const Ray* rays, const float* img,
                         const float* imgT, float* sino, hipStream_t st);
-void launch_adjoint_sym(bool fbp, int seg, int nitems, const SliceDesc* desc,
+void launch_adjoint_sym(bool fbp, int grp, int seg, int nitems, const SliceDesc* desc,
                         const AdjItem* items, const float4* views, const Ray* rays,
-                        const float* sino, const float* zero_row, float* img, hipStream_t st);
+                        const float* sino, const float* zero_row, const int* symtab, float* img,
+                        hipStream_t st);
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
                         hipStream_t st);
 

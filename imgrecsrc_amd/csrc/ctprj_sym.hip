@@ -265,40 +265,35 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
     }
 }
 
-template <bool FAN, int SEG>
-__device__ __forceinline__ void fwd_sym_dispatch(const SliceDesc& S, const FwdItem& it, float cT,
-                                                 float sT, int a0, bool mirror,
-                                                 const Ray* __restrict__ rays,
-                                                 const float* __restrict__ img,
-                                                 const float* __restrict__ imgT,
-                                                 float* __restrict__ sino, float (*red)[8][64])
+template <bool FAN, int SEG, int GRP>
+__device__ __forceinline__ void fwd_sym_group(const SliceDesc& S, const FwdItem& it, float cT,
+                                              float sT, int a0, bool mirror,
+                                              const Ray* __restrict__ rays,
+                                              const float* __restrict__ img,
+                                              const float* __restrict__ imgT,
+                                              float* __restrict__ sino, float (*red)[8][64])
 {
     // D4's canonical views lie in [0, 45] degrees and always walk rows; V4's reach 90 and
     // Z2's 180 degrees, and the steep ones walk columns
-    const bool cols = fabsf(cT) < fabsf(sT);
+    constexpr GroupMasks GM = group_masks(GRP);
+    const bool cols = GRP != kGrpD4 && fabsf(cT) < fabsf(sT);
 #define CTPRJ_RUN(MASK, COLS) \
     fwd_sym_run<FAN, MASK, COLS, SEG>(S, it, cT, sT, a0, rays, img, imgT, sino, red)
-#define CTPRJ_GROUP(G, BOTH)                                                    \
-    do {                                                                        \
-        constexpr GroupMasks GM = group_masks(G);                               \
-        if (BOTH && cols) {                                                     \
-            if (mirror) CTPRJ_RUN(GM.rot, true);                                \
-            else CTPRJ_RUN(GM.all, true);                                       \
-        } else {                                                                \
-            if (mirror) CTPRJ_RUN(GM.rot, false);                               \
-            else CTPRJ_RUN(GM.all, false);                                      \
-        }                                                                       \
-    } while (0)
-    if (S.sym == kGrpD4) CTPRJ_GROUP(kGrpD4, false);
-    else if (S.sym == kGrpV4) CTPRJ_GROUP(kGrpV4, true);
-    else CTPRJ_GROUP(kGrpZ2, true);
-#undef CTPRJ_GROUP
+    if (GRP != kGrpD4 && cols) {
+        if (mirror) CTPRJ_RUN(GM.rot, GRP != kGrpD4);
+        else CTPRJ_RUN(GM.all, GRP != kGrpD4);
+    } else {
+        if (mirror) CTPRJ_RUN(GM.rot, false);
+        else CTPRJ_RUN(GM.all, false);
+    }
 #undef CTPRJ_RUN
 }
 
-// One work item: (slice, canonical view index, first bin of 64).  SEG waves share the
-// 64 rays, each walking 1/SEG of the image rows; wave 0 sums the partials in order.
-template <int SEG>
+// One work item: (slice, canonical view index, first bin of 64) of a slice under group
+// GRP (one launch per group keeps each variant's registers to its own images).  SEG
+// waves share the 64 rays, each walking 1/SEG of the image rows; wave 0 sums the
+// partials in order.
+template <int SEG, int GRP>
 __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __restrict__ slices,
                                                           const FwdItem* __restrict__ items,
                                                           const float4* __restrict__ views,
@@ -313,11 +308,11 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
     const SliceDesc S = slices[it.slice];
     const float4 T = views[S.view_off + it.view];
     const int a0 = __float_as_int(T.z);
-    const bool mirror = mirror_view(S, a0);
-    // (fan, group, mirror) are uniform over the work-group: the barrier inside is
-    // reached by all of its waves
-    if (S.fan) fwd_sym_dispatch<true, SEG>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
-    else fwd_sym_dispatch<false, SEG>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
+    const bool mirror = a0 == 0 || group_masks(GRP).q * a0 == S.P;
+    // (fan, mirror, walk) are uniform over the work-group: the barrier inside is reached
+    // by all of its waves
+    if (S.fan) fwd_sym_group<true, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
+    else fwd_sym_group<false, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
 }
 
 // ------------------------------------------------------------------ adjoint ---
@@ -327,11 +322,14 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
 // h g' (a0, b).  On a mirror view rotations g' pair with rotation h and reflections
 // with reflection h, so that h g' is always a rotation: every entry then takes the
 // forward's key (a0, b, R^-k q) and the two directions evaluate the same float weight.
+// `tab` holds the view-row entries of (a0, g') for the eight h (wave-uniform).  The
+// bins [blo, bhi] of the pixel are read as 4-bin windows (one 16-byte load per image),
+// clamped into [0, W - 4] so they stay inside the row, reflected rows at 2pC - b.
 template <bool FAN, bool FBP, unsigned HMASK>
 __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __restrict__ rview,
                                              __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
-                                             float cT, float sT, int a0, int k2, int m2, int x0,
-                                             int y0, float* vpart)
+                                             const int* __restrict__ tab, float cT, float sT,
+                                             int k2, int m2, int x0, int y0, float* vpart)
 {
     // p = g'^-1 q0; (R^k M)^-1 = R^k M, (R^k)^-1 = R^-k
     int px, py;
@@ -341,41 +339,47 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
     int blo, bhi;
     pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
     if (blo > bhi) return;
-    // view rows of the elements h g' (wave-uniform): the sinogram row, or the zero row for
-    // views beyond np; bins read at b or, for reflections, at 2pC - b
     __amdgpu_buffer_rsrc_t rs[8];
     int soff[8];
     bool flip[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
         if (!(HMASK >> h & 1u)) continue;
-        const int k1 = h & 3, m1 = h >> 2;
-        const int k = k1 + (m1 ? -k2 : k2), m = m1 ^ m2;
-        int sg, off;
-        const int v = map_view(S, k, m, a0, sg, off);
-        const bool ok = v < S.np;
-        rs[h] = ok ? RS : RZ;
-        soff[h] = ok ? v * S.W * 4 : 0;
-        flip[h] = ok && sg < 0;
+        const int t = tab[h];
+        const bool none = (t & kRowNone) != 0;
+        rs[h] = none ? RZ : RS;
+        soff[h] = none ? 0 : (t & ~(kRowFlip | kRowNone));
+        flip[h] = (t & kRowFlip) != 0;
     }
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     float part[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) part[h] = 0.0f;
-    const int two_pc = 2 * S.pC;
-    for (int b = blo; b <= bhi; ++b) {
-        Ray r;
-        if (FAN) r = rview[b];
-        else {
-            r = rview[0];
-            r.dc = (float)(b - S.pC) * S.dSize;
-        }
-        const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
-        const float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
-        const int ob = b * 4, of = (two_pc - b) * 4;
+    const int two_pc = 2 * S.pC, smax = S.W - 4;
+    for (int s0 = blo; s0 <= bhi; s0 += 4) {
+        const int s = min(s0, smax);
+        const int ob = s * 4, of = (two_pc - s - 3) * 4;  // windows [s, s+3], [2pC-s-3, 2pC-s]
+        Quad q[8];
 #pragma unroll
         for (int h = 0; h < 8; ++h)
-            if (HMASK >> h & 1u) part[h] = fmaf(w, ldb(rs[h], flip[h] ? of : ob, soff[h]), part[h]);
+            if (HMASK >> h & 1u) q[h] = ldb4(rs[h], flip[h] ? of : ob, soff[h]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = s + j;
+            if (b < s0 || b > bhi) continue;
+            Ray r;
+            if (FAN) r = rview[b];
+            else {
+                r = rview[0];
+                r.dc = (float)(b - S.pC) * S.dSize;
+            }
+            const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
+            const float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                if (HMASK >> h & 1u)
+                    part[h] = fmaf(w, flip[h] ? q[h].q[3 - j] : q[h].q[j], part[h]);
+        }
     }
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258); qe of g'^-1 q0 at a0 is the depth of
     // every orbit pixel at its own view
@@ -390,7 +394,8 @@ template <bool FAN, bool FBP, int GRP>
 __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* __restrict__ views,
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ sino,
-                                              const float* __restrict__ zero_row, int x0, int y0,
+                                              const float* __restrict__ zero_row,
+                                              const int* __restrict__ symtab, int x0, int y0,
                                               int i0, int step, KahanAcc* acc)
 {
     constexpr GroupMasks G = group_masks(GRP);
@@ -401,6 +406,7 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
         const float4 T = views[S.view_off + i];
         const int a0 = __float_as_int(T.z);
         const Ray* rview = FAN ? rays + S.ray_off + (long long)i * S.W : rays + S.ray_off + i;
+        const int* tv = symtab + S.tab_off + (long long)i * 64;
         float vpart[8];  // this view's sums; compensated once per view
 #pragma unroll
         for (int h = 0; h < 8; ++h) vpart[h] = 0.0f;
@@ -411,12 +417,12 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
 #pragma unroll 1
             for (int e = 0; e < 8; ++e)
                 if (grot >> e & 1u)
-                    adj_sym_elem<FAN, FBP, G.rot>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                    adj_sym_elem<FAN, FBP, G.rot>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
                                                   e >> 2, x0, y0, vpart);
 #pragma unroll 1
             for (int e = 0; e < 8; ++e)
                 if (gref >> e & 1u)
-                    adj_sym_elem<FAN, FBP, REF>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                    adj_sym_elem<FAN, FBP, REF>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
                                                 e >> 2, x0, y0, vpart);
         } else {
             // a half-turn parallel scan takes the half set that maps the canonical range
@@ -425,7 +431,7 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
 #pragma unroll 1
             for (int e = 0; e < 8; ++e)
                 if (gset >> e & 1u)
-                    adj_sym_elem<FAN, FBP, G.all>(S, rview, RS, RZ, T.x, T.y, a0, e & 3,
+                    adj_sym_elem<FAN, FBP, G.all>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
                                                   e >> 2, x0, y0, vpart);
         }
 #pragma unroll
@@ -443,6 +449,7 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
                                              const Ray* __restrict__ rays,
                                              const float* __restrict__ sino,
                                              const float* __restrict__ zero_row,
+                                             const int* __restrict__ symtab,
                                              float* __restrict__ img, float (*red)[8][64])
 {
     constexpr GroupMasks G = group_masks(GRP);
@@ -454,8 +461,12 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
     const bool valid = x0 <= S.H && y0 <= (GRP == kGrpD4 ? x0 : S.H);
     KahanAcc acc[8];
     if (valid) {
-        if (S.fan) adj_sym_pixel<true, FBP, GRP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
-        else adj_sym_pixel<false, FBP, GRP>(S, views, rays, sino, zero_row, x0, y0, wave, SEG, acc);
+        if (S.fan)
+            adj_sym_pixel<true, FBP, GRP>(S, views, rays, sino, zero_row, symtab, x0, y0, wave,
+                                          SEG, acc);
+        else
+            adj_sym_pixel<false, FBP, GRP>(S, views, rays, sino, zero_row, symtab, x0, y0, wave,
+                                           SEG, acc);
     }
     if (SEG > 1) {
 #pragma unroll
@@ -492,25 +503,23 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
     }
 }
 
-// SEG waves split the canonical views (wave w takes i = w, w + SEG, ...); wave 0 sums
-// their partials in order.
-template <bool FBP, int SEG>
+// One launch per symmetry group; SEG waves split the canonical views (wave w takes
+// i = w, w + SEG, ...); wave 0 sums their partials in order.
+template <bool FBP, int SEG, int GRP>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
                                                           const Ray* __restrict__ rays,
                                                           const float* __restrict__ sino,
                                                           const float* __restrict__ zero_row,
+                                                          const int* __restrict__ symtab,
                                                           float* __restrict__ img)
 {
     __shared__ float red[SEG][8][64];
     const AdjItem it = items[blockIdx.x];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
-    // the group is uniform over the work-group: the barrier inside is reached by all
-    if (S.sym == kGrpD4) adj_sym_tile<FBP, SEG, kGrpD4>(S, it, views, rays, sino, zero_row, img, red);
-    else if (S.sym == kGrpV4) adj_sym_tile<FBP, SEG, kGrpV4>(S, it, views, rays, sino, zero_row, img, red);
-    else adj_sym_tile<FBP, SEG, kGrpZ2>(S, it, views, rays, sino, zero_row, img, red);
+    adj_sym_tile<FBP, SEG, GRP>(S, it, views, rays, sino, zero_row, symtab, img, red);
 }
 
 // Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the
@@ -528,13 +537,14 @@ __global__ void k_zero_border(const SliceDesc* __restrict__ slices, const int* _
 
 // ----------------------------------------------------------------- launchers ---
 
-void launch_forward_sym(int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
-                        const float4* views, const Ray* rays, const float* img,
-                        const float* imgT, float* sino, hipStream_t st)
+template <int GRP>
+static void launch_forward_sym_g(int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
+                                 const float4* views, const Ray* rays, const float* img,
+                                 const float* imgT, float* sino, hipStream_t st)
 {
-#define CTPRJ_LAUNCH(SEG)                                                                   \
-    hipLaunchKernelGGL(k_forward_sym<SEG>, dim3(nitems), dim3(64 * SEG), 0, st, desc, items, \
-                       views, rays, img, imgT, sino)
+#define CTPRJ_LAUNCH(SEG)                                                                     \
+    hipLaunchKernelGGL((k_forward_sym<SEG, GRP>), dim3(nitems), dim3(64 * SEG), 0, st, desc,   \
+                       items, views, rays, img, imgT, sino)
     switch (seg) {
         case 1: CTPRJ_LAUNCH(1); break;
         case 2: CTPRJ_LAUNCH(2); break;
@@ -545,37 +555,64 @@ void launch_forward_sym(int seg, int nitems, const SliceDesc* desc, const FwdIte
 #undef CTPRJ_LAUNCH
 }
 
-template <bool FBP>
-static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, const AdjItem* items,
-                                 const float4* views, const Ray* rays, const float* sino,
-                                 const float* zero_row, float* img, hipStream_t st)
+void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
+                        const float4* views, const Ray* rays, const float* img,
+                        const float* imgT, float* sino, hipStream_t st)
 {
-    switch (seg) {
-        case 1:
-            hipLaunchKernelGGL((k_adjoint_sym<FBP, 1>), dim3(nitems), dim3(64), 0, st, desc, items,
-                               views, rays, sino, zero_row, img);
-            break;
-        case 2:
-            hipLaunchKernelGGL((k_adjoint_sym<FBP, 2>), dim3(nitems), dim3(128), 0, st, desc,
-                               items, views, rays, sino, zero_row, img);
-            break;
-        case 4:
-            hipLaunchKernelGGL((k_adjoint_sym<FBP, 4>), dim3(nitems), dim3(256), 0, st, desc,
-                               items, views, rays, sino, zero_row, img);
-            break;
-        default:
-            hipLaunchKernelGGL((k_adjoint_sym<FBP, 8>), dim3(nitems), dim3(512), 0, st, desc,
-                               items, views, rays, sino, zero_row, img);
-            break;
-    }
+    if (grp == kGrpD4)
+        launch_forward_sym_g<kGrpD4>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
+    else if (grp == kGrpV4)
+        launch_forward_sym_g<kGrpV4>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
+    else
+        launch_forward_sym_g<kGrpZ2>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
 }
 
-void launch_adjoint_sym(bool fbp, int seg, int nitems, const SliceDesc* desc,
-                        const AdjItem* items, const float4* views, const Ray* rays,
-                        const float* sino, const float* zero_row, float* img, hipStream_t st)
+template <bool FBP, int GRP>
+static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, const AdjItem* items,
+                                 const float4* views, const Ray* rays, const float* sino,
+                                 const float* zero_row, const int* symtab, float* img,
+                                 hipStream_t st)
 {
-    if (fbp) launch_adjoint_sym_t<true>(seg, nitems, desc, items, views, rays, sino, zero_row, img, st);
-    else launch_adjoint_sym_t<false>(seg, nitems, desc, items, views, rays, sino, zero_row, img, st);
+#define CTPRJ_LAUNCH(SEG)                                                                   \
+    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP>), dim3(nitems), dim3(64 * SEG), 0, st, \
+                       desc, items, views, rays, sino, zero_row, symtab, img)
+    switch (seg) {
+        case 1: CTPRJ_LAUNCH(1); break;
+        case 2: CTPRJ_LAUNCH(2); break;
+        case 4: CTPRJ_LAUNCH(4); break;
+        default: CTPRJ_LAUNCH(8); break;
+    }
+#undef CTPRJ_LAUNCH
+}
+
+template <bool FBP>
+static void launch_adjoint_sym_f(int grp, int seg, int nitems, const SliceDesc* desc,
+                                 const AdjItem* items, const float4* views, const Ray* rays,
+                                 const float* sino, const float* zero_row, const int* symtab,
+                                 float* img, hipStream_t st)
+{
+    if (grp == kGrpD4)
+        launch_adjoint_sym_t<FBP, kGrpD4>(seg, nitems, desc, items, views, rays, sino, zero_row,
+                                          symtab, img, st);
+    else if (grp == kGrpV4)
+        launch_adjoint_sym_t<FBP, kGrpV4>(seg, nitems, desc, items, views, rays, sino, zero_row,
+                                          symtab, img, st);
+    else
+        launch_adjoint_sym_t<FBP, kGrpZ2>(seg, nitems, desc, items, views, rays, sino, zero_row,
+                                          symtab, img, st);
+}
+
+void launch_adjoint_sym(bool fbp, int grp, int seg, int nitems, const SliceDesc* desc,
+                        const AdjItem* items, const float4* views, const Ray* rays,
+                        const float* sino, const float* zero_row, const int* symtab, float* img,
+                        hipStream_t st)
+{
+    if (fbp)
+        launch_adjoint_sym_f<true>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                   symtab, img, st);
+    else
+        launch_adjoint_sym_f<false>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                    symtab, img, st);
 }
 
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
