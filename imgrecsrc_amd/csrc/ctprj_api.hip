@@ -333,6 +333,12 @@ int Plan::build_tables()
                         bool flip = false;
                         const int v = host_map_view(g.prjFull, fold, k1 + (m1 ? -k2 : k2),
                                                     m1 ^ m2, a0, &flip);
+                        // the kernels take the reflection of off-mirror views from
+                        // elem_flip() at compile time
+                        const bool mirror = a0 == 0 || group_masks(grp).q * a0 == g.prjFull;
+                        if (!mirror && flip != elem_flip(h, e2, fold))
+                            return fail(CTPRJ_EINVAL, "internal: reflection table mismatch "
+                                        "(slice %d, view %d, elements %d/%d)", s, a0, e2, h);
                         symtab.push_back(v < g.np ? v * g.prjWidth * 4 + (flip ? kRowFlip : 0)
                                                   : kRowNone);
                     }

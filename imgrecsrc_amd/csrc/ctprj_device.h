@@ -42,6 +42,19 @@ struct GroupMasks {
     int order;           // number of elements
 };
 
+// Whether view h g' a0 reads its bins reflected (at 2pC - b), for a canonical view a0
+// off the mirror lines: element (k, m) = h g' reflects the bins when m = 1, and a
+// half-turn parallel scan folds views in [P/2, P) onto [0, P/2) with one more
+// reflection.  For 0 < a0 < P/q the folded views are, for every group,
+//   m = 0: k >= 2;   m = 1: k = 1 or 2.
+__host__ __device__ constexpr bool elem_flip(int h, int e2, bool fold)
+{
+    const int k1 = h & 3, m1 = h >> 2, k2 = e2 & 3, m2 = e2 >> 2;
+    const int k = (k1 + (m1 ? -k2 : k2)) & 3, m = m1 ^ m2;
+    const bool folded = m ? (k == 1 || k == 2) : (k >= 2);
+    return (m != 0) != (fold && folded);
+}
+
 __host__ __device__ constexpr GroupMasks group_masks(int grp)
 {
     // D4 fold set {1, R, M, R^3 M} maps (0, 45) degrees into [0, 180); V4's {1, M} maps
@@ -108,6 +121,10 @@ __device__ __forceinline__ float strip_weight(float dist, float hb, const Ray& r
     const float gm = gcum(-hb - ad, r.b1, b2, r.s, b1x2, r.k);
     return r.h * (gp - gm);
 }
+
+// Hardware reciprocal (about 1 ulp): used only where a result selects candidates and
+// is padded by kSupportMargin, never in a weight.
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
 // Signed distance of pixel (x, y) from the ray line.
 __device__ __forceinline__ float ray_dist(float x, float y, float cosR, float sinR, float dc)

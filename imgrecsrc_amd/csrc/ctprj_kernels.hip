@@ -176,10 +176,10 @@ __device__ __forceinline__ float adjoint_pixel(const SliceDesc& S, int x, int y,
             const float qe = fmaf(fx, sT, fmaf(-fy, cT, S.d));
             const float hc = 0.5f * (cT + sT), hd = 0.5f * (cT - sT);
             // corners (x-.5,y-.5),(x+.5,y-.5),(x+.5,y+.5),(x-.5,y+.5)
-            const float t0 = __fdividef(S.d * (oc - hc), qe + hd);
-            const float t1 = __fdividef(S.d * (oc + hd), qe + hc);
-            const float t2 = __fdividef(S.d * (oc + hc), qe - hd);
-            const float t3 = __fdividef(S.d * (oc - hd), qe - hc);
+            const float t0 = S.d * (oc - hc) * fast_rcp(qe + hd);
+            const float t1 = S.d * (oc + hd) * fast_rcp(qe + hc);
+            const float t2 = S.d * (oc + hc) * fast_rcp(qe - hd);
+            const float t3 = S.d * (oc - hd) * fast_rcp(qe - hc);
             const float tl = fminf(fminf(t0, t1), fminf(t2, t3));
             const float tr = fmaxf(fmaxf(t0, t1), fmaxf(t2, t3));
             const float m = S.hbd + kSupportMargin * S.dSize;

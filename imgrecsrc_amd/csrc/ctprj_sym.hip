@@ -69,10 +69,10 @@ __device__ __forceinline__ void pixel_bins(const SliceDesc& S, const Ray* rays_v
         const float oc = fmaf(fx, cT, fy * sT);
         const float qe = fmaf(fx, sT, fmaf(-fy, cT, S.d));
         const float hc = 0.5f * (cT + sT), hd = 0.5f * (cT - sT);
-        const float t0 = __fdividef(S.d * (oc - hc), qe + hd);
-        const float t1 = __fdividef(S.d * (oc + hd), qe + hc);
-        const float t2 = __fdividef(S.d * (oc + hc), qe - hd);
-        const float t3 = __fdividef(S.d * (oc - hd), qe - hc);
+        const float t0 = S.d * (oc - hc) * fast_rcp(qe + hd);
+        const float t1 = S.d * (oc + hd) * fast_rcp(qe + hc);
+        const float t2 = S.d * (oc + hc) * fast_rcp(qe - hd);
+        const float t3 = S.d * (oc - hd) * fast_rcp(qe - hc);
         const float tl = fminf(fminf(t0, t1), fminf(t2, t3));
         const float tr = fmaxf(fmaxf(t0, t1), fmaxf(t2, t3));
         const float m = S.hbd + kSupportMargin * S.dSize;
@@ -389,6 +389,90 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
         if (HMASK >> h & 1u) vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
 }
 
+// The same gather for a canonical view off the mirror lines and a compile-time element
+// g' = E2: the reflection of every view row is fixed (elem_flip), so windows and bins are
+// indexed without selects.  Views beyond np read row 0 and their sums are discarded.
+template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD>
+__device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* __restrict__ rview,
+                                                __amdgpu_buffer_rsrc_t RS,
+                                                const int* __restrict__ tab, float cT, float sT,
+                                                int x0, int y0, float* vpart)
+{
+    constexpr int K2 = E2 & 3, M2 = E2 >> 2;
+    int px, py;
+    if (M2) apply_px(K2, 1, x0, y0, px, py);
+    else apply_px(4 - K2, 0, x0, y0, px, py);
+    const float fx = (float)px, fy = (float)py;
+    int blo, bhi;
+    pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
+    if (blo > bhi) return;
+    int soff[8];
+    unsigned none = 0;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        if (!(HMASK >> h & 1u)) continue;
+        const int t = tab[h];
+        soff[h] = (t & kRowNone) ? 0 : (t & ~(kRowFlip | kRowNone));
+        if (t & kRowNone) none |= 1u << h;
+    }
+    const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
+    float part[8];
+#pragma unroll
+    for (int h = 0; h < 8; ++h) part[h] = 0.0f;
+    const int two_pc = 2 * S.pC, smax = S.W - 4;
+    for (int s0 = blo; s0 <= bhi; s0 += 4) {
+        const int s = min(s0, smax);
+        const int ob = s * 4, of = (two_pc - s - 3) * 4;  // windows [s, s+3], [2pC-s-3, 2pC-s]
+        Quad q[8];
+#pragma unroll
+        for (int h = 0; h < 8; ++h)
+            if (HMASK >> h & 1u) q[h] = ldb4(RS, elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int b = s + j;
+            if (b < s0 || b > bhi) continue;
+            Ray r;
+            if (FAN) r = rview[b];
+            else {
+                r = rview[0];
+                r.dc = (float)(b - S.pC) * S.dSize;
+            }
+            const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
+            const float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                if (HMASK >> h & 1u)
+                    part[h] = fmaf(w, q[h].q[elem_flip(h, E2, FOLD) ? 3 - j : j], part[h]);
+        }
+    }
+    if (none) {  // wave-uniform
+#pragma unroll
+        for (int h = 0; h < 8; ++h)
+            if ((HMASK >> h & 1u) && (none >> h & 1u)) part[h] = 0.0f;
+    }
+    const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
+#pragma unroll
+    for (int h = 0; h < 8; ++h)
+        if (HMASK >> h & 1u) vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
+}
+
+// All elements of a canonical view off the mirror lines, unrolled at compile time.
+template <bool FAN, bool FBP, int GRP, bool FOLD>
+__device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* __restrict__ rview,
+                                                __amdgpu_buffer_rsrc_t RS,
+                                                const int* __restrict__ tv, float cT, float sT,
+                                                int x0, int y0, float* vpart)
+{
+    constexpr GroupMasks G = group_masks(GRP);
+    constexpr unsigned GSET = FOLD ? G.fold : G.all;
+#define CTPRJ_E(E2)                                                                        \
+    if constexpr ((GSET >> E2) & 1u)                                                        \
+        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD>(S, rview, RS, tv + 8 * E2, cT, sT, x0, y0, \
+                                                   vpart);
+    CTPRJ_E(0) CTPRJ_E(1) CTPRJ_E(2) CTPRJ_E(3) CTPRJ_E(4) CTPRJ_E(5) CTPRJ_E(6) CTPRJ_E(7)
+#undef CTPRJ_E
+}
+
 // Canonical views i = i0, i0 + step, ... for the orbit of (x0, y0) under group GRP.
 template <bool FAN, bool FBP, int GRP>
 __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* __restrict__ views,
@@ -424,15 +508,13 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
                 if (gref >> e & 1u)
                     adj_sym_elem<FAN, FBP, REF>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
                                                 e >> 2, x0, y0, vpart);
-        } else {
+        } else if (GRP != kGrpZ2 && S.fold) {
             // a half-turn parallel scan takes the half set that maps the canonical range
             // into [0, 180) and reads the rest folded
-            const unsigned gset = S.fold ? G.fold : G.all;
-#pragma unroll 1
-            for (int e = 0; e < 8; ++e)
-                if (gset >> e & 1u)
-                    adj_sym_elem<FAN, FBP, G.all>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
-                                                  e >> 2, x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2>(S, rview, RS, tv, T.x, T.y, x0, y0,
+                                                         vpart);
+        } else {
+            adj_sym_view_ct<FAN, FBP, GRP, false>(S, rview, RS, tv, T.x, T.y, x0, y0, vpart);
         }
 #pragma unroll
         for (int h = 0; h < 8; ++h)
