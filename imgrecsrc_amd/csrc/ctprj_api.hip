@@ -199,6 +199,28 @@ static std::vector<Item> xcd_order(const std::vector<Item>& in)
     return out;
 }
 
+// Batch order: each slice's items XCD-ordered on their own (so every slice spreads evenly
+// over the eight XCDs and one XCD never holds all the heavy slices of a ragged batch),
+// slices in descending order of work (n^2 np) so the light ones fill the tail.
+template <class Item>
+static std::vector<Item> batch_order(const std::vector<Item>& in,
+                                     const std::vector<ctprj_geom>& geoms)
+{
+    std::vector<std::vector<Item>> per(geoms.size());
+    for (const Item& it : in) per[it.slice].push_back(it);
+    std::vector<int> order;
+    for (int s = 0; s < (int)geoms.size(); ++s)
+        if (!per[s].empty()) order.push_back(s);
+    auto work = [&](int s) { return (double)geoms[s].n * geoms[s].n * geoms[s].np; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
+    std::vector<Item> out;
+    for (int s : order) {
+        const std::vector<Item> seg = xcd_order(per[s]);
+        out.insert(out.end(), seg.begin(), seg.end());
+    }
+    return out;
+}
+
 // Canonical views of the symmetric decomposition under group `grp`: a0 in [0, P/q]
 // whose images (g a0, folded for a half-turn parallel scan) include a computed view.
 // Returns the share of useful view evaluations (1 for a full or a half-turn scan).
@@ -410,13 +432,15 @@ int Plan::build_tables()
     return build_adj_items();
 }
 
-// Concatenates per-group lists (each XCD-ordered on its own) and records the segments.
+// Concatenates per-group lists (each in batch order) and records the segments.
 template <class Item>
-static std::vector<Item> group_segments(std::vector<Item> (&by)[4], int* start, int* count)
+static std::vector<Item> group_segments(std::vector<Item> (&by)[4],
+                                        const std::vector<ctprj_geom>& geoms, int* start,
+                                        int* count)
 {
     std::vector<Item> out;
     for (int g = 1; g < 4; ++g) {
-        std::vector<Item> seg = by[g].empty() ? by[g] : xcd_order(by[g]);
+        std::vector<Item> seg = batch_order(by[g], geoms);
         start[g] = (int)out.size();
         count[g] = (int)seg.size();
         out.insert(out.end(), seg.begin(), seg.end());
@@ -444,8 +468,8 @@ int Plan::build_fwd_items()
                 for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
         }
     }
-    items = xcd_order(items);
-    std::vector<FwdItem> sym = group_segments(by, symf_start, symf_count);
+    items = batch_order(items, geoms);
+    std::vector<FwdItem> sym = group_segments(by, geoms, symf_start, symf_count);
     free_ptr(d_fwd_items);
     free_ptr(d_symf_items);
     n_fwd_items = (int)items.size();
@@ -480,8 +504,8 @@ int Plan::build_adj_items()
                 for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
         }
     }
-    items = xcd_order(items);
-    std::vector<AdjItem> sym = group_segments(by, syma_start, syma_count);
+    items = batch_order(items, geoms);
+    std::vector<AdjItem> sym = group_segments(by, geoms, syma_start, syma_count);
     free_ptr(d_adj_items);
     free_ptr(d_syma_items);
     n_adj_items = (int)items.size();
