@@ -183,41 +183,55 @@ static int dev_alloc(void** p, size_t bytes, const char* what)
     return 0;
 }
 
-// XCD-aware order: the hardware hands work-group i to XCD i % 8.  Lay the item list
-// out so that XCD k runs a contiguous eighth of it (neighbouring views of one slice
-// share that XCD's L2).  Pads with empty items (slice = -1) to a multiple of 8.
-template <class Item>
-static std::vector<Item> xcd_order(const std::vector<Item>& in)
-{
-    const size_t G = (in.size() + 7) / 8 * 8, per = G / 8;
-    std::vector<Item> out(G);
-    for (size_t i = 0; i < G; ++i) {
-        const size_t src = (i % 8) * per + i / 8;
-        if (src < in.size()) out[i] = in[src];
-        else { out[i] = Item{}; out[i].slice = -1; }
-    }
-    return out;
-}
-
-// Batch order: each slice's items XCD-ordered on their own (so every slice spreads evenly
-// over the eight XCDs and one XCD never holds all the heavy slices of a ragged batch),
-// slices in descending order of work (n^2 np) so the light ones fill the tail.
+// Batch order.  The hardware hands work-group i to XCD i % 8, so the list is built as
+// eight per-XCD queues interleaved (queue k at positions k, k + 8, ...).  Slices go to
+// queues heaviest first (work n^2 np).  `spread` (the default, tune flag bit 3 clear):
+// every slice is split into contiguous eighths, one per queue, which balances the XCDs
+// whatever the per-slice cost.  Otherwise only slices above a quarter of one XCD's
+// share are split and the others go whole to the least loaded queue, so each XCD's L2
+// serves whole slices (uniform stacks).  Shorter queues are padded with empty items
+// (slice = -1).
 template <class Item>
 static std::vector<Item> batch_order(const std::vector<Item>& in,
-                                     const std::vector<ctprj_geom>& geoms)
+                                     const std::vector<ctprj_geom>& geoms, bool spread)
 {
     std::vector<std::vector<Item>> per(geoms.size());
     for (const Item& it : in) per[it.slice].push_back(it);
     std::vector<int> order;
-    for (int s = 0; s < (int)geoms.size(); ++s)
-        if (!per[s].empty()) order.push_back(s);
+    double total = 0;
     auto work = [&](int s) { return (double)geoms[s].n * geoms[s].n * geoms[s].np; };
+    for (int s = 0; s < (int)geoms.size(); ++s)
+        if (!per[s].empty()) {
+            order.push_back(s);
+            total += work(s);
+        }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return work(a) > work(b); });
-    std::vector<Item> out;
+    std::vector<Item> q[8];
+    double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int s : order) {
-        const std::vector<Item> seg = xcd_order(per[s]);
-        out.insert(out.end(), seg.begin(), seg.end());
+        const std::vector<Item>& its = per[s];
+        if (spread || work(s) > total / 32.0) {
+            const size_t chunk = (its.size() + 7) / 8;
+            for (int k = 0; k < 8; ++k) {
+                const size_t a = std::min(its.size(), k * chunk), b = std::min(its.size(), a + chunk);
+                q[k].insert(q[k].end(), its.begin() + a, its.begin() + b);
+                load[k] += work(s) * (double)(b - a) / (double)its.size();
+            }
+        } else {
+            const int k = (int)(std::min_element(load, load + 8) - load);
+            q[k].insert(q[k].end(), its.begin(), its.end());
+            load[k] += work(s);
+        }
     }
+    size_t L = 0;
+    for (int k = 0; k < 8; ++k) L = std::max(L, q[k].size());
+    std::vector<Item> out(8 * L);
+    for (size_t i = 0; i < L; ++i)
+        for (int k = 0; k < 8; ++k) {
+            Item& o = out[8 * i + k];
+            if (i < q[k].size()) o = q[k][i];
+            else { o = Item{}; o.slice = -1; }
+        }
     return out;
 }
 
@@ -435,12 +449,12 @@ int Plan::build_tables()
 // Concatenates per-group lists (each in batch order) and records the segments.
 template <class Item>
 static std::vector<Item> group_segments(std::vector<Item> (&by)[4],
-                                        const std::vector<ctprj_geom>& geoms, int* start,
-                                        int* count)
+                                        const std::vector<ctprj_geom>& geoms, bool spread,
+                                        int* start, int* count)
 {
     std::vector<Item> out;
     for (int g = 1; g < 4; ++g) {
-        std::vector<Item> seg = batch_order(by[g], geoms);
+        std::vector<Item> seg = batch_order(by[g], geoms, spread);
         start[g] = (int)out.size();
         count[g] = (int)seg.size();
         out.insert(out.end(), seg.begin(), seg.end());
@@ -468,8 +482,9 @@ int Plan::build_fwd_items()
                 for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
         }
     }
-    items = batch_order(items, geoms);
-    std::vector<FwdItem> sym = group_segments(by, geoms, symf_start, symf_count);
+    const bool spread = !(tune.flags & 8);
+    items = batch_order(items, geoms, spread);
+    std::vector<FwdItem> sym = group_segments(by, geoms, spread, symf_start, symf_count);
     free_ptr(d_fwd_items);
     free_ptr(d_symf_items);
     n_fwd_items = (int)items.size();
@@ -504,8 +519,9 @@ int Plan::build_adj_items()
                 for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
         }
     }
-    items = batch_order(items, geoms);
-    std::vector<AdjItem> sym = group_segments(by, geoms, syma_start, syma_count);
+    const bool spread = !(tune.flags & 8);
+    items = batch_order(items, geoms, spread);
+    std::vector<AdjItem> sym = group_segments(by, geoms, spread, syma_start, syma_count);
     free_ptr(d_adj_items);
     free_ptr(d_syma_items);
     n_adj_items = (int)items.size();
@@ -760,7 +776,7 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
         t->fwd_seg != 16)
         return fail(CTPRJ_EINVAL, "fwd_seg must be 1, 2, 4, 8 or 16");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
-    if (t->flags & ~7) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
+    if (t->flags & ~15) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
     const bool tables = (t->flags != p->tune.flags);
@@ -785,7 +801,8 @@ static std::string tune_key(const ctprj_plan* p, const char* arch)
 
 // Times the kernel shapes on this plan's geometry: general forward threads per group,
 // symmetric forward waves per ray run, general adjoint tile edge, symmetric adjoint waves
-// per tile; keeps tune.flags.
+// per tile, and (several slices) the work order of tune flag bit 3; keeps the symmetry
+// flags (bits 0-2).
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -793,7 +810,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     hipStream_t st = (hipStream_t)stream;
     char arch[64] = "gfx950";
     ctprj_device_arch(arch, sizeof(arch));
-    const std::string key = tune_key(p, arch) + "|f" + std::to_string(p->tune.flags);
+    const std::string key = tune_key(p, arch) + "|f" + std::to_string(p->tune.flags & 7);
     if (cache_path) {
         std::ifstream f(cache_path);
         std::string line;
@@ -852,6 +869,23 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     };
     const int blocks[3] = {64, 128, 256}, tiles[2] = {8, 16};
     const int fsegs[5] = {1, 2, 4, 8, 16}, asegs[4] = {1, 2, 4, 8};
+    if (!rc && p->nslices > 1) {
+        // work order: every slice spread over the XCDs, or whole slices per XCD
+        // (flag bit 3), timed on a forward + adjoint pair
+        const int base = best.flags & ~8, pol[2] = {base, base | 8};
+        float best_ms = 1e30f;
+        for (int i = 0; i < 2 && !rc; ++i) {
+            ctprj_tune t = best;
+            t.flags = pol[i];
+            float mf = 0, ma = 0;
+            if (!(rc = ctprj_plan_set_tune(p, &t)) && !(rc = time_dir(true, &mf)) &&
+                !(rc = time_dir(false, &ma)) && mf + ma < best_ms) {
+                best_ms = mf + ma;
+                best.flags = pol[i];
+            }
+        }
+        if (!rc) rc = ctprj_plan_set_tune(p, &best);
+    }
     if (!rc && p->n_fwd_items) rc = search(&ctprj_tune::fwd_block, blocks, 3, true);
     if (!rc && p->n_symf_items) rc = search(&ctprj_tune::fwd_seg, fsegs, 5, true);
     if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);

@@ -109,7 +109,9 @@ typedef struct ctprj_tune {
     int fwd_block;   /* general forward work-group: 64, 128 or 256 threads, one ray each */
     int adj_tile;    /* general adjoint pixel tile edge: 8 or 16 */
     int flags;       /* bit0: use the symmetric kernels where the geometry allows;
-                        bit1: not the 8-fold group; bit2: not the 4-fold group */
+                        bit1: not the 8-fold group; bit2: not the 4-fold group;
+                        bit3: work order keeps small slices whole on one XCD (else every
+                        slice is spread over all XCDs) */
     int adj_seg;     /* symmetric adjoint: waves per 8x8 tile of orbit base pixels
                         splitting the views: 1, 2, 4 or 8 */
     int fwd_seg;     /* symmetric forward: waves per 64 canonical rays splitting the
