@@ -234,34 +234,27 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
         for (int e = 0; e < 8; ++e)
             if (MASK >> e & 1u) red[seg][e][lane] = acc[e].s;
         __syncthreads();
-        if (seg != 0) return;
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if (!(MASK >> e & 1u)) continue;
-            float t = red[0][e][lane];
-#pragma unroll 1
-            for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
-            acc[e].s = t;
-        }
     }
+    // wave w finalises the images e = w (mod SEG): the sum of the waves' partials in wave
+    // order, written to view g_e a0
     float* out = sino + S.sino_off;
-    if (b < S.blo) {  // even detector: bin 0 carries no ray in any view
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            if (!(MASK >> e & 1u)) continue;
-            int sg, off;
-            const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
-            if (v < S.np) out[(long long)v * S.W] = 0.0f;
-        }
-        return;
-    }
-    if (!active) return;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
-        if (!(MASK >> e & 1u)) continue;
+        if (!(MASK >> e & 1u) || (SEG > 1 && e % SEG != seg)) continue;
+        float t = acc[e].s;
+        if (SEG > 1) {
+            t = red[0][e][lane];
+#pragma unroll
+            for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
+        }
         int sg, off;
         const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
-        if (v < S.np) out[(long long)v * S.W + off + sg * b] = acc[e].s;
+        if (v >= S.np) continue;
+        // folded scans: images e and e ^ 2 (R^2 apart) meet at the centre bin, whose two
+        // sums are equal up to rounding; one writer (k < 2) keeps the result deterministic
+        if (S.fold && (e & 2) && b == S.pC) continue;
+        if (b < S.blo) out[(long long)v * S.W] = 0.0f;  // even detector: bin 0 carries no ray
+        else if (active) out[(long long)v * S.W + off + sg * b] = t;
     }
 }
 
@@ -291,8 +284,8 @@ __device__ __forceinline__ void fwd_sym_group(const SliceDesc& S, const FwdItem&
 
 // One work item: (slice, canonical view index, first bin of 64) of a slice under group
 // GRP (one launch per group keeps each variant's registers to its own images).  SEG
-// waves share the 64 rays, each walking 1/SEG of the image rows; wave 0 sums the
-// partials in order.
+// waves share the 64 rays, each walking 1/SEG of the image rows, and then share the
+// final sums.
 template <int SEG, int GRP>
 __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __restrict__ slices,
                                                           const FwdItem* __restrict__ items,
@@ -555,21 +548,13 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
         for (int h = 0; h < 8; ++h)
             if (G.all >> h & 1u) red[wave][h][lane] = acc[h].s;
         __syncthreads();
-        if (wave != 0) return;
-#pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            if (!(G.all >> h & 1u)) continue;
-            float t = red[0][h][lane];
-#pragma unroll 1
-            for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
-            acc[h].s = t;
-        }
     }
     if (!valid) return;
     float* out = img + S.img_off;
     const int n = S.n, half = S.half;
     // each distinct orbit pixel h q0 once, from the first h (in element order) that
-    // reaches it
+    // reaches it; wave w finalises the elements h = w (mod SEG): the sum of the waves'
+    // partials in wave order
     int ox[8], oy[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
@@ -579,14 +564,19 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 #pragma unroll
         for (int g = 0; g < h; ++g)
             if ((G.all >> g & 1u) && ox[g] == ox[h] && oy[g] == oy[h]) fresh = false;
-        if (!fresh) continue;
-        const float val = FBP ? acc[h].s * S.fbp_scale : acc[h].s;
-        out[(long long)(oy[h] + half) * n + ox[h] + half] = val;
+        if (!fresh || (SEG > 1 && h % SEG != wave)) continue;
+        float t = acc[h].s;
+        if (SEG > 1) {
+            t = red[0][h][lane];
+#pragma unroll
+            for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
+        }
+        out[(long long)(oy[h] + half) * n + ox[h] + half] = FBP ? t * S.fbp_scale : t;
     }
 }
 
 // One launch per symmetry group; SEG waves split the canonical views (wave w takes
-// i = w, w + SEG, ...); wave 0 sums their partials in order.
+// i = w, w + SEG, ...) and then share the final sums.
 template <bool FBP, int SEG, int GRP>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,

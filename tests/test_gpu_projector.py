@@ -203,6 +203,27 @@ def test_batched_equals_single(torch_dev):
         assert torch.equal(ib[i].reshape(-1), S.adjoint(sinos[i].reshape(-1)))
 
 
+def test_work_order_does_not_change_results(torch_dev):
+    """The work order (every slice spread over the XCDs, or small slices whole per XCD:
+    tune flag bit 3) only schedules items; each item's sums are the same, so the outputs
+    are bit-identical."""
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom, gaussian
+    from imgrecsrc_amd.projector import Projector
+    geoms = [(96, 140, 101, 101, 1.0, 1.0, 384.0), (128, 185, 90, 180, 1.0, 1.0, 0.0),
+             (65, 96, 30, 120, 1.0, 1.0, 0.0), (80, 117, 45, 90, 1.0, 1.0, 0.0),
+             (70, 103, 70, 70, 1.0, 1.0, 280.0), (200, 287, 77, 77, 1.0, 1.0, 800.0)]
+    B = Projector(geoms)
+    imgs = [torch.from_numpy(ellipse_phantom(g[0], 60 + i)).cuda() for i, g in enumerate(geoms)]
+    sinos = [torch.from_numpy(gaussian((g[2], g[1]), 70 + i)).cuda() for i, g in enumerate(geoms)]
+    x, y = B.pack_images(imgs), B.pack_sinos(sinos)
+    out = []
+    for flags in (1, 9):
+        B.set_tune(flags=flags)
+        out.append((B.forward(x).clone(), B.adjoint(y).clone()))
+    assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
+
+
 def test_nnz_matches_oracle(torch_dev, oracle_mod):
     """Device count of non-zero FP32 weights vs the FP64 count: overlaps of a pixel corner
     with a strip edge below ~1e-9 px^2 round to exactly 0 in FP32, so the FP32 count is

@@ -8,7 +8,8 @@ transposes.  Keys related by the exact half-turn identity W(a0, b, p) == W(a0, 2
 of a folded parallel scan are identified; no other geometric equivalence is assumed (at
 45 degrees the diagonal mirror is not exact in FP32).  A planted mistake in the model
 (all elements on a mirror view, a missing fold element, a hole in the fundamental
-domain, a dropped dedup) makes these checks fail, which the planted-mistake test
+domain, a dropped dedup, two writers of a folded scan's centre bin) makes these checks
+fail, which the planted-mistake test
 confirms.  The model checks the decomposition's consistency only: that each view map
 is the right geometric one is pinned by the GPU comparison against the FP64 oracle.
 """
@@ -97,18 +98,26 @@ def decompose(n, W, np_, P, fan, flags=1, plant=None):
     fwd = {}
     bmax = pC if fold else bhi
     pixels = [(x, y) for y in range(-H, H + 1) for x in range(-H, H + 1)]
+    writes = defaultdict(int)
     for a0 in cs:
         els = bits(rot if mirror(a0) and plant != "mirror" else allm)
         for b in range(blo, bmax + 1):
-            for p in pixels:
-                for e in els:
-                    v, sg, off = mv(P, pC, fold, e & 3, e >> 2, a0)
-                    if v >= np_: continue
+            for e in els:
+                v, sg, off = mv(P, pC, fold, e & 3, e >> 2, a0)
+                if v >= np_: continue
+                # the kernel's single writer of a folded scan's centre bin
+                if fold and (e & 2) and b == pC and plant != "centre":
+                    continue
+                writes[(v, off + sg * b)] += 1
+                for p in pixels:
                     tgt = (v, off + sg * b, apply_px(e & 3, e >> 2, *p))
                     key = norm(a0, b, p)
                     if tgt in fwd and fwd[tgt] != key:
                         raise AssertionError(f"forward writes {tgt} twice: {fwd[tgt]} / {key}")
                     fwd[tgt] = key
+    # every sinogram entry is written by exactly one (work item, image)
+    if any(c != 1 for c in writes.values()):
+        raise AssertionError("a sinogram entry has more than one writer")
     # adjoint: orbit base pixels in the group's fundamental domain (8x8 tiles, as the
     # host lays them out), element sets per view kind, first-occurrence writes
     nt = (H + 1 + 7) // 8
@@ -191,6 +200,7 @@ def test_each_entry_once_with_one_key(case):
 
 
 @pytest.mark.parametrize("plant,case", [
+    ("centre", (9, 14, 20, 40, False, 1)), ("centre", (8, 13, 9, 18, False, 1)),
     ("mirror", (8, 12, 16, 16, True, 1)), ("mirror", (7, 12, 13, 13, True, 1)),
     ("domain", (9, 14, 18, 18, True, 1)), ("fold", (9, 14, 20, 40, False, 1)),
     ("fold", (8, 13, 9, 18, False, 1)), ("dedup", (7, 12, 13, 13, True, 1)),
