@@ -78,7 +78,10 @@ SIGNATURES = [
     ("ctprj_forward", _i, [_vp, _vp, _vp, _vp]),
     ("ctprj_adjoint", _i, [_vp, _vp, _vp, _vp]),
     ("ctprj_fbp", _i, [_vp, _vp, _vp, _vp]),
+    ("ctprj_poly_iout", _i, [_i, _dp, _i, _dp, _vp, _ll, _vp, _vp, _vp, _vp]),
     ("ctprj_b1_iout", _i, [_dp, _i, _dp, _vp, _ll, _vp, _vp, _vp, _vp]),
+    ("ctprj_plan_set_bhc_model", _i, [_vp, _i, _i]),
+    ("ctprj_plan_get_bhc_model", _i, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     ("ctprj_bhc_grad_alpha", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_alpha_rays", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_grad_ie", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),

@@ -1,10 +1,15 @@
-"""Polychromatic Poisson beam-hardening model on the device.
+"""Polychromatic beam-hardening model on the device.
 
 Mirrors of the reference's MATLAB contract, computed by libctprj:
 
-* ``Spline('b1', knots).polyIout(s, Ie)`` -> (BL, sBL, ssBL)   -- bhc/Spline.m:11-28,122-203
+* ``Spline(sType, knots).polyIout(s, Ie)`` -> (BL, sBL, ssBL), sType 'b1' / 'b0' / 'dis'
+  -- bhc/Spline.m:11-28,122-302
 * ``poissLAlpha(Imea, Ie, alpha, proj, Ts, knots)`` -> (f, g, hw) -- bhc/poissLAlpha.m
-* ``poissLI(Imea, alpha_or_s, Ie, proj, Ts, knots)`` -> (f, g, H) -- bhc/poissLI.m
+* ``poissLI(Imea, Ie, proj, Ts, knots, alpha=|sraw=)`` -> (f, g, H) -- bhc/poissLI.m
+* ``gaussLAlpha`` / ``gaussLI`` (same arguments) -- bhc/gaussLAlpha.m, bhc/gaussLI.m
+
+Every likelihood call takes ``basis='b1'|'b0'|'dis'`` (bhc/beamhardenSpline.m:60
+opt.spectBasis); its knots are E+2 values for 'b1' and 'dis', E+1 for 'b0'.
 
 ``kappa_logspace`` / ``spline_knots`` reproduce the mass-attenuation grid and the
 outer knots of bhc/beamhardenSpline.m:138-139,164-165; ``normalize_ie`` the
@@ -54,11 +59,23 @@ def normalize_ie(Ie, knots) -> np.ndarray:
     return Ie / float(basis_at_zero(knots) @ Ie)
 
 
-def _knots_ie(knots, Ie):
+BASIS = {"b1": 0, "b0": 1, "dis": 2}      # CTPRJ_BASIS_*
+NOISE = {"poisson": 0, "gaussian": 1}     # CTPRJ_NOISE_*
+
+
+def _basis_id(basis: str) -> int:
+    try:
+        return BASIS[basis.lower()]
+    except KeyError:
+        raise ValueError(f"unknown spectrum basis '{basis}' (b1, b0, dis)") from None
+
+
+def _knots_ie(knots, Ie, basis: str = "b1"):
+    """E (number of bases) and ctypes arrays of the knots and Ie of `basis`."""
     k = np.asarray(knots, np.float64).ravel()
-    E = k.size - 2
+    E = k.size - (1 if _basis_id(basis) == BASIS["b0"] else 2)
     if E < 1 or E > MAX_E:
-        raise ValueError(f"need 3..{MAX_E + 2} knots, got {k.size}")
+        raise ValueError(f"{k.size} knots give {E} '{basis}' bases (need 1..{MAX_E})")
     kp, kh = dbl_array(k)
     if Ie is None:
         return E, kp, kh, None, None
@@ -67,6 +84,11 @@ def _knots_ie(knots, Ie):
         raise ValueError(f"Ie has {ie.size} entries, knots define {E} bases")
     ip, ih = dbl_array(ie)
     return E, kp, kh, ip, ih
+
+
+def set_model(proj: Projector, basis: str = "b1", noise: str = "poisson") -> None:
+    """Spectrum basis and noise model of the plan's beam-hardening calls."""
+    check(lib().ctprj_plan_set_bhc_model(proj.handle, _basis_id(basis), NOISE[noise.lower()]))
 
 
 def _f32(t: torch.Tensor, n: int, what: str) -> torch.Tensor:
@@ -78,38 +100,35 @@ def _f32(t: torch.Tensor, n: int, what: str) -> torch.Tensor:
 
 
 class Spline:
-    """``Spline('b1', knots)`` of bhc/Spline.m with the polyIout closure on the device."""
+    """``Spline(sType, knots)`` of bhc/Spline.m with the polyIout closure on the device;
+    sType 'b1' (hats, E+2 knots), 'b0' (boxes, E+1 knots) or 'dis' (nodes, E+2 knots)."""
 
     def __init__(self, sType: str, knots):
-        if sType.lower() != "b1":
-            raise NotImplementedError(f"spectrum basis '{sType}': only 'b1' runs on the device")
-        self.sType = "b1"
+        self.sType = sType.lower()
+        self.basis = _basis_id(self.sType)
         self.knots = np.asarray(knots, np.float64).ravel()
 
     def polyIout(self, s: torch.Tensor, Ie=None, want: int = 3):
         """Ie given: (BL, sBL, ssBL) per ray value.  Ie None: the count x E basis matrix."""
         n = s.numel()
         s = _f32(s.reshape(-1), n, "s")
-        E, kp, _kh, ip, _ih = _knots_ie(self.knots, Ie)
+        E, kp, _kh, ip, _ih = _knots_ie(self.knots, Ie, self.sType)
         if Ie is None:
             A = torch.empty((n, E), dtype=torch.float32, device=s.device)
-            check(lib().ctprj_b1_iout(kp, E, None, _ptr(s), n, _ptr(A), None, None,
-                                      _stream_ptr()))
+            check(lib().ctprj_poly_iout(self.basis, kp, E, None, _ptr(s), n, _ptr(A), None,
+                                        None, _stream_ptr()))
             return A
         outs = [torch.empty(n, dtype=torch.float32, device=s.device) for _ in range(want)]
         ptrs = [_ptr(o) for o in outs] + [None] * (3 - want)
-        check(lib().ctprj_b1_iout(kp, E, ip, _ptr(s), n, *ptrs, _stream_ptr()))
+        check(lib().ctprj_poly_iout(self.basis, kp, E, ip, _ptr(s), n, *ptrs, _stream_ptr()))
         return tuple(outs)
 
 
-def poissLAlpha(Imea: torch.Tensor, Ie, alpha: torch.Tensor, proj: Projector, Ts: float,
-                knots, want_grad: bool = True, want_hess: bool = False):
-    """[f, g(, hw)] of bhc/poissLAlpha.m with Phi = Ts*A, Phit = Ts*A^T, polyIout = b1.
-
-    f: float64 scalar tensor (FP64 reduction); g: float32 image; hw: Hessian ray weight."""
+def _grad_alpha(noise, Imea, Ie, alpha, proj, Ts, knots, want_grad, want_hess, basis):
     alpha = _f32(alpha.reshape(-1), proj.img_elems, "alpha")
     Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
-    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
+    set_model(proj, basis, noise)
     dev = alpha.device
     f = torch.zeros(1, dtype=torch.float64, device=dev)
     g = torch.empty(proj.img_elems, dtype=torch.float32, device=dev) if want_grad else None
@@ -125,12 +144,29 @@ def poissLAlpha(Imea: torch.Tensor, Ie, alpha: torch.Tensor, proj: Projector, Ts
     return tuple(out) if len(out) > 1 else out[0]
 
 
+def poissLAlpha(Imea: torch.Tensor, Ie, alpha: torch.Tensor, proj: Projector, Ts: float,
+                knots, want_grad: bool = True, want_hess: bool = False, basis: str = "b1"):
+    """[f, g(, hw)] of bhc/poissLAlpha.m with Phi = Ts*A, Phit = Ts*A^T and polyIout of
+    `basis`.  f: float64 scalar tensor (FP64 reduction); g: float32 image; hw: Hessian
+    ray weight."""
+    return _grad_alpha("poisson", Imea, Ie, alpha, proj, Ts, knots, want_grad, want_hess, basis)
+
+
+def gaussLAlpha(Imea: torch.Tensor, Ie, alpha: torch.Tensor, proj: Projector, Ts: float,
+                knots, want_grad: bool = True, want_hess: bool = False, basis: str = "b1"):
+    """[f, g(, hw)] of bhc/gaussLAlpha.m: Err = log(BL/Imea), f = Err'Err/2,
+    g = Phit(Err.*sBL./BL), hw = (1-Err).*(sBL./BL).^2 + Err.*ssBL./BL."""
+    return _grad_alpha("gaussian", Imea, Ie, alpha, proj, Ts, knots, want_grad, want_hess, basis)
+
+
 def alpha_rays(proj: Projector, sraw: torch.Tensor, Imea: torch.Tensor, Ie, knots, Ts: float,
-               want_w: bool):
-    """Per-ray pass of poissLAlpha on a stored projection sraw = A*alpha: (f, w or None)."""
+               want_w: bool, basis: str = "b1", noise: str = "poisson"):
+    """Per-ray pass of poissLAlpha / gaussLAlpha on a stored projection sraw = A*alpha:
+    (f, w or None)."""
     sraw = _f32(sraw.reshape(-1), proj.sino_elems, "sraw")
     Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
-    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
+    set_model(proj, basis, noise)
     f = torch.zeros(1, dtype=torch.float64, device=sraw.device)
     w = torch.empty_like(sraw) if want_w else None
     check(lib().ctprj_bhc_alpha_rays(proj.handle, _ptr(sraw), _ptr(Imea), kp, E, ip, float(Ts),
@@ -139,15 +175,12 @@ def alpha_rays(proj: Projector, sraw: torch.Tensor, Imea: torch.Tensor, Ie, knot
     return f[0], w
 
 
-def poissLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
-            alpha: torch.Tensor | None = None, sraw: torch.Tensor | None = None):
-    """[f, g, H] of bhc/poissLI.m with A = polyIout(Phi(alpha), []) never formed.
-
-    Give alpha (projected inside) or sraw = A*alpha.  Returns float64 tensors."""
+def _grad_ie(noise, Imea, Ie, proj, Ts, knots, alpha, sraw, basis):
     if (alpha is None) == (sraw is None):
         raise ValueError("give exactly one of alpha and sraw")
     Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
-    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie)
+    E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
+    set_model(proj, basis, noise)
     dev = Imea.device
     f = torch.zeros(1, dtype=torch.float64, device=dev)
     g = torch.zeros(E, dtype=torch.float64, device=dev)
@@ -159,5 +192,23 @@ def poissLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
     return f[0], g, H
 
 
-__all__ = ["Spline", "poissLAlpha", "poissLI", "alpha_rays", "kappa_logspace", "kappa_logspan",
-           "spline_knots", "basis_at_zero", "normalize_ie"]
+def poissLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
+            alpha: torch.Tensor | None = None, sraw: torch.Tensor | None = None,
+            basis: str = "b1"):
+    """[f, g, H] of bhc/poissLI.m with A = polyIout(Phi(alpha), []) never formed.
+
+    Give alpha (projected inside) or sraw = A*alpha.  Returns float64 tensors."""
+    return _grad_ie("poisson", Imea, Ie, proj, Ts, knots, alpha, sraw, basis)
+
+
+def gaussLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
+            alpha: torch.Tensor | None = None, sraw: torch.Tensor | None = None,
+            basis: str = "b1"):
+    """[f, g, H] of bhc/gaussLI.m: Err = log(Ir/Imea), f = Err'Err/2, g = A'(Err./Ir),
+    H = A' diag((1-Err)./Ir.^2) A."""
+    return _grad_ie("gaussian", Imea, Ie, proj, Ts, knots, alpha, sraw, basis)
+
+
+__all__ = ["Spline", "poissLAlpha", "poissLI", "gaussLAlpha", "gaussLI", "alpha_rays",
+           "set_model", "kappa_logspace", "kappa_logspan", "spline_knots", "basis_at_zero",
+           "normalize_ie", "BASIS", "NOISE"]

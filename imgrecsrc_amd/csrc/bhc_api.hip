@@ -1,5 +1,5 @@
 // Host entry points of the beam-hardening model (bhc/poissLAlpha.m, bhc/poissLI.m,
-// bhc/Spline.m:b1Iout) on top of a projector plan.
+// bhc/gaussLAlpha.m, bhc/gaussLI.m, bhc/Spline.m polyIout) on top of a projector plan.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -9,18 +9,29 @@
 
 namespace ctprj {
 
-static int make_params(const double* knots, int E, const double* Ie, BhcParams* P)
+int make_bhc_params(const double* knots, int E, const double* Ie, int basis, int noise,
+                    BhcParams* P)
 {
     if (!knots) return fail(CTPRJ_EINVAL, "knots is NULL");
     if (E < 1 || E > CTPRJ_MAX_E) return fail(CTPRJ_EINVAL, "E=%d must be in [1, %d]", E, CTPRJ_MAX_E);
-    for (int j = 0; j < E + 2; ++j)
+    if (basis != CTPRJ_BASIS_B1 && basis != CTPRJ_BASIS_B0 && basis != CTPRJ_BASIS_DIS)
+        return fail(CTPRJ_EINVAL, "unknown spectrum basis %d", basis);
+    if (noise != CTPRJ_NOISE_POISSON && noise != CTPRJ_NOISE_GAUSSIAN)
+        return fail(CTPRJ_EINVAL, "unknown noise model %d", noise);
+    const int nk = (basis == CTPRJ_BASIS_B0) ? E + 1 : E + 2;  // knots read
+    for (int j = 0; j < nk; ++j)
         if (!std::isfinite(knots[j])) return fail(CTPRJ_EINVAL, "knot %d not finite", j);
-    for (int j = 0; j < E + 1; ++j)
+    for (int j = 0; j + 1 < nk; ++j)
         if (!(knots[j + 1] > knots[j])) return fail(CTPRJ_EINVAL, "knots must increase (at %d)", j);
     *P = BhcParams{};
     P->E = E;
-    for (int j = 0; j < E + 2; ++j) P->knot[j] = (float)knots[j];
-    for (int j = 0; j < E + 1; ++j) P->delta[j] = (float)(knots[j + 1] - knots[j]);
+    P->basis = basis;
+    P->noise = noise;
+    for (int j = 0; j < nk; ++j) P->knot[j] = (float)knots[j];
+    if (basis == CTPRJ_BASIS_DIS)
+        for (int j = 0; j < E; ++j) P->delta[j] = (float)((knots[j + 2] - knots[j]) / 2.0);
+    else
+        for (int j = 0; j + 1 < nk; ++j) P->delta[j] = (float)(knots[j + 1] - knots[j]);
     for (int j = 0; j < E; ++j) {
         const double v = Ie ? Ie[j] : 0.0;
         if (!std::isfinite(v)) return fail(CTPRJ_EINVAL, "Ie[%d] not finite", j);
@@ -28,6 +39,11 @@ static int make_params(const double* knots, int E, const double* Ie, BhcParams* 
         P->Ied[j] = v;
     }
     return 0;
+}
+
+static int make_params(const Plan* p, const double* knots, int E, const double* Ie, BhcParams* P)
+{
+    return make_bhc_params(knots, E, Ie, p->bhc_basis, p->bhc_noise, P);
 }
 
 static int ensure_red(Plan* p, long long doubles)
@@ -58,19 +74,46 @@ using namespace ctprj;
 
 struct ctprj_plan : public Plan {};
 
+extern "C" int ctprj_poly_iout(int basis, const double* knots, int E, const double* Ie,
+                               const float* d_s, long long count, float* d_BL, float* d_sBL,
+                               float* d_ssBL, void* stream)
+{
+    BhcParams P;
+    int rc = make_bhc_params(knots, E, Ie, basis, CTPRJ_NOISE_POISSON, &P);
+    if (rc) return rc;
+    if (count < 0 || (count > 0 && !d_s)) return fail(CTPRJ_EINVAL, "ctprj_poly_iout: bad rays");
+    const bool matrix = (Ie == nullptr);
+    if (matrix && !d_BL) return fail(CTPRJ_EINVAL, "ctprj_poly_iout: matrix mode needs BL");
+    launch_poly_iout(P, d_s, count, d_BL, matrix ? nullptr : d_sBL, matrix ? nullptr : d_ssBL,
+                     matrix, (hipStream_t)stream);
+    return last_launch("poly_iout launch");
+}
+
 extern "C" int ctprj_b1_iout(const double* knots, int E, const double* Ie, const float* d_s,
                              long long count, float* d_BL, float* d_sBL, float* d_ssBL,
                              void* stream)
 {
-    BhcParams P;
-    int rc = make_params(knots, E, Ie, &P);
-    if (rc) return rc;
-    if (count < 0 || (count > 0 && !d_s)) return fail(CTPRJ_EINVAL, "ctprj_b1_iout: bad rays");
-    const bool matrix = (Ie == nullptr);
-    if (matrix && !d_BL) return fail(CTPRJ_EINVAL, "ctprj_b1_iout: matrix mode needs BL");
-    launch_b1_iout(P, d_s, count, d_BL, matrix ? nullptr : d_sBL, matrix ? nullptr : d_ssBL,
-                   matrix, (hipStream_t)stream);
-    return last_launch("b1_iout launch");
+    return ctprj_poly_iout(CTPRJ_BASIS_B1, knots, E, Ie, d_s, count, d_BL, d_sBL, d_ssBL, stream);
+}
+
+extern "C" int ctprj_plan_set_bhc_model(ctprj_plan* p, int basis, int noise)
+{
+    if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
+    if (basis != CTPRJ_BASIS_B1 && basis != CTPRJ_BASIS_B0 && basis != CTPRJ_BASIS_DIS)
+        return fail(CTPRJ_EINVAL, "unknown spectrum basis %d", basis);
+    if (noise != CTPRJ_NOISE_POISSON && noise != CTPRJ_NOISE_GAUSSIAN)
+        return fail(CTPRJ_EINVAL, "unknown noise model %d", noise);
+    p->bhc_basis = basis;
+    p->bhc_noise = noise;
+    return 0;
+}
+
+extern "C" int ctprj_plan_get_bhc_model(const ctprj_plan* p, int* basis, int* noise)
+{
+    if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
+    if (basis) *basis = p->bhc_basis;
+    if (noise) *noise = p->bhc_noise;
+    return 0;
 }
 
 extern "C" int ctprj_bhc_grad_alpha(ctprj_plan* p, const float* d_alpha, const float* d_Imea,
@@ -80,7 +123,7 @@ extern "C" int ctprj_bhc_grad_alpha(ctprj_plan* p, const float* d_alpha, const f
     if (!p || !d_alpha || !d_Imea || !d_f) return fail(CTPRJ_EINVAL, "bhc_grad_alpha: NULL argument");
     if (!Ie) return fail(CTPRJ_EINVAL, "bhc_grad_alpha: Ie is NULL");
     BhcParams P;
-    int rc = make_params(knots, E, Ie, &P);
+    int rc = make_params(p, knots, E, Ie, &P);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
@@ -103,7 +146,7 @@ extern "C" int ctprj_bhc_alpha_rays(ctprj_plan* p, const float* d_sraw, const fl
     if (!p || !d_sraw || !d_Imea || !d_f) return fail(CTPRJ_EINVAL, "bhc_alpha_rays: NULL argument");
     if (!Ie) return fail(CTPRJ_EINVAL, "bhc_alpha_rays: Ie is NULL");
     BhcParams P;
-    int rc = make_params(knots, E, Ie, &P);
+    int rc = make_params(p, knots, E, Ie, &P);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
@@ -123,7 +166,7 @@ extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const floa
     if (!d_alpha && !d_s) return fail(CTPRJ_EINVAL, "bhc_grad_ie: need alpha or s");
     if (!Ie) return fail(CTPRJ_EINVAL, "bhc_grad_ie: Ie is NULL");
     BhcParams P;
-    int rc = make_params(knots, E, Ie, &P);
+    int rc = make_params(p, knots, E, Ie, &P);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);

@@ -7,24 +7,35 @@
 
 namespace ctprj {
 
-// Kernel arguments of the spectrum model: E hat bases on the E+2 knots.
+// Kernel arguments of the spectrum model (bhc/Spline.m): E basis functions of the
+// mass-attenuation spectrum and the noise model of the likelihood.
+//   b1  (CTPRJ_BASIS_B1):  hats on the E+2 knots k_0..k_{E+1}, delta[j] = k_{j+1} - k_j
+//   b0  (CTPRJ_BASIS_B0):  boxes on the E+1 knots k_0..k_E,     delta[j] = k_{j+1} - k_j
+//   dis (CTPRJ_BASIS_DIS): nodes k_1..k_E of the E+2 knots,      delta[j] = (k_{j+2} - k_j)/2
 struct BhcParams {
     int E;
-    float knot[CTPRJ_MAX_E + 2];    // k_0 .. k_{E+1}
-    float delta[CTPRJ_MAX_E + 1];   // k_{j+1} - k_j (rounded from double)
+    int basis;                      // CTPRJ_BASIS_*
+    int noise;                      // CTPRJ_NOISE_*
+    float knot[CTPRJ_MAX_E + 2];
+    float delta[CTPRJ_MAX_E + 1];   // (rounded from double)
     float Ie[CTPRJ_MAX_E];          // basis coefficients (FP32 per-ray sums)
     double Ied[CTPRJ_MAX_E];        // basis coefficients (FP64 I-epsilon sums)
 };
+
+// Validates the knots of `basis` (E+2, or E+1 for b0, increasing) and fills P.
+int make_bhc_params(const double* knots, int E, const double* Ie, int basis, int noise,
+                    BhcParams* P);
 
 constexpr int kBhcBlock = 256;
 
 // values reduced by the I-epsilon pass: E(E+1)/2 Hessian entries, E gradient, 1 NLL
 __host__ __device__ inline int bhc_ie_nvals(int E) { return E * (E + 1) / 2 + E + 1; }
 
-void launch_b1_iout(const BhcParams& P, const float* s, long long count, float* BL, float* sBL,
-                    float* ssBL, bool matrix, hipStream_t st);
-// per-ray alpha pass: w = Ts (1 - Imea/BL) sBL (may alias s), hw, FP64 NLL partials
-// (ray values are sscale * s[i])
+void launch_poly_iout(const BhcParams& P, const float* s, long long count, float* BL,
+                      float* sBL, float* ssBL, bool matrix, hipStream_t st);
+// per-ray alpha pass: gradient weight w (may alias s), Hessian weight hw, FP64 NLL
+// partials (ray values are sscale * s[i]); Poisson: w = Ts (1 - Imea/BL) sBL,
+// Gaussian: w = Ts log(BL/Imea) sBL/BL
 void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                           long long count, float Ts, float* w, float* hw, double* partials,
                           hipStream_t st);

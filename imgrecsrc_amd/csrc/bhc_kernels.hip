@@ -1,7 +1,7 @@
-// Polychromatic Poisson beam-hardening model on gfx950: the B1-spline Laplace transform
-// of the mass-attenuation spectrum (bhc/Spline.m:122-203), the per-ray likelihood and
-// gradient weights of bhc/poissLAlpha.m, and the I-epsilon gradient / Hessian of
-// bhc/poissLI.m.  Per-ray spectral values are FP32 and recomputed wherever needed (no
+// Polychromatic beam-hardening model on gfx950: the Laplace transform of the
+// mass-attenuation spectrum in the b1 / b0 / dis bases (bhc/Spline.m:122-302), the
+// per-ray likelihood and gradient weights of bhc/poissLAlpha.m and bhc/gaussLAlpha.m,
+// and the I-epsilon gradient / Hessian of bhc/poissLI.m and bhc/gaussLI.m.  Per-ray spectral values are FP32 and recomputed wherever needed (no
 // ray x basis matrix is stored); the negative log-likelihood and every I-epsilon
 // reduction are FP64 with a fixed summation order: per work-group partials, then one
 // ordered pass.
@@ -46,20 +46,46 @@ __device__ __forceinline__ Moments moments(float z)
     return M;
 }
 
-// Spectrum basis values for one ray value s.  Basis j (0..E-1) is the hat on
-// [k_j, k_{j+2}] peaking at k_{j+1}.  Substituting kappa = k_j + Delta_j u in
-// phi_j(s) = int hat_j(kappa) exp(-kappa s) dkappa, with Delta_j = k_{j+1} - k_j,
-// z_j = Delta_j s and w_j = Delta_j exp(-k_j s), gives per segment j = [k_j, k_{j+1}]
+// Spectrum basis values for one ray value s, by basis type; calls f(j, phi, dphi, ddphi)
+// for j = 0..E-1 (derivatives in s); WANT = number of orders needed (1..3).
+//
+// b1: basis j is the hat on [k_j, k_{j+2}] peaking at k_{j+1}.  Substituting
+// kappa = k_j + Delta_j u in phi_j(s) = int hat_j(kappa) exp(-kappa s) dkappa, with
+// Delta_j = k_{j+1} - k_j, z_j = Delta_j s and w_j = Delta_j exp(-k_j s), gives per
+// segment j = [k_j, k_{j+1}]
 //   left half of basis j   : w_j M1                ,  -w_j (k_j M1 + Delta_j M2)
 //                            w_j (k_j^2 M1 + 2 k_j Delta_j M2 + Delta_j^2 M3)
 //   right half of basis j-1: w_j (M0-M1)           ,  -w_j (k_j (M0-M1) + Delta_j (M1-M2))
 //                            w_j (k_j^2 (M0-M1) + 2 k_j Delta_j (M1-M2) + Delta_j^2 (M2-M3))
 // for phi, phi' and phi'' (all moments at z_j).  These equal the closed forms of
 // bhc/Spline.m:154-201 without their small-s cancellation.
-// Calls f(j, phi, dphi, ddphi) for j = 0..E-1; WANT = number of orders needed (1..3).
+// b0: basis j is the box [k_j, k_{j+1}] (bhc/Spline.m:205-262):
+//   w_j M0,  -w_j (k_j M0 + Delta_j M1),  w_j (k_j^2 M0 + 2 k_j Delta_j M1 + Delta_j^2 M2).
+// dis: basis j is the node k_{j+1} with weight (k_{j+2} - k_j)/2 (bhc/Spline.m:264-302):
+//   dk exp(-k s),  -k phi,  k^2 phi.
 template <int WANT, class F>
 __device__ __forceinline__ void spline_basis(const BhcParams& P, float s, F&& f)
 {
+    if (P.basis == CTPRJ_BASIS_DIS) {
+        for (int j = 0; j < P.E; ++j) {
+            const float kj = P.knot[j + 1];
+            const float p = P.delta[j] * expf(-kj * s);
+            f(j, p, WANT >= 2 ? -kj * p : 0.0f, WANT >= 3 ? kj * kj * p : 0.0f);
+        }
+        return;
+    }
+    if (P.basis == CTPRJ_BASIS_B0) {
+        for (int j = 0; j < P.E; ++j) {
+            const float kj = P.knot[j], dj = P.delta[j];
+            const float w = dj * expf(-kj * s);
+            const Moments M = moments(dj * s);
+            float d = 0.0f, dd = 0.0f;
+            if (WANT >= 2) d = -w * fmaf(kj, M.m0, dj * M.m1);
+            if (WANT >= 3) dd = w * fmaf(kj * kj, M.m0, fmaf(2.0f * kj * dj, M.m1, dj * dj * M.m2));
+            f(j, w * M.m0, d, dd);
+        }
+        return;
+    }
     float cp = 0.0f, cd = 0.0f, cdd = 0.0f;  // left half of the pending basis j-1
     for (int j = 0; j <= P.E; ++j) {
         const float kj = P.knot[j], dj = P.delta[j];
@@ -85,7 +111,7 @@ __device__ __forceinline__ void spline_basis(const BhcParams& P, float s, F&& f)
 // ----------------------------------------------------------------- polyIout ---
 
 template <int WANT>
-__global__ __launch_bounds__(kBhcBlock) void k_b1_iout(BhcParams P, const float* __restrict__ s,
+__global__ __launch_bounds__(kBhcBlock) void k_poly_iout(BhcParams P, const float* __restrict__ s,
                                                        long long count, float* __restrict__ BL,
                                                        float* __restrict__ sBL,
                                                        float* __restrict__ ssBL, int matrix)
@@ -132,6 +158,10 @@ __device__ __forceinline__ double nll_term(double bl, double im)
     return (im > 0.0) ? (bl - im) - im * log(bl / im) : bl;
 }
 
+// Gaussian (log-domain) residual of one ray, bhc/gaussLAlpha.m:10 / bhc/gaussLI.m:3:
+// Err = log(BL/Imea); the NLL term is Err^2/2.
+__device__ __forceinline__ double gauss_err(double bl, double im) { return log(bl / im); }
+
 // ------------------------------------------------------------- alpha pass ---
 
 // w may alias s (each thread reads s[i] before writing w[i]): no __restrict__ on them.
@@ -154,12 +184,20 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
             bd = fmaf(P.Ie[j], d, bd);
             if (HESS) bdd = fmaf(P.Ie[j], dd, bdd);
         });
-        f = nll_term((double)b, (double)im);
-        const float r = 1.0f - im / b;
-        if (w) w[i] = Ts * (r * bd);  // Phit((1-Imea./BL).*sBL) with Phit = Ts*A^T
-        if (HESS && hw) {
-            const float q = bd / b;
-            hw[i] = fmaf(q * q, im, bdd * r);  // bhc/poissLAlpha.m:15
+        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {
+            const double err = gauss_err((double)b, (double)im);
+            f = 0.5 * err * err;
+            const float e = (float)err, q = bd / b;
+            if (w) w[i] = Ts * (e * q);  // Phit(Err.*sBL./BL), bhc/gaussLAlpha.m:16
+            if (HESS && hw) hw[i] = fmaf(1.0f - e, q * q, e * (bdd / b));  // :18
+        } else {
+            f = nll_term((double)b, (double)im);
+            const float r = 1.0f - im / b;
+            if (w) w[i] = Ts * (r * bd);  // Phit((1-Imea./BL).*sBL) with Phit = Ts*A^T
+            if (HESS && hw) {
+                const float q = bd / b;
+                hw[i] = fmaf(q * q, im, bdd * r);  // bhc/poissLAlpha.m:15
+            }
         }
     }
     const double tot = block_sum(f, sh);
@@ -170,8 +208,10 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 
 // Each work-group takes kBhcBlock rays: the basis values phi_ij go to LDS, then thread v
 // forms value v of [H upper triangle (row-major j<=k), g, f] over the group's rays:
-//   H_jk = sum_i phi_ij phi_ik Imea_i/Ir_i^2,  g_j = sum_i phi_ij (1 - Imea_i/Ir_i),
-//   f = sum_i nll(Ir_i, Imea_i),  Ir_i = sum_j Ie_j phi_ij (FP64)   (bhc/poissLI.m).
+//   H_jk = sum_i phi_ij phi_ik q_i,  g_j = sum_i phi_ij r_i,  f = sum_i f_i,
+//   Ir_i = sum_j Ie_j phi_ij (FP64), with
+//   Poisson  (bhc/poissLI.m):  q = Imea/Ir^2,       r = 1 - Imea/Ir,  f_i = nll(Ir, Imea)
+//   Gaussian (bhc/gaussLI.m):  q = (1 - Err)/Ir^2,  r = Err/Ir,       f_i = Err^2/2.
 __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s,
                                                           const float* __restrict__ Imea,
                                                           long long count, float sscale,
@@ -190,9 +230,16 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
             phi[t][j] = p;
             ir = fma(P.Ied[j], (double)p, ir);
         });
-        q[t] = im / (ir * ir);
-        r[t] = 1.0 - im / ir;
-        ft[t] = nll_term(ir, im);
+        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
+            const double err = gauss_err(ir, im);
+            q[t] = (1.0 - err) / (ir * ir);
+            r[t] = err / ir;
+            ft[t] = 0.5 * err * err;
+        } else {
+            q[t] = im / (ir * ir);
+            r[t] = 1.0 - im / ir;
+            ft[t] = nll_term(ir, im);
+        }
     } else {
         for (int j = 0; j < E; ++j) phi[t][j] = 0.0f;
         q[t] = 0.0;
@@ -256,19 +303,19 @@ __global__ void k_unpack_ie(const double* __restrict__ red, int E, double* __res
 
 // -------------------------------------------------------------------- launchers ---
 
-void launch_b1_iout(const BhcParams& P, const float* s, long long count, float* BL, float* sBL,
-                    float* ssBL, bool matrix, hipStream_t st)
+void launch_poly_iout(const BhcParams& P, const float* s, long long count, float* BL,
+                      float* sBL, float* ssBL, bool matrix, hipStream_t st)
 {
     const long long blocks = bhc_blocks(count);
     if (blocks <= 0) return;
     if (matrix || ssBL)
-        hipLaunchKernelGGL(k_b1_iout<3>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
+        hipLaunchKernelGGL(k_poly_iout<3>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
                            count, BL, sBL, ssBL, matrix ? 1 : 0);
     else if (sBL)
-        hipLaunchKernelGGL(k_b1_iout<2>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
+        hipLaunchKernelGGL(k_poly_iout<2>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
                            count, BL, sBL, ssBL, 0);
     else
-        hipLaunchKernelGGL(k_b1_iout<1>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
+        hipLaunchKernelGGL(k_poly_iout<1>, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s,
                            count, BL, sBL, ssBL, 0);
 }
 

@@ -223,23 +223,15 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
         return fail(CTPRJ_EINVAL, "ctprj_pnpg_create: NULL argument");
     if (!(t0 > 0.0) || !std::isfinite(t0)) return fail(CTPRJ_EINVAL, "t0 must be > 0");
     *out = nullptr;
-    if (E < 1 || E > CTPRJ_MAX_E) return fail(CTPRJ_EINVAL, "E=%d out of range", E);
+    BhcParams bp;
+    int rc0 = make_bhc_params(knots, E, Ie, plan->bhc_basis, plan->bhc_noise, &bp);
+    if (rc0) return rc0;
     ctprj_pnpg* s = new (std::nothrow) ctprj_pnpg();
     if (!s) return fail(CTPRJ_ENOMEM, "host allocation");
     s->plan = plan;
     s->d_Imea = d_Imea;
     s->Ts = Ts;
-    BhcParams& P = s->bp;
-    P.E = E;
-    for (int j = 0; j < E + 2; ++j) P.knot[j] = (float)knots[j];
-    for (int j = 0; j < E + 1; ++j) {
-        if (!(knots[j + 1] > knots[j])) { delete s; return fail(CTPRJ_EINVAL, "knots must increase"); }
-        P.delta[j] = (float)(knots[j + 1] - knots[j]);
-    }
-    for (int j = 0; j < E; ++j) {
-        P.Ie[j] = (float)Ie[j];
-        P.Ied[j] = Ie[j];
-    }
+    s->bp = bp;
     s->pp = PnpgParams{0.5, 0.9, 2.0, 0.25, 1, 0, 1};
     s->nimg = plan->img_elems;
     s->nsino = plan->sino_elems;

@@ -59,6 +59,7 @@ struct Plan {
     double* d_red = nullptr;     // reduction partials (beam-hardening)
     double* d_red_out = nullptr; // reduced sums
     long long red_cap = 0;       // doubles held by d_red
+    int bhc_basis = CTPRJ_BASIS_B1, bhc_noise = CTPRJ_NOISE_POISSON;  // spectrum model
 
     ~Plan();
     int build_tables();          // geometry tables for the current tune.flags

@@ -1,5 +1,6 @@
 """On-device PNPG alpha step (npg/PNPG.m:50-167, non-negativity prox) over a projector
-plan and the polychromatic Poisson objective of bhc/poissLAlpha.m.
+plan and the polychromatic objective of bhc/poissLAlpha.m (or bhc/gaussLAlpha.m), in the
+b1 / b0 / dis spectrum basis.
 
 ``PNPG.step(alpha)`` enqueues one line-search trial with no host synchronisation, so a
 run of trials can be captured once in a HIP graph (``torch.cuda.CUDAGraph``) and
@@ -13,16 +14,18 @@ import numpy as np
 import torch
 
 from ._lib import PnpgInfo, check, dbl_array, lib
-from .bhc import _knots_ie, poissLAlpha
+from .bhc import _knots_ie, gaussLAlpha, poissLAlpha, set_model
 from .projector import Projector, _ptr, _stream_ptr
 
 
-def bb_step(proj: Projector, Imea, Ie, alpha, Ts, knots, rel=1e-5) -> float:
+def bb_step(proj: Projector, Imea, Ie, alpha, Ts, knots, rel=1e-5, basis: str = "b1",
+            noise: str = "poisson") -> float:
     """Barzilai-Borwein estimate of the inverse step (npg/Methods.m:100-107):
     t = |<g1 - g2, d>| / ||d||^2 with d = rel * g1/||g1||, g2 = grad(alpha - d)."""
-    _, g1 = poissLAlpha(Imea, Ie, alpha, proj, Ts, knots)
+    obj = gaussLAlpha if noise.lower() == "gaussian" else poissLAlpha
+    _, g1 = obj(Imea, Ie, alpha, proj, Ts, knots, basis=basis)
     d = rel * g1 / torch.linalg.vector_norm(g1.double()).float()
-    _, g2 = poissLAlpha(Imea, Ie, alpha - d, proj, Ts, knots)
+    _, g2 = obj(Imea, Ie, alpha - d, proj, Ts, knots, basis=basis)
     num = torch.dot((g1 - g2).double(), d.double()).abs().item()
     den = torch.dot(d.double(), d.double()).item()
     t = num / den if den > 0 else 1.0
@@ -31,7 +34,8 @@ def bb_step(proj: Projector, Imea, Ie, alpha, Ts, knots, rel=1e-5) -> float:
 
 class PNPG:
     def __init__(self, proj: Projector, Imea: torch.Tensor, knots, Ie, Ts: float,
-                 alpha0: torch.Tensor, t0: float | None = None):
+                 alpha0: torch.Tensor, t0: float | None = None, basis: str = "b1",
+                 noise: str = "poisson"):
         self.proj = proj
         self.Imea = Imea.reshape(-1).contiguous()
         self.knots = np.asarray(knots, np.float64)
@@ -39,9 +43,11 @@ class PNPG:
         self.Ts = float(Ts)
         a0 = alpha0.reshape(-1).contiguous()
         if t0 is None:
-            t0 = bb_step(proj, self.Imea, self.Ie, a0, self.Ts, self.knots)
+            t0 = bb_step(proj, self.Imea, self.Ie, a0, self.Ts, self.knots, basis=basis,
+                         noise=noise)
         self.t0 = float(t0)
-        E, kp, self._kh, ip, self._ih = _knots_ie(self.knots, self.Ie)
+        E, kp, self._kh, ip, self._ih = _knots_ie(self.knots, self.Ie, basis)
+        set_model(proj, basis, noise)  # the solver keeps the model it is created with
         h = ctypes.c_void_p()
         check(lib().ctprj_pnpg_create(proj.handle, _ptr(self.Imea), kp, E, ip, self.Ts, _ptr(a0),
                                       ctypes.c_double(self.t0), ctypes.byref(h)))

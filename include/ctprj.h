@@ -132,26 +132,46 @@ int ctprj_adjoint(ctprj_plan* plan, const float* d_sino, float* d_img, void* str
 int ctprj_fbp(ctprj_plan* plan, const float* d_sino, float* d_img, void* stream);
 
 /* ------------------------------------------------------------------------------
- * Polychromatic Poisson beam-hardening model (bhc/poissLAlpha.m, bhc/poissLI.m,
- * bhc/Spline.m:b1Iout).  `knots` holds the E+2 hat-basis nodes
- * [kappa_1/q, kappa_1..kappa_E, kappa_E*q] (bhc/beamhardenSpline.m:164-165); Ie the E
- * basis coefficients; Ts the operator scale (Phi = Ts*A, tomography/ConfigCT.m:118-119).
- * Host arrays for knots/Ie (E <= CTPRJ_MAX_E); device arrays for images/sinograms.
- * NLL and I-epsilon reductions are FP64; projections are FP32.
+ * Polychromatic beam-hardening model (bhc/poissLAlpha.m, bhc/poissLI.m,
+ * bhc/gaussLAlpha.m, bhc/gaussLI.m, bhc/Spline.m polyIout).  Ie holds the E basis
+ * coefficients of the mass-attenuation spectrum; `knots` the basis nodes:
+ *   CTPRJ_BASIS_B1  (Spline 'b1', hats):  E+2 knots [kappa_1/q, kappa_1..kappa_E, kappa_E*q]
+ *                   (bhc/beamhardenSpline.m:164-165);
+ *   CTPRJ_BASIS_B0  (Spline 'b0', boxes): E+1 knots, box j = [k_j, k_{j+1}];
+ *   CTPRJ_BASIS_DIS (Spline 'dis'):       E+2 knots, node j = k_{j+1} with weight
+ *                   (k_{j+2} - k_j)/2 (bhc/Spline.m:271-272).
+ * Ts is the operator scale (Phi = Ts*A, tomography/ConfigCT.m:118-119).  Host arrays for
+ * knots/Ie (E <= CTPRJ_MAX_E); device arrays for images/sinograms.  NLL and I-epsilon
+ * reductions are FP64; projections are FP32.
  * ------------------------------------------------------------------------------ */
+enum { CTPRJ_BASIS_B1 = 0, CTPRJ_BASIS_B0 = 1, CTPRJ_BASIS_DIS = 2 };
+enum { CTPRJ_NOISE_POISSON = 0, CTPRJ_NOISE_GAUSSIAN = 1 };
 
-/* Spectrum sum of bhc/Spline.m:122-203 on device: for every ray value s[i],
- * BL[i] = sum_j Ie[j] phi_j(s[i]), sBL = d/ds BL, ssBL = d2/ds2 BL.  Any of the outputs
- * may be NULL.  With Ie == NULL the basis matrix phi_j(s[i]) is written to BL as
- * count x E (row-major) instead. */
+/* Spectrum sum of bhc/Spline.m (b1Iout :122-203, b0Iout :205-262, disIout :264-302) on
+ * device: for every ray value s[i], BL[i] = sum_j Ie[j] phi_j(s[i]), sBL = d/ds BL,
+ * ssBL = d2/ds2 BL.  Any of the outputs may be NULL.  With Ie == NULL the basis matrix
+ * phi_j(s[i]) is written to BL as count x E (row-major) instead. */
+int ctprj_poly_iout(int basis, const double* knots, int E, const double* Ie,
+                    const float* d_s, long long count, float* d_BL, float* d_sBL,
+                    float* d_ssBL, void* stream);
+/* ctprj_poly_iout with CTPRJ_BASIS_B1 (bhc/Spline.m:122-203). */
 int ctprj_b1_iout(const double* knots, int E, const double* Ie, const float* d_s,
                   long long count, float* d_BL, float* d_sBL, float* d_ssBL, void* stream);
+
+/* Spectrum basis and noise model of the plan's beam-hardening calls below
+ * (bhc/beamhardenSpline.m:60 opt.spectBasis, :155-162 opt.noiseType); a new plan starts
+ * at CTPRJ_BASIS_B1 / CTPRJ_NOISE_POISSON.  Gaussian: Err = log(BL/Imea), f = Err'Err/2
+ * (bhc/gaussLAlpha.m, bhc/gaussLI.m). */
+int ctprj_plan_set_bhc_model(ctprj_plan* plan, int basis, int noise);
+int ctprj_plan_get_bhc_model(const ctprj_plan* plan, int* basis, int* noise);
 
 /* [f, g] = poissLAlpha(Imea, Ie, alpha, Phi, Phit, polyIout) (bhc/poissLAlpha.m:1-13):
  * s = Ts*A*alpha, f = sum(BL - Imea) - sum Imea*log(BL/Imea) (FP64, 0*log(.) = 0 for
  * Imea = 0), grad = Ts*A^T[(1 - Imea/BL) * sBL].  d_f: one device double.
  * d_grad may be NULL (cost only: one forward projection).  d_hw, if non-NULL, receives
- * the Hessian ray weight (sBL/BL)^2*Imea + ssBL*(1 - Imea/BL) (bhc/poissLAlpha.m:15). */
+ * the Hessian ray weight (sBL/BL)^2*Imea + ssBL*(1 - Imea/BL) (bhc/poissLAlpha.m:15).
+ * Under CTPRJ_NOISE_GAUSSIAN this is gaussLAlpha (bhc/gaussLAlpha.m:1-21): f = Err'Err/2,
+ * grad = Ts*A^T[Err * sBL/BL], Hessian weight (1 - Err)(sBL/BL)^2 + Err*ssBL/BL. */
 int ctprj_bhc_grad_alpha(ctprj_plan* plan, const float* d_alpha, const float* d_Imea,
                          const double* knots, int E, const double* Ie, float Ts,
                          double* d_f, float* d_grad, float* d_hw, void* stream);
@@ -171,14 +191,15 @@ int ctprj_bhc_alpha_rays(ctprj_plan* plan, const float* d_sraw, const float* d_I
  * H = A^T diag(Imea/Ir^2) A (E x E doubles, row-major) -- FP64 sums, written to device
  * memory; the ray x basis matrix A is never formed.  Ray values are s = Ts * d_sraw with
  * d_sraw = A*alpha (packed); pass d_sraw = NULL together with d_alpha to project inside.
- * d_f / d_g / d_H may be NULL. */
+ * d_f / d_g / d_H may be NULL.  Under CTPRJ_NOISE_GAUSSIAN this is gaussLI
+ * (bhc/gaussLI.m): f = Err'Err/2, g = A^T(Err/Ir), H = A^T diag((1 - Err)/Ir^2) A. */
 int ctprj_bhc_grad_ie(ctprj_plan* plan, const float* d_alpha, const float* d_sraw,
                       const float* d_Imea, const double* knots, int E, const double* Ie,
                       float Ts, double* d_f, double* d_g, double* d_H, void* stream);
 
 /* ------------------------------------------------------------------------------
  * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
- * poissLAlpha objective): no reference counterpart as a C call -- the reference runs
+ * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference counterpart as a C call -- the reference runs
  * this loop in MATLAB (bhc/beamhardenSpline.m:340-384 -> npg/NPG.m/PNPG.m).
  * ------------------------------------------------------------------------------ */
 typedef struct ctprj_pnpg ctprj_pnpg;

@@ -7,14 +7,17 @@ Follows, line by line:
   * ``dis_iout`` -- bhc/Spline.m:264-302
   * ``poiss_l_alpha`` -- bhc/poissLAlpha.m:1-19
   * ``poiss_l_i``     -- bhc/poissLI.m:1-26
+  * ``gauss_l_alpha`` -- bhc/gaussLAlpha.m:1-21
+  * ``gauss_l_i``     -- bhc/gaussLI.m:1-12
   * knots / kappa grid -- bhc/beamhardenSpline.m:138-139,164-165,182
+plus the moment forms ``b1_iout_stable`` / ``b0_iout_stable`` (the kernels' formulas).
 
 Pinned without MATLAB (the reference's BHC code is MATLAB and cannot run here) by:
-quadrature of phi_j(s) = int hat_j(kappa) exp(-kappa s) dkappa (scipy.integrate.quad),
-the closed value phi_j(0) = (c-a)/2, an independent stable evaluation
-(``b1_iout_quad``), central finite differences of the likelihoods, and the identity
-g . Ie = sum(Ir - Imea) of poissLI.  Exact MATLAB outputs of the Taylor branch are
-"parity unpinned" beyond those checks.
+quadrature of phi_j(s) = int basis_j(kappa) exp(-kappa s) dkappa for the hat and box
+bases (scipy.integrate.quad), the closed values phi_j(0), finite differences of the
+dis basis' derivatives, central finite differences of the likelihoods, and the
+identity g . Ie = sum(Ir - Imea) of poissLI.  Exact MATLAB outputs of the Taylor
+branches are "parity unpinned" beyond those checks.
 """
 from __future__ import annotations
 
@@ -235,6 +238,38 @@ def dis_iout(k, s, I=None, nout=3):
     return BL, sBL, ssBL
 
 
+def b0_iout_stable(k, s, I=None, nout=3):
+    """The B0 (box) spectrum transform by segment moments, smooth in s: box j = [a, b],
+    Delta = b - a, w = Delta exp(-a s), z = Delta s:
+      phi = w M0,  phi' = -w (a M0 + Delta M1),  phi'' = w (a^2 M0 + 2 a Delta M1 + Delta^2 M2)
+    (the kernels' formula, in FP64)."""
+    k = np.asarray(k, np.float64).ravel()
+    s = np.asarray(s, np.float64).ravel()
+    E = len(k) - 1
+    P = np.zeros((3, len(s), E))
+    for j in range(E):
+        a, d = k[j], k[j + 1] - k[j]
+        w = d * np.exp(-a * s)
+        M0, M1, M2, _ = _moments(d * s)
+        P[0, :, j] = w * M0
+        P[1, :, j] = -w * (a * M0 + d * M1)
+        P[2, :, j] = w * (a * a * M0 + 2 * a * d * M1 + d * d * M2)
+    if I is None:
+        return P[0], P[1], P[2]
+    I = np.asarray(I, np.float64)
+    return P[0] @ I, P[1] @ I, P[2] @ I
+
+
+def b0_iout_quad(k, s, j):
+    """phi_j(s), phi_j'(s), phi_j''(s) of box j by adaptive quadrature (independent pin)."""
+    from scipy.integrate import quad
+    a, b = k[j], k[j + 1]
+    f0 = quad(lambda x: np.exp(-x * s), a, b, epsabs=0, epsrel=1e-13, limit=200)[0]
+    f1 = -quad(lambda x: x * np.exp(-x * s), a, b, epsabs=0, epsrel=1e-13, limit=200)[0]
+    f2 = quad(lambda x: x * x * np.exp(-x * s), a, b, epsabs=0, epsrel=1e-13, limit=200)[0]
+    return f0, f1, f2
+
+
 def _nll(BL, Imea):
     """sum(BL-Imea) - Imea'*log(BL./Imea) with 0*log(.) = 0 for Imea = 0."""
     BL = np.asarray(BL, np.float64)
@@ -261,6 +296,30 @@ def poiss_l_i(Imea, A, Ie):
     g = A.T @ (1 - ratio)
     f = _nll(Ir, Imea)
     w = ratio / Ir
+    H = A.T @ (A * w[:, None])
+    return f, g, H
+
+
+def gauss_l_alpha(Imea, Ie, alpha, Phi, Phit, polyIout=b1_iout, knots=None):
+    """bhc/gaussLAlpha.m: Err = log(BL/Imea), f = Err'Err/2, g = Phit(Err.*sBL./BL),
+    Hessian ray weight (1-Err).*(sBL./BL).^2 + Err.*ssBL./BL."""
+    R = Phi(alpha)
+    BL, sBL, ssBL = polyIout(knots, R, Ie)
+    Err = np.log(BL / Imea)
+    f = Err @ Err / 2
+    g = Phit(Err * sBL / BL)
+    hw = (1 - Err) * (sBL / BL) ** 2 + Err * ssBL / BL
+    return f, g, hw
+
+
+def gauss_l_i(Imea, A, Ie):
+    """bhc/gaussLI.m: Ir = A Ie, Err = log(Ir./Imea), f = Err'Err/2, g = A'(Err./Ir),
+    H = A' diag((1-Err)./Ir.^2) A."""
+    Ir = A @ np.asarray(Ie, np.float64)
+    Err = np.log(Ir / Imea)
+    f = Err @ Err / 2
+    g = A.T @ (Err / Ir)
+    w = (1 - Err) / Ir ** 2
     H = A.T @ (A * w[:, None])
     return f, g, H
 

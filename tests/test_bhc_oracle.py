@@ -7,7 +7,12 @@ transform used by the HIP kernels:
 * the Taylor branch of bhc/Spline.m:156-165 continuous at its threshold;
 * poissLAlpha / poissLI gradients and Hessians against central finite differences
   (h = 1e-3 as in the workload definition, and smaller), and the identity
-  g . Ie = sum(Ir - Imea) of poissLI; gradient 0 at Imea = Ir."""
+  g . Ie = sum(Ir - Imea) of poissLI; gradient 0 at Imea = Ir;
+* the box (b0) basis against quadrature of the box Laplace transform, its reference
+  form against the moment form; the discrete (dis) basis' derivatives against finite
+  differences and its value at s = 0;
+* gaussLAlpha / gaussLI gradients and Hessians against finite differences, gradient 0
+  at Imea = Ir."""
 import numpy as np
 import pytest
 
@@ -139,3 +144,73 @@ def test_zero_counts_guard():
     Imea[::7] = 0.0
     f, g, H = B.poiss_l_i(Imea, Amat, Ie)
     assert np.isfinite(f) and np.all(np.isfinite(g)) and np.all(np.isfinite(H))
+
+
+KN0 = np.logspace(np.log10(0.03), np.log10(30.0), 21)   # b0: 21 edges -> 20 boxes
+
+
+@pytest.mark.parametrize("s", [0.0, 1e-7, 1e-3, 0.05, 0.4, 2.0, 9.0, 30.0, -0.01])
+def test_b0_against_quadrature(s):
+    stable = B.b0_iout_stable(KN0, np.array([s]), None)
+    ref = B.b0_iout(KN0, np.array([s]), None)
+    ref0 = B.b0_iout(KN0, np.array([0.0]), None)
+    for j in range(0, 20, 3):
+        q = B.b0_iout_quad(KN0, s, j)
+        for p in range(3):
+            assert abs(stable[p][0, j] - q[p]) <= 1e-10 * abs(q[p]) + 1e-300, (s, j, p)
+            # the reference's closed form + Taylor branch: its own 1e-6-of-phi(0) target
+            assert abs(ref[p][0, j] - q[p]) <= 2e-6 * abs(ref0[p][0, j]) + 1e-9 * abs(q[p])
+    assert np.allclose(B.b0_iout_stable(KN0, np.array([0.0]), None)[0][0], np.diff(KN0),
+                       rtol=1e-14)
+
+
+def test_dis_basis_derivatives_and_zero():
+    s = np.logspace(-3, 1, 30)
+    BL, sBL, ssBL = B.dis_iout(KN, s, None)
+    h = 1e-6
+    Bp = B.dis_iout(KN, s + h, None)
+    Bm = B.dis_iout(KN, s - h, None)
+    assert np.allclose((Bp[0] - Bm[0]) / (2 * h), sBL, rtol=1e-6, atol=1e-12)
+    assert np.allclose((Bp[1] - Bm[1]) / (2 * h), ssBL, rtol=1e-6, atol=1e-12)
+    BL0, _, _ = B.dis_iout(KN, np.array([0.0]), None)
+    assert np.allclose(BL0[0], (KN[2:] - KN[:-2]) / 2, rtol=1e-14)
+
+
+def test_gauss_l_alpha_finite_differences():
+    A, alpha, Ie, Imea = _toy_problem(seed=5)
+    Phi, Phit = (lambda x: A @ x), (lambda y: A.T @ y)
+    pol = lambda k, s, I: B.b1_iout_stable(k, s, I)  # noqa: E731
+    f, g, hw = B.gauss_l_alpha(Imea, Ie, alpha, Phi, Phit, pol, KN)
+    rng = np.random.default_rng(6)
+    for _ in range(5):
+        v = rng.standard_normal(alpha.size)
+        fun = lambda t, v=v: B.gauss_l_alpha(Imea, Ie, alpha + t * v, Phi, Phit, pol, KN)[0]  # noqa: E731
+        _, rich = central(fun, 1e-3)
+        gv = g @ v
+        assert abs(rich - gv) <= 1e-7 * abs(gv)
+        h = 1e-3
+        fdd = (fun(h) - 2 * f + fun(-h)) / h ** 2
+        hv = (A @ v) @ (hw * (A @ v))
+        assert abs(fdd - hv) <= 1e-3 * abs(hv)
+    # a planted sign error in the Hessian weight is caught by the same check
+    bad = -hw
+    v = rng.standard_normal(alpha.size)
+    fun = lambda t: B.gauss_l_alpha(Imea, Ie, alpha + t * v, Phi, Phit, pol, KN)[0]  # noqa: E731
+    fdd = (fun(1e-3) - 2 * f + fun(-1e-3)) / 1e-6
+    assert abs(fdd - (A @ v) @ (bad * (A @ v))) > 0.5 * abs(fdd)
+
+
+def test_gauss_l_i_finite_differences():
+    A, alpha, Ie, Imea = _toy_problem(seed=7)
+    Amat, _, _ = B.b1_iout_stable(KN, A @ alpha, None)
+    f, g, H = B.gauss_l_i(Imea, Amat, Ie)
+    for j in range(20):
+        e = np.zeros(20)
+        e[j] = 1.0
+        _, rich = central(lambda t: B.gauss_l_i(Imea, Amat, Ie + t * e)[0], 1e-3)
+        assert abs(rich - g[j]) <= 1e-7 * np.abs(g).max()
+        _, gd = central(lambda t: B.gauss_l_i(Imea, Amat, Ie + t * e)[1], 1e-3)
+        assert np.allclose(gd, H[:, j], rtol=1e-6, atol=1e-9 * np.abs(H).max())
+    Ir = Amat @ Ie
+    f0, g0, _ = B.gauss_l_i(Ir, Amat, Ie)
+    assert f0 == 0.0 and np.abs(g0).max() == 0.0

@@ -172,3 +172,100 @@ def test_zero_counts(problem):
     im[::5] = 0
     f, g = poissLAlpha(im, P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"])
     assert np.isfinite(f.item()) and bool(g.isfinite().all())
+
+
+# ---------------------------------------------------------------- model variants ---
+# Spectrum bases of bhc/Spline.m ('b0' boxes on E+1 edges, 'dis' nodes) and the Gaussian
+# noise model of bhc/gaussLAlpha.m / bhc/gaussLI.m, against the FP64 restatements.
+
+KN0 = np.logspace(np.log10(0.03), np.log10(30.0), E + 1)
+
+
+def _basis(P, basis):
+    from oracle import bhc_oracle as B
+    if basis == "b0":
+        return KN0, (lambda k, s, I: B.b0_iout_stable(k, s, I))
+    if basis == "dis":
+        return P["knots"], (lambda k, s, I: B.dis_iout(k, s, I))
+    return P["knots"], (lambda k, s, I: B.b1_iout_stable(k, s, I))
+
+
+@pytest.mark.parametrize("basis", ["b0", "dis"])
+def test_poly_iout_bases_device(problem, basis):
+    import torch
+    from imgrecsrc_amd.bhc import Spline
+    s = np.concatenate([[0.0, 1e-7, 1e-4], np.logspace(-3, 1.3, 500), [-0.02]]).astype(np.float32)
+    knots, pol = _basis(problem, basis)
+    st = torch.from_numpy(s).cuda()
+    sp = Spline(basis, knots)
+    got = [t.cpu().numpy() for t in sp.polyIout(st, problem["Ie"])]
+    ref = pol(knots, s.astype(np.float64), problem["Ie"])
+    for g, r in zip(got, ref):
+        assert np.all(np.abs(g - r) <= 1e-5 * np.abs(r[0]) + 1e-5 * np.abs(r))
+    A = sp.polyIout(st, None).cpu().numpy()
+    Ar = pol(knots, s.astype(np.float64), None)[0]
+    assert np.all(np.abs(A - Ar) <= 1e-5 * np.abs(Ar[0])[None, :] + 1e-5 * np.abs(Ar))
+
+
+@pytest.mark.parametrize("basis,noise", [("b1", "gaussian"), ("b0", "poisson"),
+                                         ("dis", "poisson"), ("dis", "gaussian")])
+def test_alpha_objective_variants(problem, oracle_mod, basis, noise):
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import gaussLAlpha, poissLAlpha
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    knots, pol = _basis(P, basis)
+    proj = Projector([GEO])
+    fn = gaussLAlpha if noise == "gaussian" else poissLAlpha
+    f, g, hw = fn(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], knots, want_hess=True,
+                  basis=basis)
+    cf, Ts = P["cf"], P["Ts"]
+    Phi = lambda a: Ts * oracle_mod.forward(cf, a.reshape(GEO[0], GEO[0]))[0].ravel()  # noqa: E731
+    Phit = lambda y: Ts * oracle_mod.adjoint(cf, y.reshape(GEO[2], GEO[1]))[0].ravel()  # noqa: E731
+    ofn = B.gauss_l_alpha if noise == "gaussian" else B.poiss_l_alpha
+    f64, g64, hw64 = ofn(P["Imea"], P["Ie"], P["alpha"].ravel(), Phi, Phit, pol, knots)
+    ef, eg = abs(f.item() - f64) / abs(f64), rel_l2(g.cpu().numpy(), g64)
+    print(f"{noise}/{basis} alpha objective: f rel {ef:.2e}, grad rel L2 {eg:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5
+    assert rel_l2(hw.cpu().numpy(), hw64) <= 1e-4
+
+
+@pytest.mark.parametrize("basis,noise", [("b1", "gaussian"), ("b0", "poisson"),
+                                         ("dis", "gaussian")])
+def test_ie_objective_variants(problem, oracle_mod, basis, noise):
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import gaussLI, poissLI
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    knots, pol = _basis(P, basis)
+    proj = Projector([GEO])
+    fn = gaussLI if noise == "gaussian" else poissLI
+    f, g, H = fn(P["Imea_t"], P["Ie"], proj, P["Ts"], knots, alpha=P["alpha_t"], basis=basis)
+    raw, _ = oracle_mod.forward(P["cf"], P["alpha"])
+    A = pol(knots, P["Ts"] * raw.ravel(), None)[0]
+    ofn = B.gauss_l_i if noise == "gaussian" else B.poiss_l_i
+    f64, g64, H64 = ofn(P["Imea"], A, P["Ie"])
+    ef = abs(f.item() - f64) / abs(f64)
+    eg, eh = rel_l2(g.cpu().numpy(), g64), rel_l2(H.cpu().numpy(), H64)
+    print(f"{noise}/{basis} Ie objective: f rel {ef:.2e}, g rel {eg:.2e}, H rel {eh:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5 and eh <= 1e-5
+
+
+def test_gaussian_gradient_finite_differences(problem):
+    """Central differences of the device Gaussian NLL along alpha directions."""
+    import torch
+    from imgrecsrc_amd.bhc import gaussLAlpha
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    f, g = gaussLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], P["knots"], basis="dis")
+    rng = np.random.default_rng(9)
+    h = 1e-3
+    for _ in range(3):
+        v = torch.from_numpy(rng.standard_normal(GEO[0] ** 2).astype(np.float32)).cuda()
+        fp = gaussLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"] + h * v, proj, P["Ts"], P["knots"],
+                         want_grad=False, basis="dis").item()
+        fm = gaussLAlpha(P["Imea_t"], P["Ie"], P["alpha_t"] - h * v, proj, P["Ts"], P["knots"],
+                         want_grad=False, basis="dis").item()
+        fd, gv = (fp - fm) / (2 * h), torch.dot(g.double(), v.double()).item()
+        assert abs(fd - gv) <= 2e-3 * abs(gv), (fd, gv)
