@@ -53,6 +53,12 @@ class PnpgInfo(ctypes.Structure):
                 ("restarts", ctypes.c_int), ("last_accepted", ctypes.c_int)]
 
 
+class IeInfo(ctypes.Structure):
+    _fields_ = [("cost", ctypes.c_double), ("decrement", ctypes.c_double),
+                ("iterations", ctypes.c_int), ("passes", ctypes.c_int),
+                ("converged", ctypes.c_int)]
+
+
 class Tune(ctypes.Structure):
     _fields_ = [("fwd_block", ctypes.c_int), ("adj_tile", ctypes.c_int),
                 ("flags", ctypes.c_int), ("adj_seg", ctypes.c_int), ("fwd_seg", ctypes.c_int)]
@@ -85,6 +91,8 @@ SIGNATURES = [
     ("ctprj_bhc_grad_alpha", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_alpha_rays", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_grad_ie", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
+    ("ctprj_ie_solve", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _i, ctypes.c_double,
+                            ctypes.POINTER(IeInfo), _vp]),
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),

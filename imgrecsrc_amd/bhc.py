@@ -23,7 +23,7 @@ import ctypes
 import numpy as np
 import torch
 
-from ._lib import MAX_E, check, dbl_array, lib
+from ._lib import MAX_E, IeInfo, check, dbl_array, lib
 from .projector import Projector, _ptr, _stream_ptr
 
 
@@ -209,6 +209,25 @@ def gaussLI(Imea: torch.Tensor, Ie, proj: Projector, Ts: float, knots,
     return _grad_ie("gaussian", Imea, Ie, proj, Ts, knots, alpha, sraw, basis)
 
 
-__all__ = ["Spline", "poissLAlpha", "poissLI", "gaussLAlpha", "gaussLI", "alpha_rays",
+def ie_solve(Imea: torch.Tensor, Ie0, proj: Projector, Ts: float, knots,
+             alpha: torch.Tensor | None = None, sraw: torch.Tensor | None = None,
+             basis: str = "b1", noise: str = "poisson", maxit: int = 50, tol: float = 1e-12):
+    """The I-epsilon step of bhc/beamhardenSpline.m:446-451 (IeStep = ActiveSet): minimise
+    poissLI / gaussLI over Ie >= 0 for the projection of alpha (or sraw = A*alpha).
+    Returns (Ie as a float64 numpy array, info dict)."""
+    if (alpha is None) == (sraw is None):
+        raise ValueError("give exactly one of alpha and sraw")
+    Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
+    E, kp, _kh, ip, ih = _knots_ie(knots, Ie0, basis)
+    set_model(proj, basis, noise)
+    a = _ptr(_f32(alpha.reshape(-1), proj.img_elems, "alpha")) if alpha is not None else None
+    s = _ptr(_f32(sraw.reshape(-1), proj.sino_elems, "sraw")) if sraw is not None else None
+    info = IeInfo()
+    check(lib().ctprj_ie_solve(proj.handle, a, s, _ptr(Imea), kp, E, ip, float(Ts), int(maxit),
+                               float(tol), ctypes.byref(info), _stream_ptr()))
+    return np.array(ih[:E], np.float64), {f: getattr(info, f) for f, _ in IeInfo._fields_}
+
+
+__all__ = ["ie_solve", "Spline", "poissLAlpha", "poissLI", "gaussLAlpha", "gaussLI", "alpha_rays",
            "set_model", "kappa_logspace", "kappa_logspan", "spline_knots", "basis_at_zero",
            "normalize_ie", "BASIS", "NOISE"]

@@ -2,7 +2,9 @@
 // bhc/gaussLAlpha.m, bhc/gaussLI.m, bhc/Spline.m polyIout) on top of a projector plan.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "bhc_internal.h"
 #include "ctprj_internal.h"
@@ -183,4 +185,186 @@ extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const floa
     if ((rc = last_launch("bhc ie reduce launch"))) return rc;
     launch_unpack_ie(p->d_red_out, E, d_f, d_g, d_H, st);
     return last_launch("bhc ie unpack launch");
+}
+
+// ------------------------------------------------------------- I-epsilon solve ---
+
+namespace ctprj {
+
+// Cholesky solve of the SPD system M x = r (n <= CTPRJ_MAX_E, FP64, in place on copies).
+// Returns false when M is not numerically positive definite.
+static bool chol_solve(std::vector<double> M, int n, std::vector<double> r, std::vector<double>* x)
+{
+    for (int j = 0; j < n; ++j) {
+        double d = M[j * n + j];
+        for (int k = 0; k < j; ++k) d -= M[j * n + k] * M[j * n + k];
+        if (!(d > 0.0)) return false;
+        const double l = std::sqrt(d);
+        M[j * n + j] = l;
+        for (int i = j + 1; i < n; ++i) {
+            double v = M[i * n + j];
+            for (int k = 0; k < j; ++k) v -= M[i * n + k] * M[j * n + k];
+            M[i * n + j] = v / l;
+        }
+    }
+    for (int i = 0; i < n; ++i) {
+        double v = r[i];
+        for (int k = 0; k < i; ++k) v -= M[i * n + k] * r[k];
+        r[i] = v / M[i * n + i];
+    }
+    for (int i = n - 1; i >= 0; --i) {
+        double v = r[i];
+        for (int k = i + 1; k < n; ++k) v -= M[k * n + i] * r[k];
+        r[i] = v / M[i * n + i];
+    }
+    *x = r;
+    return true;
+}
+
+}  // namespace ctprj
+
+extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* d_sraw,
+                              const float* d_Imea, const double* knots, int E, double* Ie,
+                              float Ts, int maxit, double tol, ctprj_ie_info* info, void* stream)
+{
+    if (!p || !d_Imea || !Ie) return fail(CTPRJ_EINVAL, "ie_solve: NULL argument");
+    if (!d_alpha && !d_sraw) return fail(CTPRJ_EINVAL, "ie_solve: need alpha or s");
+    if (maxit < 1) return fail(CTPRJ_EINVAL, "ie_solve: maxit must be >= 1");
+    BhcParams P;
+    int rc = make_params(p, knots, E, Ie, &P);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
+    const int nv = bhc_ie_nvals(E);
+    if ((rc = ensure_red(p, blocks * nv))) return rc;
+    const float* s = d_sraw;
+    if (!s) {  // the projection stays fixed over the I-epsilon step
+        if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
+        s = p->d_sino_ws;
+    }
+    std::vector<double> x(E), red(nv);
+    for (int j = 0; j < E; ++j) x[j] = std::max(Ie[j], 0.0);
+    int passes = 0;
+    auto set_ie = [&](const std::vector<double>& v) {
+        for (int j = 0; j < E; ++j) {
+            P.Ie[j] = (float)v[j];
+            P.Ied[j] = v[j];
+        }
+    };
+    // f, g, H at x (FP64 ray sums on the device)
+    auto full = [&](double* f, std::vector<double>* g, std::vector<double>* H) -> int {
+        set_ie(x);
+        launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st);
+        launch_reduce_partials(p->d_red, blocks, nv, p->d_red_out, st);
+        hipError_t e = hipMemcpyAsync(red.data(), p->d_red_out, sizeof(double) * nv,
+                                      hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "ie_solve pass");
+        ++passes;
+        const int nh = E * (E + 1) / 2;
+        g->assign(red.begin() + nh, red.begin() + nh + E);
+        H->assign((size_t)E * E, 0.0);
+        for (int v = 0, j = 0, k = 0; v < nh; ++v) {
+            (*H)[j * E + k] = (*H)[k * E + j] = red[v];
+            if (++k == E) k = ++j;
+        }
+        *f = red[nh + E];
+        return 0;
+    };
+    auto cost = [&](const std::vector<double>& v, double* f) -> int {
+        set_ie(v);
+        launch_bhc_ie_cost(P, s, Ts, d_Imea, rays, p->d_red, st);
+        launch_reduce_partials(p->d_red, blocks, 1, p->d_red_out, st);
+        hipError_t e = hipMemcpyAsync(f, p->d_red_out, sizeof(double), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "ie_solve cost pass");
+        ++passes;
+        return 0;
+    };
+    double f = 0.0, dec = 0.0, lam = 1e-3;
+    std::vector<double> g, H;
+    int it = 0, converged = 0;
+    if ((rc = full(&f, &g, &H))) return rc;
+    for (it = 0; it < maxit; ++it) {
+        // two-metric projected Newton (Bertsekas) with a Levenberg-Marquardt shift:
+        // eps-active bounds take a diagonally scaled step, the free block a damped Newton
+        // step; c_j = max(|H_jj|, 1e-6 max|H_kk|) keeps the scale positive where H is
+        // indefinite (gaussLI's Hessian where Err > 1)
+        double w2 = 0.0, xmax = 0.0, hmax = 0.0;
+        for (int j = 0; j < E; ++j) {
+            const double pg = x[j] - std::max(x[j] - g[j], 0.0);
+            w2 += pg * pg;
+            xmax = std::max(xmax, x[j]);
+            hmax = std::max(hmax, std::fabs(H[j * E + j]));
+        }
+        const double eps = std::min(1e-2 * xmax, std::sqrt(w2));
+        std::vector<double> c(E), d(E, 0.0);
+        std::vector<int> F, A;
+        for (int j = 0; j < E; ++j) {
+            c[j] = std::max(std::fabs(H[j * E + j]), 1e-6 * std::max(hmax, 1e-300));
+            if (x[j] <= eps && g[j] > 0.0) {
+                A.push_back(j);
+                d[j] = g[j] / c[j];
+            } else {
+                F.push_back(j);
+            }
+        }
+        const int nf = (int)F.size();
+        if (nf > 0) {
+            std::vector<double> M((size_t)nf * nf), r(nf), sol;
+            for (int a = 0; a < nf; ++a) {
+                r[a] = g[F[a]];
+                for (int b = 0; b < nf; ++b) M[a * nf + b] = H[F[a] * E + F[b]];
+            }
+            for (;;) {
+                std::vector<double> Ml = M;
+                for (int a = 0; a < nf; ++a) Ml[a * nf + a] += lam * c[F[a]];
+                if (chol_solve(Ml, nf, r, &sol) || lam > 1e30) break;
+                lam *= 10.0;
+            }
+            if (sol.size() == (size_t)nf)
+                for (int a = 0; a < nf; ++a) d[F[a]] = sol[a];
+        }
+        double gdF = 0.0;
+        for (int j : F) gdF += g[j] * d[j];
+        dec = gdF;
+        for (int j : A) dec += g[j] * std::min(x[j], d[j]);
+        if (!(dec > tol * std::max(std::fabs(f), 1.0))) {
+            converged = 1;
+            break;
+        }
+        // backtracking on the projected path x(t) = max(x - t d, 0), halving, with
+        // f(x(t)) <= f - 1e-4 (t g_F.d_F + g_A.(x_A - x_A(t)))
+        double t = 1.0, fn = 0.0;
+        bool ok = false;
+        std::vector<double> xn(E);
+        for (int trial = 0; trial < 30 && !ok; ++trial) {
+            double ga = 0.0;
+            for (int j = 0; j < E; ++j) xn[j] = std::max(x[j] - t * d[j], 0.0);
+            for (int j : A) ga += g[j] * (x[j] - xn[j]);
+            if ((rc = cost(xn, &fn))) return rc;
+            ok = fn <= f - 1e-4 * (t * gdF + ga);
+            if (!ok) t *= 0.5;
+        }
+        if (!ok) {  // stiffen the damping and retry from the same point
+            lam *= 10.0;
+            if (lam > 1e12) {
+                converged = 1;
+                break;
+            }
+            continue;
+        }
+        lam = (t == 1.0) ? std::max(lam / 3.0, 1e-12) : lam * 4.0;
+        x = xn;
+        if ((rc = full(&f, &g, &H))) return rc;
+    }
+    for (int j = 0; j < E; ++j) Ie[j] = x[j];
+    if (info) {
+        info->cost = f;
+        info->decrement = dec;
+        info->iterations = it;
+        info->passes = passes;
+        info->converged = converged;
+    }
+    return 0;
 }

@@ -42,6 +42,9 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
 // per-ray I-epsilon pass: per work-group partial [H upper, g, f]
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                        long long count, double* partials, hipStream_t st);
+// cost only of the I-epsilon pass: one FP64 partial per work-group
+void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
+                        long long count, double* partials, hipStream_t st);
 // out[v] = sum_p partials[p*nvals + v] in a fixed order
 void launch_reduce_partials(const double* partials, long long nparts, int nvals, double* out,
                             hipStream_t st);

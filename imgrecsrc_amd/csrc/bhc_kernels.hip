@@ -267,6 +267,34 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
     }
 }
 
+// Cost only of the I-epsilon pass: f_i of every ray with Ir_i formed exactly as in
+// k_bhc_ie_ray (FP32 basis values, FP64 sum over j in order), one partial per group.
+__global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_cost(BhcParams P, const float* __restrict__ s,
+                                                           const float* __restrict__ Imea,
+                                                           long long count, float sscale,
+                                                           double* __restrict__ partials)
+{
+    __shared__ double sh[kBhcBlock];
+    const long long i = (long long)blockIdx.x * kBhcBlock + threadIdx.x;
+    double f = 0.0;
+    if (i < count) {
+        const float si = sscale * s[i];
+        const double im = (double)Imea[i];
+        double ir = 0.0;
+        spline_basis<1>(P, si, [&](int j, float p, float, float) {
+            ir = fma(P.Ied[j], (double)p, ir);
+        });
+        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {
+            const double err = gauss_err(ir, im);
+            f = 0.5 * err * err;
+        } else {
+            f = nll_term(ir, im);
+        }
+    }
+    const double tot = block_sum(f, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+}
+
 // out[v] = sum over p of partials[p*nvals + v]: one work-group per value, each thread
 // sums a fixed stride of partials in order, then a fixed LDS tree.
 __global__ __launch_bounds__(kBhcBlock) void k_reduce_partials(const double* __restrict__ partials,
@@ -339,6 +367,15 @@ void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const f
     const long long blocks = bhc_blocks(count);
     if (blocks <= 0) return;
     hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s, Imea,
+                       count, sscale, partials);
+}
+
+void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
+                        long long count, double* partials, hipStream_t st)
+{
+    const long long blocks = bhc_blocks(count);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_bhc_ie_cost, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s, Imea,
                        count, sscale, partials);
 }
 

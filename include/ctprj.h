@@ -197,6 +197,29 @@ int ctprj_bhc_grad_ie(ctprj_plan* plan, const float* d_alpha, const float* d_sra
                       const float* d_Imea, const double* knots, int E, const double* Ie,
                       float Ts, double* d_f, double* d_g, double* d_H, void* stream);
 
+/* I-epsilon step: minimises the objective of ctprj_bhc_grad_ie over Ie >= 0 for a fixed
+ * projection (the IeStep of bhc/beamhardenSpline.m:446-451; bhc/ActiveSet.m and
+ * bhc/LBFGSB.m there).  Two-metric projected Newton with a Levenberg-Marquardt shift:
+ * f, g, H are FP64 sums over all rays on the device (one pass per iterate, one
+ * cost-only pass per backtracking trial); the E x E damped Cholesky solve, the
+ * eps-active set and the backtracking (halving, sufficient decrease 1e-4) run on the
+ * host in FP64.  The spectrum basis matrix is a discretised Laplace transform (condition
+ * numbers near 1e18 for E = 20), so like the reference's capped IeStep a call is a fixed
+ * budget of iterations.  Ie (host, E doubles) is the start point on entry and the
+ * iterate on return.  Stops when the projected Newton decrement is at most
+ * tol*max(|f|, 1), after maxit iterations, or when the damping exceeds 1e12.  Blocks
+ * until done (one small device-to-host copy per pass). */
+typedef struct ctprj_ie_info {
+    double cost;       /* objective at the returned Ie */
+    double decrement;  /* last Newton decrement g.d */
+    int iterations;    /* Newton iterations taken */
+    int passes;        /* ray passes (full and cost-only) */
+    int converged;     /* 1 if the decrement or no-progress test stopped it */
+} ctprj_ie_info;
+int ctprj_ie_solve(ctprj_plan* plan, const float* d_alpha, const float* d_sraw,
+                   const float* d_Imea, const double* knots, int E, double* Ie, float Ts,
+                   int maxit, double tol, ctprj_ie_info* info, void* stream);
+
 /* ------------------------------------------------------------------------------
  * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
  * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference counterpart as a C call -- the reference runs

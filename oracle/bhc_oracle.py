@@ -10,7 +10,9 @@ Follows, line by line:
   * ``gauss_l_alpha`` -- bhc/gaussLAlpha.m:1-21
   * ``gauss_l_i``     -- bhc/gaussLI.m:1-12
   * knots / kappa grid -- bhc/beamhardenSpline.m:138-139,164-165,182
-plus the moment forms ``b1_iout_stable`` / ``b0_iout_stable`` (the kernels' formulas).
+plus the moment forms ``b1_iout_stable`` / ``b0_iout_stable`` (the kernels' formulas),
+the I-epsilon step ``ie_newton`` (the library's damped projected Newton, restated) and
+an independent minimiser ``ie_minimize`` (scipy L-BFGS-B with bounds).
 
 Pinned without MATLAB (the reference's BHC code is MATLAB and cannot run here) by:
 quadrature of phi_j(s) = int basis_j(kappa) exp(-kappa s) dkappa for the hat and box
@@ -322,6 +324,83 @@ def gauss_l_i(Imea, A, Ie):
     w = (1 - Err) / Ir ** 2
     H = A.T @ (A * w[:, None])
     return f, g, H
+
+
+def ie_newton(Imea, A, Ie0, noise="poisson", maxit=50, tol=1e-14, trace=None):
+    """Damped two-metric projected Newton minimisation of poissLI / gaussLI over Ie >= 0
+    (Bertsekas' projected Newton with a Levenberg-Marquardt shift; the algorithm of
+    ctprj_ie_solve, restated in FP64):
+      w = |x - max(x - g, 0)|, eps = min(1e-2 max(x), w),
+      c_j = max(|H_jj|, 1e-6 max_k |H_kk|)  (the diagonal scale; H may be indefinite:
+      gaussLI's Hessian where Err > 1),
+      active A = {x_j <= eps, g_j > 0}: d_A = g_A / c_A,
+      free F: (H_FF + lam diag(c_F)) d_F = g_F (Cholesky; lam x10 while it fails),
+      x(t) = max(x - t d, 0), t = 1, 1/2, ... until
+      f(x(t)) <= f - 1e-4 (t g_F.d_F + g_A.(x_A - x_A(t)));
+      lam /= 3 after a full step, x4 after a shortened one, x10 (same point) when no
+      step decreases f.
+    Stops when g_F.d_F + sum_A g_j min(x_j, d_j) <= tol max(|f|, 1), after maxit
+    iterations, or when lam exceeds 1e12.  `trace`, if a list, receives f per accepted
+    iterate."""
+    obj = gauss_l_i if noise == "gaussian" else poiss_l_i
+    x = np.maximum(np.asarray(Ie0, np.float64), 0.0)
+    f, g, H = obj(Imea, A, x)
+    lam = 1e-3
+    if trace is not None:
+        trace.append(f)
+    for it in range(maxit):
+        w = np.linalg.norm(x - np.maximum(x - g, 0.0))
+        eps = min(1e-2 * x.max(), w)
+        act = (x <= eps) & (g > 0)
+        F = ~act
+        hd = np.abs(np.diag(H))
+        c = np.maximum(hd, 1e-6 * max(hd.max(), 1e-300))
+        d = np.zeros_like(x)
+        d[act] = g[act] / c[act]
+        if F.any():
+            M = H[np.ix_(F, F)]
+            D = np.diag(c[F])
+            while True:
+                try:
+                    L = np.linalg.cholesky(M + lam * D)
+                    d[F] = np.linalg.solve(L.T, np.linalg.solve(L, g[F]))
+                    break
+                except np.linalg.LinAlgError:
+                    lam *= 10
+        gdF = g[F] @ d[F]
+        dec = gdF + g[act] @ np.minimum(x[act], d[act])
+        if not dec > tol * max(abs(f), 1.0):
+            return x, f, it
+        t, ok = 1.0, False
+        for _ in range(30):
+            xn = np.maximum(x - t * d, 0.0)
+            fn = obj(Imea, A, xn)[0]
+            if fn <= f - 1e-4 * (t * gdF + g[act] @ (x[act] - xn[act])):
+                ok = True
+                break
+            t *= 0.5
+        if not ok:
+            lam *= 10
+            if lam > 1e12:
+                return x, f, it
+            continue
+        lam = max(lam / 3, 1e-12) if t == 1.0 else lam * 4
+        x = xn
+        f, g, H = obj(Imea, A, x)
+        if trace is not None:
+            trace.append(f)
+    return x, f, maxit
+
+
+def ie_minimize(Imea, A, Ie0, noise="poisson"):
+    """Independent minimiser of poissLI / gaussLI over Ie >= 0: scipy's bounded
+    L-BFGS-B with tight tolerances."""
+    from scipy.optimize import minimize
+    obj = gauss_l_i if noise == "gaussian" else poiss_l_i
+    r = minimize(lambda v: obj(Imea, A, v)[:2], np.maximum(np.asarray(Ie0, np.float64), 0),
+                 jac=True, method="L-BFGS-B", bounds=[(0, None)] * A.shape[1],
+                 options={"ftol": 1e-15, "gtol": 1e-12, "maxiter": 20000, "maxcor": 30})
+    return r.x, r.fun
 
 
 def kappa_logspace(E, lo=0.03, hi=30.0):

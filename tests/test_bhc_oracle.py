@@ -214,3 +214,57 @@ def test_gauss_l_i_finite_differences():
     Ir = Amat @ Ie
     f0, g0, _ = B.gauss_l_i(Ir, Amat, Ie)
     assert f0 == 0.0 and np.abs(g0).max() == 0.0
+
+
+def _ie_problem(seed, noise, E=20):
+    """Poisson or log-normal data of a spectrum with empty bins, so the optimum has
+    active bounds.  E = 20: the realistic, severely ill-conditioned basis matrix
+    (a discretised Laplace transform); E = 6 on a coarse grid: well conditioned."""
+    rng = np.random.default_rng(seed)
+    kn = B.spline_knots(B.kappa_logspace(E, 0.03, 30.0))
+    s = rng.uniform(0.0, 3.0, 400)
+    Amat, _, _ = B.b1_iout_stable(kn, s, None)
+    truth = np.exp(-0.5 * ((np.arange(E) - E / 2) / (E / 6)) ** 2)
+    truth[:E // 5] = 0.0
+    truth[-max(E // 6, 1):] = 0.0
+    truth /= ((kn[2:] - kn[:-2]) / 2) @ truth
+    Ir = Amat @ truth
+    Imea = (rng.poisson(1e5 * Ir) / 1e5 if noise == "poisson"
+            else Ir * np.exp(0.01 * rng.standard_normal(Ir.size)))
+    return Amat, Imea, np.full(E, 1.0 / E) / (((kn[2:] - kn[:-2]) / 2).sum() / E)
+
+
+@pytest.mark.parametrize("noise", ["poisson", "gaussian"])
+def test_ie_newton_well_conditioned_reaches_the_minimiser(noise):
+    """On a well-conditioned problem the damped projected Newton restatement (the
+    library's I-epsilon step) and scipy's bounded L-BFGS-B reach the same minimum, with
+    the bounds that the data leave empty active and the KKT conditions satisfied."""
+    Amat, Imea, Ie0 = _ie_problem(11, noise, E=6)
+    obj = B.gauss_l_i if noise == "gaussian" else B.poiss_l_i
+    x, f, its = B.ie_newton(Imea, Amat, Ie0, noise, maxit=100)
+    xr, fr = B.ie_minimize(Imea, Amat, Ie0, noise)
+    print(f"{noise}: newton f={f:.15e} ({its} it), L-BFGS-B f={fr:.15e}, x={x}")
+    assert abs(f - fr) <= 1e-9 * abs(fr)
+    _, g, _ = obj(Imea, Amat, x)
+    gs = np.abs(obj(Imea, Amat, Ie0)[1]).max()
+    assert np.all(np.abs(g[x > 0]) <= 1e-7 * gs)
+    assert np.all(g[x == 0] >= -1e-7 * gs)
+    assert np.sum(x == 0) >= 1
+
+
+@pytest.mark.parametrize("noise", ["poisson", "gaussian"])
+def test_ie_newton_ill_conditioned_descends(noise):
+    """E = 20 (condition number ~1e18): every accepted iterate lowers f, the solver ends
+    within 1e-3 of L-BFGS-B's objective or below it, and a planted ascent step (the sign
+    of d flipped) is rejected by the sufficient-decrease test."""
+    Amat, Imea, Ie0 = _ie_problem(12, noise)
+    obj = B.gauss_l_i if noise == "gaussian" else B.poiss_l_i
+    tr = []
+    x, f, its = B.ie_newton(Imea, Amat, Ie0, noise, maxit=200, trace=tr)
+    assert np.all(np.diff(tr) < 0)
+    _, fr = B.ie_minimize(Imea, Amat, Ie0, noise)
+    print(f"{noise}: newton f={f:.12e} after {its} it, L-BFGS-B f={fr:.12e}")
+    assert f <= fr + 1e-3 * abs(fr)
+    f0, g0, _ = obj(Imea, Amat, Ie0)
+    up = np.maximum(Ie0 + 1e-2 * g0 / np.abs(g0).max() * Ie0.max(), 0)
+    assert obj(Imea, Amat, up)[0] > f0 - 1e-4 * (g0 @ (Ie0 - up))

@@ -269,3 +269,41 @@ def test_gaussian_gradient_finite_differences(problem):
                          want_grad=False, basis="dis").item()
         fd, gv = (fp - fm) / (2 * h), torch.dot(g.double(), v.double()).item()
         assert abs(fd - gv) <= 2e-3 * abs(gv), (fd, gv)
+
+
+@pytest.mark.parametrize("noise", ["poisson", "gaussian"])
+def test_ie_solve_device(problem, oracle_mod, noise):
+    """The I-epsilon step on the device (ctprj_ie_solve): on a well-conditioned spectrum
+    model (E = 6) it reaches the FP64 minimiser of scipy's bounded L-BFGS-B, with the
+    empty bin found; on the realistic one (E = 20) it lowers the FP64 objective."""
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import ie_solve
+    from imgrecsrc_amd.projector import Projector
+    P = problem
+    proj = Projector([GEO])
+    raw, _ = oracle_mod.forward(P["cf"], P["alpha"])
+    s = P["Ts"] * raw.ravel()
+    obj = B.gauss_l_i if noise == "gaussian" else B.poiss_l_i
+    rng = np.random.default_rng(21)
+    for Ek in (6, 20):
+        kn = B.spline_knots(B.kappa_logspace(Ek, 0.03, 30.0))
+        A = B.b1_iout_stable(kn, s, None)[0]
+        truth = np.exp(-0.5 * ((np.arange(Ek) - Ek / 2) / (Ek / 6)) ** 2)
+        truth[-1] = 0.0
+        truth /= ((kn[2:] - kn[:-2]) / 2) @ truth
+        Ir = A @ truth
+        Imea = (rng.poisson(1e5 * Ir) / 1e5 if noise == "poisson"
+                else Ir * np.exp(0.01 * rng.standard_normal(Ir.size)))
+        Ie0 = np.full(Ek, 1.0) / ((kn[2:] - kn[:-2]) / 2).sum()
+        im = torch.from_numpy(Imea.astype(np.float32)).cuda()
+        x, info = ie_solve(im, Ie0, proj, P["Ts"], kn, alpha=P["alpha_t"], noise=noise, maxit=100)
+        f_dev = obj(Imea.astype(np.float32).astype(np.float64), A, x)[0]
+        f0 = obj(Imea.astype(np.float32).astype(np.float64), A, Ie0)[0]
+        print(f"{noise} E={Ek}: f0 {f0:.6e} -> {f_dev:.9e} ({info})")
+        assert info["iterations"] >= 1 and f_dev < f0
+        if Ek == 6:
+            xr, fr = B.ie_minimize(Imea.astype(np.float32).astype(np.float64), A, Ie0, noise)
+            print(f"   L-BFGS-B f {fr:.9e}, x {np.round(xr, 5)}, device x {np.round(x, 5)}")
+            assert abs(f_dev - fr) <= 1e-5 * abs(fr)
+            assert np.allclose(x, xr, rtol=1e-3, atol=1e-4 * np.abs(xr).max())
