@@ -420,6 +420,31 @@ def main():
                           "speedup_vs_general": round(t_g / t_b, 3),
                           "slices_per_group": {str(k): groups.count(k) for k in range(4)}}
 
+    if not args.no_extras:
+        # ---- the reference-compatible boundary: host arrays through prj_run, PCIe included
+        import ctypes
+        from imgrecsrc_amd._lib import BWD_BIT, FWD_BIT, check, lib
+        L = lib()
+        check(L.prj_setup(n, W, npj, geo[3], geo[4], geo[5], geo[6]))
+        h_img = np.ascontiguousarray(img_np, dtype=np.float32)
+        h_sino = np.empty(npj * W, np.float32)
+        h_back = np.empty(n * n, np.float32)
+
+        def host_pair():
+            check(L.prj_run(h_img.ctypes.data, h_sino.ctypes.data, FWD_BIT))
+            check(L.prj_run(h_back.ctypes.data, h_sino.ctypes.data, BWD_BIT))
+
+        host_pair()
+        reps_h = 10
+        t0 = time.perf_counter()
+        for _ in range(reps_h):
+            host_pair()
+        t_h = (time.perf_counter() - t0) / reps_h
+        out["pcie_inclusive"] = {"GUPS": round(2.0 * nnz / t_h / 1e9, 1),
+                                 "ms_per_pair": round(1e3 * t_h, 3),
+                                 "path": "prj_setup + prj_run(FWD), prj_run(BWD) on pageable "
+                                         "host arrays (the MEX boundary), default shapes"}
+
     if not args.no_extras and args.pnpg_slices > 0:
         out["pnpg"] = pnpg_leg(args.pnpg_slices, args.pnpg_trials, dev, stream)
 
