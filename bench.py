@@ -367,6 +367,30 @@ def main():
                       "roofline_evals_per_s": round(roof_evals, 1),
                       "frac": round((1e3 / t_ga) / roof_evals, 4), "nnz": bnnz}
 
+        # ---- I-epsilon step (20 iterations) and the measurement simulator; both block
+        from imgrecsrc_amd.bhc import gen_beam_harden, ie_solve
+        Ie0 = normalize_ie(np.ones(20), knots)
+        ie_solve(Imea, Ie0, PB, Ts, knots, alpha=a_eval, maxit=2)
+        t0 = time.perf_counter()
+        _, info = ie_solve(Imea, Ie0, PB, Ts, knots, alpha=a_eval, maxit=20, tol=0.0)
+        t_ie = (time.perf_counter() - t0) * 1e3
+        eps_s = np.arange(8.0, 141.0)  # synthetic 140 kV tungsten-like spectrum, 1 keV bins
+        iota_s = np.maximum(140.0 - eps_s, 0.0) / eps_s * np.exp(-(12.0 / eps_s) ** 3)
+        kap_s = (0.12 * (eps_s / 60.0) ** -2.7 + 0.03)[None, :]
+        gen_beam_harden(PB, alpha, [7.874], kap_s, eps_s, iota_s)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(5):
+            sim = gen_beam_harden(PB, alpha, [7.874], kap_s, eps_s, iota_s)
+        torch.cuda.synchronize()
+        t_sim = (time.perf_counter() - t0) * 1e3 / 5
+        out["bhc"].update({
+            "ie_solve_ms": round(t_ie, 3), "ie_solve_iterations": info["iterations"],
+            "ie_solve_passes": info["passes"],
+            "gen_beam_harden_ms": round(t_sim, 3), "gen_beam_harden_Ts": sim["Ts"],
+            "gen_beam_harden_trials": int(round(np.log(sim["Ts"]) / np.log(0.8))) + 1,
+            "gen_beam_harden_energies": int(eps_s.size)})
+
         # ---- 64 ragged slices: one launch per operator vs one launch per slice
         log("batched leg")
         geoms = ragged_geoms()

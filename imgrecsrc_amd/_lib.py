@@ -93,6 +93,10 @@ SIGNATURES = [
     ("ctprj_bhc_grad_ie", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_ie_solve", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _i, ctypes.c_double,
                             ctypes.POINTER(IeInfo), _vp]),
+    ("ctprj_poly_measure", _i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _i, ctypes.c_double, _vp,
+                                _dp, _dp, _vp]),
+    ("ctprj_gen_beam_harden", _i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _i, ctypes.c_double, _vp,
+                                   _dp, _dp, _vp]),
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),

@@ -7,6 +7,10 @@ Mirrors of the reference's MATLAB contract, computed by libctprj:
 * ``poissLAlpha(Imea, Ie, alpha, proj, Ts, knots)`` -> (f, g, hw) -- bhc/poissLAlpha.m
 * ``poissLI(Imea, Ie, proj, Ts, knots, alpha=|sraw=)`` -> (f, g, H) -- bhc/poissLI.m
 * ``gaussLAlpha`` / ``gaussLI`` (same arguments) -- bhc/gaussLAlpha.m, bhc/gaussLI.m
+* ``ie_solve`` -- the I-epsilon step of bhc/beamhardenSpline.m:446-451
+* ``gen_beam_harden`` / ``poly_measure`` -- the measurement simulator bhc/genBeamHarden.m,
+  with ``mass_atten_kappa`` (its table interpolation) and ``read_spectrum``
+  (bhc/readSpectrum.m)
 
 Every likelihood call takes ``basis='b1'|'b0'|'dis'`` (bhc/beamhardenSpline.m:60
 opt.spectBasis); its knots are E+2 values for 'b1' and 'dis', E+1 for 'b0'.
@@ -228,6 +232,93 @@ def ie_solve(Imea: torch.Tensor, Ie0, proj: Projector, Ts: float, knots,
     return np.array(ih[:E], np.float64), {f: getattr(info, f) for f, _ in IeInfo._fields_}
 
 
-__all__ = ["ie_solve", "Spline", "poissLAlpha", "poissLI", "gaussLAlpha", "gaussLI", "alpha_rays",
+# ------------------------------------------------------------------ simulator ---
+
+def read_spectrum(path) -> tuple[np.ndarray, np.ndarray]:
+    """(epsilon, iota) from a two-column text spectrum file, as bhc/readSpectrum.m reads
+    the tungsten_<kV>kV_<ripple>_1Gy.data files (energy in keV, photons per bin)."""
+    data = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    if data.shape[1] < 2:
+        raise ValueError(f"{path}: expected two columns")
+    return data[:, 0].copy(), data[:, 1].copy()
+
+
+def mass_atten_kappa(energy_mev, mu_rho, epsilon) -> np.ndarray:
+    """Mass attenuation at the spectrum energies epsilon (keV) from a table (energy in MeV,
+    mass attenuation coefficient), as bhc/genBeamHarden.m:19-29 does with the NIST tables:
+    the table points inside [min eps, max eps] plus one on each side, interpolated by a
+    not-a-knot cubic spline (MATLAB interp1 'spline') in log energy / log attenuation."""
+    from scipy.interpolate import CubicSpline
+
+    e = np.asarray(energy_mev, np.float64).ravel() * 1e3
+    mu = np.asarray(mu_rho, np.float64).ravel()
+    eps = np.asarray(epsilon, np.float64).ravel()
+    if e.shape != mu.shape:
+        raise ValueError("energy and attenuation tables differ in length")
+    inside = np.nonzero((e >= eps.min()) & (e <= eps.max()))[0]
+    if inside.size == 0:
+        raise ValueError("no table energy inside the spectrum's range")
+    i1, i2 = inside[0] - 1, inside[-1] + 1
+    if i1 < 0 or i2 >= e.size:
+        raise ValueError("the table does not extend one point beyond the spectrum's range")
+    x, y = np.log(e[i1:i2 + 1]), np.log(mu[i1:i2 + 1])
+    if np.any(np.diff(x) <= 0):
+        raise ValueError("table energies must increase inside the interpolation range")
+    return np.exp(CubicSpline(x, y, bc_type="not-a-knot")(np.log(eps)))
+
+
+def _sim_inputs(K, density, kappa, epsilon, iota):
+    density = np.asarray(density, np.float64).ravel()
+    eps = np.asarray(epsilon, np.float64).ravel()
+    io = np.asarray(iota, np.float64).ravel()
+    kap = np.asarray(kappa, np.float64).reshape(K, -1)
+    if density.size != K or kap.shape[1] != eps.size or io.size != eps.size:
+        raise ValueError("density (K), kappa (K x L), epsilon (L) and iota (L) disagree")
+    return density, kap, eps, io
+
+
+def _simulate(entry, proj, src, K, density, kappa, epsilon, iota, scale):
+    density, kap, eps, io = _sim_inputs(K, density, kappa, epsilon, iota)
+    dens_p, dens_h = dbl_array(density)
+    kap_p, kap_h = dbl_array(kap.ravel())
+    eps_p, eps_h = dbl_array(eps)
+    io_p, io_h = dbl_array(io)
+    Imea = torch.empty(proj.sino_elems, dtype=torch.float32, device=src.device)
+    ts, imax = ctypes.c_double(), ctypes.c_double()
+    check(entry(proj.handle, _ptr(src), K, dens_p, kap_p, eps_p, io_p, eps.size, float(scale),
+                _ptr(Imea), ctypes.byref(ts), ctypes.byref(imax), _stream_ptr()))
+    k_max = int(np.argmax(density))  # the first of the densest materials
+    return {"Imea": Imea, "Ts": ts.value, "Imax": imax.value, "iota": io / imax.value,
+            "epsilon": eps, "kappa": kap[k_max].copy(), "density": float(density[k_max])}
+
+
+def gen_beam_harden(proj: Projector, maps, density, kappa, epsilon, iota,
+                    scale: float = 2.0 ** -12 * 1.1) -> dict:
+    """Polychromatic measurements of K materials (bhc/genBeamHarden.m): maps is a K x
+    img_elems float32 CUDA tensor (or a list of K images), density K values, kappa K x L
+    attenuations at the L energies epsilon, iota the spectrum there.  Returns the fields of
+    the reference's (CTdata, args): Imea (normalised, FP32 sinogram), Ts, iota / max I,
+    epsilon, and kappa / density of the densest material."""
+    if isinstance(maps, (list, tuple)):
+        maps = torch.stack([m.reshape(-1) for m in maps])
+    K = max(1, maps.numel() // proj.img_elems)
+    maps = _f32(maps.reshape(-1), K * proj.img_elems, "maps")
+    return _simulate(lib().ctprj_gen_beam_harden, proj, maps, K, density, kappa, epsilon,
+                     iota, scale)
+
+
+def poly_measure(proj: Projector, y, density, kappa, epsilon, iota,
+                 scale: float = 2.0 ** -12 * 1.1) -> dict:
+    """gen_beam_harden on given projections y (K x sino_elems, y_k = A x_k before the
+    density factor)."""
+    if isinstance(y, (list, tuple)):
+        y = torch.stack([t.reshape(-1) for t in y])
+    K = max(1, y.numel() // proj.sino_elems)
+    y = _f32(y.reshape(-1), K * proj.sino_elems, "y")
+    return _simulate(lib().ctprj_poly_measure, proj, y, K, density, kappa, epsilon, iota, scale)
+
+
+__all__ = ["gen_beam_harden", "poly_measure", "mass_atten_kappa", "read_spectrum", "ie_solve",
+           "Spline", "poissLAlpha", "poissLI", "gaussLAlpha", "gaussLI", "alpha_rays",
            "set_model", "kappa_logspace", "kappa_logspan", "spline_knots", "basis_at_zero",
            "normalize_ie", "BASIS", "NOISE"]

@@ -158,6 +158,7 @@ Plan::~Plan()
     free_ptr(d_sino_ws);
     free_ptr(d_red);
     free_ptr(d_red_out);
+    free_ptr(d_sim);
 }
 
 template <class T>
@@ -284,6 +285,8 @@ static int choose_group(const ctprj_geom& g, int flags, bool* fold, std::vector<
         if (grp == kGrpV4 && (P % 2 || (flags & 4))) continue;
         const bool fo = !fan && grp != kGrpZ2 && 2 * g.np <= P;
         std::vector<int> c;
+        // the symmetric adjoint addresses a fan slice's ray table with 32-bit byte offsets
+        if (fan && g.prjWidth * (double)sizeof(Ray) * (P / 2 + 2) >= 2147483648.0) break;
         if (symmetric_views(g, grp, fo, &c) >= 0.5) {
             *fold = fo;
             *canon = c;

@@ -60,6 +60,8 @@ struct Plan {
     double* d_red_out = nullptr; // reduced sums
     long long red_cap = 0;       // doubles held by d_red
     int bhc_basis = CTPRJ_BASIS_B1, bhc_noise = CTPRJ_NOISE_POISSON;  // spectrum model
+    void* d_sim = nullptr;       // measurement simulator workspace
+    size_t sim_cap = 0;          // bytes held by d_sim
 
     ~Plan();
     int build_tables();          // geometry tables for the current tune.flags

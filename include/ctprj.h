@@ -221,9 +221,39 @@ int ctprj_ie_solve(ctprj_plan* plan, const float* d_alpha, const float* d_sraw,
                    int maxit, double tol, ctprj_ie_info* info, void* stream);
 
 /* ------------------------------------------------------------------------------
+ * Polychromatic measurement simulator (bhc/genBeamHarden.m:53-79).  For K materials
+ * with projections y_k = density_k * (A x_k), ray t measures
+ *   I(t) = sum_l exp(-Ts * sum_k y_k(t) kappa[k][l]) * iota[l] * dEps[l]
+ * with dEps the widths of the energy bins [eps_1, midpoints, eps_L] (:53-54), in FP64
+ * in the reference's evaluation order.  Ts starts at 1 and is multiplied by 0.8 until
+ * min I / max I >= scale (:69-78; the reference's default scale is 1.1 * 2^-12);
+ * d_Imea (sinogram layout, FP32) is then I / max I, *Ts the chosen Ts and *Imax = max I
+ * (the reference divides iota by it).  One Ts and one max for all rays of the plan (one
+ * slice per plan reproduces the reference's per-slice call).  kappa is K x L row-major
+ * (material k's mass attenuation at the L energies, e.g. interpolated from the NIST
+ * tables as bhc/genBeamHarden.m:19-29 does), epsilon strictly increasing, L >= 2,
+ * 0 <= scale <= 1.  Allocates plan workspace on first use; blocks until Ts is found (one
+ * small device-to-host copy per trial).
+ * ------------------------------------------------------------------------------ */
+#define CTPRJ_SIM_MAX_K 8
+/* Ts search and normalisation on given per-material projections d_y (K sinograms of
+ * the plan's size, contiguous; the density factors are applied inside, in FP64). */
+int ctprj_poly_measure(ctprj_plan* plan, const float* d_y, int K, const double* density,
+                       const double* kappa, const double* epsilon, const double* iota,
+                       int L, double scale, float* d_Imea, double* Ts, double* Imax,
+                       void* stream);
+/* The same after projecting K density maps d_maps (K images of the plan's size,
+ * contiguous) with the plan's forward projector. */
+int ctprj_gen_beam_harden(ctprj_plan* plan, const float* d_maps, int K, const double* density,
+                          const double* kappa, const double* epsilon, const double* iota,
+                          int L, double scale, float* d_Imea, double* Ts, double* Imax,
+                          void* stream);
+
+/* ------------------------------------------------------------------------------
  * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
- * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference counterpart as a C call -- the reference runs
- * this loop in MATLAB (bhc/beamhardenSpline.m:340-384 -> npg/NPG.m/PNPG.m).
+ * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference
+ * counterpart as a C call -- the reference runs this loop in MATLAB
+ * (bhc/beamhardenSpline.m:340-384 -> npg/NPG.m/PNPG.m).
  * ------------------------------------------------------------------------------ */
 typedef struct ctprj_pnpg ctprj_pnpg;
 typedef struct ctprj_pnpg_info {

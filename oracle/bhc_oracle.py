@@ -11,14 +11,18 @@ Follows, line by line:
   * ``gauss_l_i``     -- bhc/gaussLI.m:1-12
   * knots / kappa grid -- bhc/beamhardenSpline.m:138-139,164-165,182
 plus the moment forms ``b1_iout_stable`` / ``b0_iout_stable`` (the kernels' formulas),
-the I-epsilon step ``ie_newton`` (the library's damped projected Newton, restated) and
-an independent minimiser ``ie_minimize`` (scipy L-BFGS-B with bounds).
+the I-epsilon step ``ie_newton`` (the library's damped projected Newton, restated),
+an independent minimiser ``ie_minimize`` (scipy L-BFGS-B with bounds), and the
+measurement simulator ``gen_beam_harden`` (bhc/genBeamHarden.m:53-79) with
+``energy_bins``.
 
 Pinned without MATLAB (the reference's BHC code is MATLAB and cannot run here) by:
 quadrature of phi_j(s) = int basis_j(kappa) exp(-kappa s) dkappa for the hat and box
 bases (scipy.integrate.quad), the closed values phi_j(0), finite differences of the
 dis basis' derivatives, central finite differences of the likelihoods, and the
-identity g . Ie = sum(Ir - Imea) of poissLI.  Exact MATLAB outputs of the Taylor
+identity g . Ie = sum(Ir - Imea) of poissLI; the simulator by its monochromatic closed
+form (Ts = the largest 0.8^j with exp(-Ts kappa rho (max y - min y)) >= scale) and
+hand-computed two-energy cases.  Exact MATLAB outputs of the Taylor
 branches are "parity unpinned" beyond those checks.
 """
 from __future__ import annotations
@@ -411,3 +415,37 @@ def spline_knots(kappa):
     k = np.asarray(kappa, np.float64)
     q = k[1] / k[0]
     return np.concatenate([[k[0] / q], k, [k[-1] * q]])
+
+
+def energy_bins(epsilon):
+    """Widths of the energy bins [eps_1, midpoints, eps_L] (bhc/genBeamHarden.m:53-54)."""
+    eps = np.asarray(epsilon, np.float64).ravel()
+    edges = np.concatenate([[eps[0]], (eps[:-1] + eps[1:]) / 2, [eps[-1]]])
+    return edges[1:] - edges[:-1]
+
+
+def gen_beam_harden(y, density, kappa, epsilon, iota, scale=2.0 ** -12 * 1.1):
+    """bhc/genBeamHarden.m:53-79: y = K FP32 projections A x_k; returns (I / max I, Ts,
+    max I) in FP64, evaluated in the reference's order (no fused multiply-adds)."""
+    deps = energy_bins(epsilon)
+    K = len(y)
+    yk = [np.asarray(y[k], np.float32).astype(np.float64).ravel() * float(density[k])
+          for k in range(K)]
+    kap = np.asarray(kappa, np.float64).reshape(K, -1)
+    io = np.asarray(iota, np.float64).ravel()
+    Ts = 1.0
+    while True:
+        Imea = np.zeros_like(yk[0])
+        for i in range(len(deps)):
+            pa = np.zeros_like(yk[0])
+            for k in range(K):
+                pa = pa + yk[k] * kap[k, i]
+            pa = pa * Ts
+            Imea = Imea + np.exp(-pa) * io[i] * deps[i]
+        temp = Imea.max()
+        Imea = Imea / temp
+        if Imea.min() < scale:
+            Ts = Ts * 0.8
+        else:
+            return Imea, Ts, temp
+
