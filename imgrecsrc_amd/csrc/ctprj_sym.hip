@@ -382,12 +382,36 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
         if (HMASK >> h & 1u) vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
 }
 
+// One fan-beam ray record through a buffer view of the slice's ray table (byte offsets:
+// voff the bin, soff the view row).
+__device__ __forceinline__ Ray ldray(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    const auto a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+    const auto b = __builtin_amdgcn_raw_buffer_load_b128(r, voff + 16, soff, 0);
+    Ray x;
+    x.cosR = __uint_as_float(a[0]);
+    x.sinR = __uint_as_float(a[1]);
+    x.dc = __uint_as_float(a[2]);
+    x.hbc = __uint_as_float(a[3]);
+    x.h = __uint_as_float(b[0]);
+    x.b1 = __uint_as_float(b[1]);
+    x.s = __uint_as_float(b[2]);
+    x.k = __uint_as_float(b[3]);
+    return x;
+}
+
 // The same gather for a canonical view off the mirror lines and a compile-time element
 // g' = E2: the reflection of every view row is fixed (elem_flip), so windows and bins are
-// indexed without selects.  Views beyond np read row 0 and their sums are discarded.
+// indexed without selects.  Sums go straight into the view's partials; the windows of
+// views beyond np are zeroed (wave-uniform branch), which adds exact zeros.  Fan beam:
+// the rays of a window's first three bins are loaded together before the image windows
+// (a pixel meets at most three bins when the detector pitch is at least the pixel's;
+// the fourth is loaded where needed), RR being the slice's ray table and rrow the
+// view's row.
 template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD>
 __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* __restrict__ rview,
                                                 __amdgpu_buffer_rsrc_t RS,
+                                                __amdgpu_buffer_rsrc_t RR, int rrow,
                                                 const int* __restrict__ tab, float cT, float sT,
                                                 int x0, int y0, float* vpart)
 {
@@ -409,50 +433,52 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
         if (t & kRowNone) none |= 1u << h;
     }
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
-    float part[8];
-#pragma unroll
-    for (int h = 0; h < 8; ++h) part[h] = 0.0f;
+    // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
+    const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
     const int two_pc = 2 * S.pC, smax = S.W - 4;
     for (int s0 = blo; s0 <= bhi; s0 += 4) {
         const int s = min(s0, smax);
         const int ob = s * 4, of = (two_pc - s - 3) * 4;  // windows [s, s+3], [2pC-s-3, 2pC-s]
+        Ray rq[3];
+        if (FAN) {
+#pragma unroll
+            for (int j = 0; j < 3; ++j) rq[j] = ldray(RR, (s + j) * (int)sizeof(Ray), rrow);
+        }
         Quad q[8];
 #pragma unroll
         for (int h = 0; h < 8; ++h)
             if (HMASK >> h & 1u) q[h] = ldb4(RS, elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
+        if (none) {  // wave-uniform
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                if ((HMASK >> h & 1u) && (none >> h & 1u)) q[h] = Quad{{0.0f, 0.0f, 0.0f, 0.0f}};
+        }
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const int b = s + j;
             if (b < s0 || b > bhi) continue;
             Ray r;
-            if (FAN) r = rview[b];
+            if (FAN) r = j < 3 ? rq[j] : ldray(RR, b * (int)sizeof(Ray), rrow);
             else {
                 r = rview[0];
                 r.dc = (float)(b - S.pC) * S.dSize;
             }
             const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
-            const float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+            float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+            if (FBP && FAN) w *= fw;
 #pragma unroll
             for (int h = 0; h < 8; ++h)
                 if (HMASK >> h & 1u)
-                    part[h] = fmaf(w, q[h].q[elem_flip(h, E2, FOLD) ? 3 - j : j], part[h]);
+                    vpart[h] = fmaf(w, q[h].q[elem_flip(h, E2, FOLD) ? 3 - j : j], vpart[h]);
         }
     }
-    if (none) {  // wave-uniform
-#pragma unroll
-        for (int h = 0; h < 8; ++h)
-            if ((HMASK >> h & 1u) && (none >> h & 1u)) part[h] = 0.0f;
-    }
-    const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
-#pragma unroll
-    for (int h = 0; h < 8; ++h)
-        if (HMASK >> h & 1u) vpart[h] += (FBP && FAN) ? part[h] * fw : part[h];
 }
 
 // All elements of a canonical view off the mirror lines, unrolled at compile time.
 template <bool FAN, bool FBP, int GRP, bool FOLD>
 __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* __restrict__ rview,
                                                 __amdgpu_buffer_rsrc_t RS,
+                                                __amdgpu_buffer_rsrc_t RR, int rrow,
                                                 const int* __restrict__ tv, float cT, float sT,
                                                 int x0, int y0, float* vpart)
 {
@@ -460,8 +486,8 @@ __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* _
     constexpr unsigned GSET = FOLD ? G.fold : G.all;
 #define CTPRJ_E(E2)                                                                        \
     if constexpr ((GSET >> E2) & 1u)                                                        \
-        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD>(S, rview, RS, tv + 8 * E2, cT, sT, x0, y0, \
-                                                   vpart);
+        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD>(S, rview, RS, RR, rrow, tv + 8 * E2, cT, \
+                                                   sT, x0, y0, vpart);
     CTPRJ_E(0) CTPRJ_E(1) CTPRJ_E(2) CTPRJ_E(3) CTPRJ_E(4) CTPRJ_E(5) CTPRJ_E(6) CTPRJ_E(7)
 #undef CTPRJ_E
 }
@@ -479,11 +505,16 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
     constexpr unsigned REF = G.all & ~G.rot;
     const __amdgpu_buffer_rsrc_t RS = slice_rsrc(sino + S.sino_off, S.np * S.W);
     const __amdgpu_buffer_rsrc_t RZ = slice_rsrc(zero_row, S.W);
+    // fan: the slice's ray table (ncanon rows of W records, below 2 GiB)
+    const __amdgpu_buffer_rsrc_t RR = slice_rsrc(
+        reinterpret_cast<const float*>(rays + S.ray_off),
+        FAN ? S.ncanon * S.W * (int)(sizeof(Ray) / sizeof(float)) : 0);
     for (int i = i0; i < S.ncanon; i += step) {
         const float4 T = views[S.view_off + i];
         const int a0 = __float_as_int(T.z);
         const Ray* rview = FAN ? rays + S.ray_off + (long long)i * S.W : rays + S.ray_off + i;
         const int* tv = symtab + S.tab_off + (long long)i * 64;
+        const int rrow = FAN ? i * S.W * (int)sizeof(Ray) : 0;
         float vpart[8];  // this view's sums; compensated once per view
 #pragma unroll
         for (int h = 0; h < 8; ++h) vpart[h] = 0.0f;
@@ -504,10 +535,11 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
         } else if (GRP != kGrpZ2 && S.fold) {
             // a half-turn parallel scan takes the half set that maps the canonical range
             // into [0, 180) and reads the rest folded
-            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2>(S, rview, RS, tv, T.x, T.y, x0, y0,
-                                                         vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2>(S, rview, RS, RR, rrow, tv, T.x, T.y,
+                                                         x0, y0, vpart);
         } else {
-            adj_sym_view_ct<FAN, FBP, GRP, false>(S, rview, RS, tv, T.x, T.y, x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, false>(S, rview, RS, RR, rrow, tv, T.x, T.y, x0, y0,
+                                                  vpart);
         }
 #pragma unroll
         for (int h = 0; h < 8; ++h)
