@@ -97,6 +97,13 @@ SIGNATURES = [
                                 _dp, _dp, _vp]),
     ("ctprj_gen_beam_harden", _i, [_vp, _vp, _i, _dp, _dp, _dp, _dp, _i, ctypes.c_double, _vp,
                                    _dp, _dp, _vp]),
+    ("ctprj_wavelet_create", _i, [_i, _i, _i, _dp, _i, _i, ctypes.POINTER(_vp)]),
+    ("ctprj_wavelet_destroy", _i, [_vp]),
+    ("ctprj_dwt", _i, [_vp, _vp, _vp, _vp]),
+    ("ctprj_idwt", _i, [_vp, _vp, _vp, _vp]),
+    ("ctprj_wavelet_l1", _i, [_vp, _vp, _dp, _vp]),
+    ("ctprj_prox_wavelet_admm", _i, [_vp, _vp, ctypes.c_double, ctypes.c_double, _i, _vp, _vp,
+                                     ctypes.POINTER(ctypes.c_int), _vp]),
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),

@@ -250,6 +250,37 @@ int ctprj_gen_beam_harden(ctprj_plan* plan, const float* d_maps, int K, const do
                           void* stream);
 
 /* ------------------------------------------------------------------------------
+ * Orthogonal wavelet transforms and the wavelet-l1 proximal step: the sparsity prior
+ * of the default BHC configuration (opt.proximal = 'wvltADMM',
+ * bhc/beamhardenSpline.m:36,204-209), whose Psi / Psit are midwt / mdwt of the Rice
+ * Wavelet Toolbox (tomography/ConfigCT.m:82-86).  A context holds `batch` images of
+ * rows x cols (both divisible by 2^L; row-major, the MATLAB row index as row), the
+ * scaling filter h (even length <= 32, e.g. daubcqf) and the level count L.  Images are
+ * FP32 device arrays; the transforms and the ADMM state are FP64.  Create allocates
+ * seven FP64 image-sized buffers.
+ * ------------------------------------------------------------------------------ */
+typedef struct ctprj_wavelet ctprj_wavelet;
+int ctprj_wavelet_create(int rows, int cols, int batch, const double* h, int lh, int L,
+                         ctprj_wavelet** out);
+int ctprj_wavelet_destroy(ctprj_wavelet* w);
+/* y = mdwt(x, h, L) (rwt/mdwt_r.c MDWT) of each image: periodised two-channel analysis,
+ * rows then columns, L levels on the top-left block; lowpass first.  x may equal y. */
+int ctprj_dwt(ctprj_wavelet* w, const float* d_x, float* d_y, void* stream);
+/* x = midwt(y, h, L) (rwt/midwt_r.c MIDWT), the inverse.  y may equal x. */
+int ctprj_idwt(ctprj_wavelet* w, const float* d_y, float* d_x, void* stream);
+/* *l1 = ||mdwt(x)||_1 over all images (FP64; synchronises `stream`). */
+int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, void* stream);
+/* x = argmin 0.5||x - a||^2 + I(x >= 0) + u ||mdwt(x)||_1 by the ADMM of npg/admm.m
+ * (scaled by ||a||, adaptive penalty rho, stopping when both the change of the split
+ * variable and the residual are at most min(relTol, 1e-3), or after maxItr + 1
+ * iterations); d_init (may be NULL: a) starts the split variable.  *iterations gets the
+ * iteration count (innerSearch).  Host-driven: one small device-to-host copy per
+ * iteration. */
+int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u, double relTol,
+                            int maxItr, const float* d_init, float* d_out, int* iterations,
+                            void* stream);
+
+/* ------------------------------------------------------------------------------
  * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
  * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference
  * counterpart as a C call -- the reference runs this loop in MATLAB

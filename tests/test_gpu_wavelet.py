@@ -1,0 +1,89 @@
+"""GPU parity of the wavelet transforms (rwt/mdwt_r.c, rwt/midwt_r.c) and the ADMM
+wavelet prox (npg/admm.m) against the FP64 restatement.  The transforms run in FP64 in
+the reference's summation order, so their FP32 outputs equal the rounded restatement bit
+for bit; the ADMM solution agrees to 1e-6 of its scale (its stopping tests compare FP64
+sums whose order differs)."""
+import numpy as np
+import pytest
+
+from oracle import wavelet_oracle as W
+
+pytestmark = pytest.mark.gpu
+
+CASES = [(2, 3, 16, 16), (4, 3, 32, 16), (6, 4, 64, 64), (8, 2, 8, 12), (20, 2, 16, 16),
+         (6, 6, 128, 128)]
+
+
+def _wv(N, L, R, C, batch=1):
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf
+    h = daubcqf(N)[0]
+    return Wavelet(R, C, h, L, batch=batch), h
+
+
+@pytest.mark.parametrize("N,L,R,C", CASES)
+def test_dwt_idwt_bitwise(N, L, R, C):
+    import torch
+    w, h = _wv(N, L, R, C)
+    x = np.random.default_rng(N + L).standard_normal((R, C)).astype(np.float32)
+    xt = torch.from_numpy(x).cuda().reshape(-1)
+    y = w.mdwt(xt).cpu().numpy().reshape(R, C)
+    yr = W.mdwt(x.astype(np.float64), h, L)
+    np.testing.assert_array_equal(y, yr.astype(np.float32))
+    z = w.midwt(torch.from_numpy(y).cuda().reshape(-1)).cpu().numpy().reshape(R, C)
+    np.testing.assert_array_equal(z, W.midwt(y.astype(np.float64), h, L).astype(np.float32))
+    np.testing.assert_allclose(z, x, atol=2e-6 * np.abs(x).max())
+    # in place
+    w.mdwt(xt, out=xt)
+    np.testing.assert_array_equal(xt.cpu().numpy().reshape(R, C), y)
+
+
+def test_batched_images_transform_independently():
+    import torch
+    w, h = _wv(4, 3, 32, 32, batch=3)
+    x = np.random.default_rng(9).standard_normal((3, 32, 32)).astype(np.float32)
+    y = w.mdwt(torch.from_numpy(x).cuda().reshape(-1)).cpu().numpy().reshape(3, 32, 32)
+    for b in range(3):
+        np.testing.assert_array_equal(y[b], W.mdwt(x[b].astype(np.float64), h, 3).astype(np.float32))
+    assert w.penalty(torch.from_numpy(x).cuda().reshape(-1)) == pytest.approx(
+        sum(W.l1_penalty(x[b].astype(np.float64), h, 3) for b in range(3)), rel=1e-12)
+
+
+@pytest.mark.parametrize("N,L,shape,u,nonneg_active",
+                         [(4, 3, (32, 32), 0.5, False), (6, 3, (64, 64), 2.0, True),
+                          (2, 4, (32, 48), 1.0, True)])
+def test_admm_matches_oracle(N, L, shape, u, nonneg_active):
+    import torch
+    w, h = _wv(N, L, *shape)
+    rng = np.random.default_rng(N * 7)
+    a = rng.standard_normal(shape)
+    if not nonneg_active:
+        a = a + 10.0
+    a = a.astype(np.float32)
+    init = np.maximum(a, 0).astype(np.float32)
+    out, itr = w.admm(torch.from_numpy(a).cuda().reshape(-1), u, relativeTol=1e-7, maxItr=3000,
+                      init=torch.from_numpy(init).cuda().reshape(-1))
+    ref, itr_ref = W.admm(a.astype(np.float64), u, h, L, relative_tol=1e-7, max_itr=3000,
+                          init=init.astype(np.float64))
+    got = out.cpu().numpy().reshape(shape)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    print(f"N={N} L={L} {shape}: iterations {itr} (oracle {itr_ref}), max rel diff {err:.2e}")
+    assert err <= 1e-6
+    assert abs(itr - itr_ref) <= max(2, itr_ref // 50)
+    assert got.min() >= 0
+    if not nonneg_active:
+        closed = W.midwt(W.soft_thresh(W.mdwt(a.astype(np.float64), h, L), u), h, L)
+        np.testing.assert_allclose(got, closed, atol=2e-5 * np.abs(a).max())
+
+
+def test_admm_zero_input_and_bad_arguments():
+    import torch
+    w, _ = _wv(2, 2, 8, 8)
+    z = torch.zeros(64, dtype=torch.float32, device="cuda")
+    out, itr = w.admm(z, 1.0)
+    assert itr == 0 and not torch.any(out)
+    from imgrecsrc_amd._lib import CtprjError
+    from imgrecsrc_amd.wavelet import Wavelet
+    with pytest.raises(CtprjError):
+        Wavelet(12, 12, [0.5, 0.5], 3)  # 12 not divisible by 8
+    with pytest.raises(CtprjError):
+        Wavelet(16, 16, [1.0, 0.5, 0.2], 2)  # odd filter length
