@@ -50,7 +50,9 @@ class Geom(ctypes.Structure):
 class PnpgInfo(ctypes.Structure):
     _fields_ = [("cost", ctypes.c_double), ("t", ctypes.c_double), ("theta", ctypes.c_double),
                 ("iterations", ctypes.c_int), ("trials", ctypes.c_int), ("rejects", ctypes.c_int),
-                ("restarts", ctypes.c_int), ("last_accepted", ctypes.c_int)]
+                ("restarts", ctypes.c_int), ("last_accepted", ctypes.c_int),
+                ("difAlpha", ctypes.c_double), ("admmTol", ctypes.c_double),
+                ("innerSearch", ctypes.c_int), ("maxInnerItr", ctypes.c_int)]
 
 
 class IeInfo(ctypes.Structure):
@@ -107,6 +109,7 @@ SIGNATURES = [
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),
+    ("ctprj_pnpg_set_wavelet", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),
     ("ctprj_pnpg_destroy", _i, [_vp]),
     ("ctprj_last_error", ctypes.c_char_p, []),

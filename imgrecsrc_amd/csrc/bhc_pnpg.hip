@@ -11,8 +11,10 @@
 // restart (PNPG.m:103-112) likewise resets theta and retries.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <new>
+#include <vector>
 
 #include "bhc_internal.h"
 #include "ctprj_internal.h"
@@ -172,6 +174,54 @@ __global__ void k_pnpg_decide(PnpgState* S, PnpgParams P)
     S->ppp = 0;
 }
 
+// wavelet prox: the ADMM input xbar - grad/t   (PNPG.m:78-79)
+__global__ void k_pnpg_gradstep(const PnpgState* __restrict__ S, const float* __restrict__ xbar,
+                                const float* __restrict__ grad, float* __restrict__ a,
+                                long long n)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float it = (float)(1.0 / S->t);
+    a[i] = fmaf(-it, grad[i], xbar[i]);
+}
+
+// wavelet prox: FP64 partials of <newX - xbar, grad>, ||newX - xbar||^2,
+// ||xbar - alpha||_1, ||alpha - newX||^2, ||newX||^2 (5 per work-group)
+__global__ __launch_bounds__(256) void k_pnpg_wv_sums(const float* __restrict__ xbar,
+                                                      const float* __restrict__ newx,
+                                                      const float* __restrict__ grad,
+                                                      const float* __restrict__ alpha, long long n,
+                                                      double* __restrict__ partials)
+{
+    __shared__ double sh[5][256];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (i < n) {
+        const double x = newx[i], xb = xbar[i], al = alpha[i];
+        const double d = x - xb, e = al - x;
+        v[0] = d * (double)grad[i];
+        v[1] = d * d;
+        v[2] = fabs(xb - al);
+        v[3] = e * e;
+        v[4] = x * x;
+    }
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) sh[k][t] = v[k];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int k = 0; k < 5; ++k) sh[k][t] += sh[k][t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < 5; ++k) partials[5 * (long long)blockIdx.x + k] = sh[k][0];
+    }
+}
+
 // on acceptance: preAlpha <- alpha, alpha <- newX   (PNPG.m:148-151)
 __global__ void k_pnpg_commit(const PnpgState* __restrict__ S, float* __restrict__ alpha,
                               float* __restrict__ pre, const float* __restrict__ newx, long long n)
@@ -200,10 +250,15 @@ struct ctprj_pnpg {
     double* d_part = nullptr;     // prox partials (2 per work-group)
     double* d_ray_part = nullptr; // NLL partials
     long long nimg = 0, nsino = 0, prox_blocks = 0, ray_blocks = 0;
+    // wavelet prox (PNPG.m properties admmTol, maxInnerItr, maxPossibleInnerItr)
+    ctprj_wavelet* wv = nullptr;
+    double u = 0.0, admmTol = 1e-2, difAlpha = 1.0;
+    int maxInnerItr = 100, maxPossibleInnerItr = 1000, innerSearch = 0;
+    double* d_wpart = nullptr;    // 5 partials per work-group
     ~ctprj_pnpg()
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
-                        (void*)d_s, (void*)d_part, (void*)d_ray_part})
+                        (void*)d_s, (void*)d_part, (void*)d_ray_part, (void*)d_wpart})
             if (p) (void)hipFree(p);
     }
 };
@@ -281,10 +336,159 @@ static int pnpg_func(ctprj_pnpg* s, const float* x, double* dst, float* grad, hi
     return e == hipSuccess ? 0 : hip_fail(e, "pnpg objective");
 }
 
+static int state_get(ctprj_pnpg* s, PnpgState* h, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(h, s->d_state, sizeof(*h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? 0 : hip_fail(e, "pnpg state read");
+}
+
+static int state_put(ctprj_pnpg* s, const PnpgState* h, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(s->d_state, h, sizeof(*h), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    return e == hipSuccess ? 0 : hip_fail(e, "pnpg state write");
+}
+
+// One trial with the wavelet prox: the same sequence as the device trial, with the
+// decisions of PNPG.m:64-160 taken on the host (the ADMM inner loop needs it anyway).
+static int pnpg_step_wavelet(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
+{
+    const PnpgParams& P = s->pp;
+    PnpgState h;
+    int rc = state_get(s, &h, st);
+    if (rc) return rc;
+    if (h.ppp == 0) {  // PNPG.m:56-63
+        h.incStep = 0;
+        if (P.adaptive && h.cumu >= h.cumuTol) {
+            h.t *= P.stepIncre;
+            h.cumu = 0;
+            h.incStep = 1;
+        }
+    }
+    h.ppp += 1;
+    h.trials += 1;
+    const double B = h.t / h.preT;
+    h.newTheta = (h.p == 0) ? 1.0 : 1.0 / P.gamma + std::sqrt(P.a + B * h.theta * h.theta);
+    h.coef = (h.theta - 1.0) / h.newTheta;
+    h.accept = 0;
+    if ((rc = state_put(s, &h, st))) return rc;
+    const unsigned gi = (unsigned)((s->nimg + 255) / 256);
+    PnpgState* S = s->d_state;
+    hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg);
+    if ((rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st))) return rc;
+    hipLaunchKernelGGL(k_pnpg_gradstep, dim3(gi), dim3(256), 0, st, S, s->d_xbar, s->d_grad,
+                       s->d_newx, s->nimg);
+    // newX = admm(xbar - grad/t, u/t, admmTol*difAlpha, maxInnerItr, alpha)  (PNPG.m:78-82)
+    if ((rc = ctprj_prox_wavelet_admm(s->wv, s->d_newx, s->u / h.t, s->admmTol * s->difAlpha,
+                                      s->maxInnerItr, d_alpha, s->d_newx, &s->innerSearch, st)))
+        return rc;
+    if ((rc = pnpg_func(s, s->d_newx, &S->f_new, nullptr, st))) return rc;
+    hipLaunchKernelGGL(k_pnpg_wv_sums, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, s->d_xbar,
+                       s->d_newx, s->d_grad, d_alpha, s->nimg, s->d_wpart);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "pnpg wavelet trial");
+    std::vector<double> part((size_t)5 * s->prox_blocks);
+    e = hipMemcpyAsync(part.data(), s->d_wpart, sizeof(double) * part.size(),
+                       hipMemcpyDeviceToHost, st);
+    if (e != hipSuccess) return hip_fail(e, "pnpg wavelet sums");
+    PnpgState r;
+    if ((rc = state_get(s, &r, st))) return rc;  // f_old, f_new (synchronises)
+    double sum[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (long long b = 0; b < s->prox_blocks; ++b)
+        for (int k = 0; k < 5; ++k) sum[k] += part[5 * b + k];
+    h.f_old = r.f_old;
+    h.f_new = r.f_new;
+    h.dg = sum[0];
+    h.dd = sum[1];
+    // majorization test of the smooth part (utils/Utils.m:470-480)
+    const bool holds = (h.f_new - h.f_old) <= h.dg + 0.5 * h.t * h.dd;
+    bool goodMM = true;
+    if (!holds) {
+        if (h.ppp <= 20 && h.t > 0) {
+            h.t = h.t / P.stepShrnk;
+            if (h.incStep) {
+                if (P.incCumuTol) h.cumuTol += 4;
+                h.incStep = 0;
+            }
+            h.rejects += 1;
+            return state_put(s, &h, st);
+        }
+        goodMM = false;
+    }
+    double pen_new, pen_bar;
+    if ((rc = ctprj_wavelet_l1(s->wv, s->d_newx, &pen_new, st)) ||
+        (rc = ctprj_wavelet_l1(s->wv, s->d_xbar, &pen_bar, st)))
+        return rc;
+    double newObj = h.f_new + s->u * pen_new;
+    const double objBar = h.f_old + s->u * pen_bar;
+    bool keep = false;  // "force it to converge": newX = alpha (PNPG.m:141-144)
+    if (newObj - h.cost > 0) {
+        if (goodMM && sum[2] != 0.0 && P.restart >= 0) {  // PNPG.m:103-112
+            h.theta = 1.0;
+            h.restarts += 1;
+            h.ppp = 0;
+            return state_put(s, &h, st);
+        } else if (!goodMM || objBar < newObj) {
+            if (!goodMM) h.theta = 1.0;
+            if (s->innerSearch < s->maxInnerItr && s->admmTol > 1e-6) {
+                s->admmTol /= 10;
+                h.ppp = 0;
+                return state_put(s, &h, st);
+            } else if (s->innerSearch >= s->maxInnerItr &&
+                       s->maxInnerItr < s->maxPossibleInnerItr) {
+                s->maxInnerItr *= 10;
+                h.ppp = 0;
+                return state_put(s, &h, st);
+            }
+            newObj = h.cost;
+            keep = true;
+        }
+    }
+    // accept (PNPG.m:147-160)
+    h.theta = h.newTheta;
+    h.cost = newObj;
+    s->difAlpha = keep ? 0.0 : std::sqrt(sum[3]) / std::max(std::sqrt(sum[4]), 1e-10);
+    h.preT = h.t;
+    h.cumu = (h.ppp == 1 && P.adaptive) ? h.cumu + 1 : 0;
+    h.p += 1;
+    h.ppp = 0;
+    h.accept = 1;
+    if (keep) {
+        e = hipMemcpyAsync(s->d_newx, d_alpha, sizeof(float) * (size_t)s->nimg,
+                           hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "pnpg keep");
+    }
+    if ((rc = state_put(s, &h, st))) return rc;
+    hipLaunchKernelGGL(k_pnpg_commit, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_newx, s->nimg);
+    e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "pnpg wavelet commit");
+}
+
+extern "C" int ctprj_pnpg_set_wavelet(ctprj_pnpg* s, ctprj_wavelet* w, double u)
+{
+    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_wavelet: NULL solver");
+    if (w && wavelet_elems(w) != s->nimg)
+        return fail(CTPRJ_EINVAL, "wavelet context holds %lld pixels, the plan %lld",
+                    wavelet_elems(w), s->nimg);
+    if (!std::isfinite(u) || u < 0) return fail(CTPRJ_EINVAL, "u must be finite and >= 0");
+    PnpgState h;
+    int rc = state_get(s, &h, nullptr);
+    if (rc) return rc;
+    if (h.trials) return fail(CTPRJ_ESTATE, "set_wavelet after the first trial");
+    if (w && !s->d_wpart &&
+        (rc = pnpg_alloc((void**)&s->d_wpart, sizeof(double) * 5 * s->prox_blocks, "pnpg sums")))
+        return rc;
+    s->wv = w;
+    s->u = w ? u : 0.0;
+    return 0;
+}
+
 extern "C" int ctprj_pnpg_step(ctprj_pnpg* s, float* d_alpha, void* stream)
 {
     if (!s || !d_alpha) return fail(CTPRJ_EINVAL, "ctprj_pnpg_step: NULL argument");
     hipStream_t st = (hipStream_t)stream;
+    if (s->wv) return pnpg_step_wavelet(s, d_alpha, st);
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
     hipLaunchKernelGGL(k_pnpg_begin, dim3(1), dim3(64), 0, st, S, s->pp);
@@ -317,6 +521,10 @@ extern "C" int ctprj_pnpg_stats(ctprj_pnpg* s, ctprj_pnpg_info* info, void* stre
     info->rejects = st.rejects;
     info->restarts = st.restarts;
     info->last_accepted = st.accept;
+    info->difAlpha = s->difAlpha;
+    info->admmTol = s->admmTol;
+    info->innerSearch = s->innerSearch;
+    info->maxInnerItr = s->maxInnerItr;
     return 0;
 }
 

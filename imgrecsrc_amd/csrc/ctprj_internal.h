@@ -100,4 +100,7 @@ void launch_adjoint_sym(bool fbp, int grp, int seg, int nitems, const SliceDesc*
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
                         hipStream_t st);
 
+// Images held by a wavelet context (rows * cols * batch).
+long long wavelet_elems(const ctprj_wavelet* w);
+
 }  // namespace ctprj

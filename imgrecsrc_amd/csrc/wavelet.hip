@@ -362,6 +362,8 @@ extern "C" int ctprj_wavelet_create(int rows, int cols, int batch, const double*
     return 0;
 }
 
+long long ctprj::wavelet_elems(const ctprj_wavelet* w) { return w ? w->n : 0; }
+
 extern "C" int ctprj_wavelet_destroy(ctprj_wavelet* w)
 {
     delete w;

@@ -4,7 +4,9 @@ b1 / b0 / dis spectrum basis.
 
 ``PNPG.step(alpha)`` enqueues one line-search trial with no host synchronisation, so a
 run of trials can be captured once in a HIP graph (``torch.cuda.CUDAGraph``) and
-replayed; ``stats()`` reads the device scalars.
+replayed; ``stats()`` reads the device scalars.  ``PNPG.set_wavelet(w, u)`` switches to
+the reference's default prox ('wvltADMM': non-negativity plus u ||W alpha||_1 solved by
+npg/admm.m), whose trials synchronise.
 """
 from __future__ import annotations
 
@@ -52,6 +54,14 @@ class PNPG:
         check(lib().ctprj_pnpg_create(proj.handle, _ptr(self.Imea), kp, E, ip, self.Ts, _ptr(a0),
                                       ctypes.c_double(self.t0), ctypes.byref(h)))
         self._h = h
+
+    def set_wavelet(self, wavelet, u: float):
+        """Wavelet l1 prox u ||mdwt(alpha)||_1 (opt.proximal = 'wvltADMM'); ``wavelet`` is an
+        imgrecsrc_amd.wavelet.Wavelet over the plan's images (None: non-negativity only).
+        Steps then synchronise and cannot be graph-captured."""
+        self._wavelet = wavelet  # keeps the context alive as long as the solver
+        check(lib().ctprj_pnpg_set_wavelet(self._h, wavelet.handle if wavelet is not None
+                                           else None, ctypes.c_double(float(u))))
 
     def step(self, alpha: torch.Tensor, stream=None):
         check(lib().ctprj_pnpg_step(self._h, _ptr(alpha), _stream_ptr(stream)))

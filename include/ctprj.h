@@ -296,6 +296,10 @@ typedef struct ctprj_pnpg_info {
     int rejects;       /* trials rejected by the majorization test */
     int restarts;      /* momentum restarts */
     int last_accepted; /* 1 if the last trial was accepted */
+    double difAlpha;   /* wavelet prox: relative change of the last accepted step */
+    double admmTol;    /* wavelet prox: ADMM tolerance factor (npg/PNPG.m admmTol) */
+    int innerSearch;   /* wavelet prox: ADMM iterations of the last trial */
+    int maxInnerItr;   /* wavelet prox: ADMM iteration cap */
 } ctprj_pnpg_info;
 
 /* Solver state over `plan` (all slices share one objective): Imea (device, packed like
@@ -310,6 +314,17 @@ int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const double* knots
 int ctprj_pnpg_step(ctprj_pnpg* solver, float* d_alpha, void* stream);
 /* Copies the device scalars to `info` (synchronises `stream`). */
 int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
+/* Switches the prox to the wavelet l1 penalty u ||mdwt(alpha)||_1 with non-negativity
+ * (opt.proximal = 'wvltADMM', bhc/beamhardenSpline.m:204-209), solved per trial by
+ * ctprj_prox_wavelet_admm with tolerance admmTol * difAlpha and the reference's
+ * admmTol / maxInnerItr schedule (npg/PNPG.m:113-146); the objective then includes
+ * u ||W alpha||_1.  `w` must hold exactly the plan's images (one batch image per slice
+ * of equal size) and outlive the solver; w = NULL returns to the non-negativity prox.
+ * Only before the first trial.  In this mode ctprj_pnpg_step synchronises `stream`
+ * (the ADMM stopping tests run on the host) and cannot be captured in a graph.  On a
+ * failed line search (PNPG.m:94-103) theta restarts at 1; the reference's reset()
+ * additionally re-estimates t from the Hessian, which is not done here. */
+int ctprj_pnpg_set_wavelet(ctprj_pnpg* solver, ctprj_wavelet* w, double u);
 int ctprj_pnpg_destroy(ctprj_pnpg* solver);
 
 /* ------------------------------------------------------------------------------

@@ -112,3 +112,63 @@ def test_pnpg_graph_replay_matches_eager(problem):
     assert s_graph.stats()["trials"] == K * R
     assert torch.equal(a_graph, a_eager)
     assert s_graph.stats()["cost"] == s_eager.stats()["cost"]
+
+
+def test_pnpg_wavelet_prox_matches_fp64_restatement(oracle_mod):
+    """The default BHC prox ('wvltADMM'): trials against the FP64 restatement with the
+    wavelet ADMM of npg/admm.m; decisions identical, objective (NLL + u ||W alpha||_1) to
+    1e-5, iterate to 1e-4 of its scale."""
+    import torch
+    from oracle import bhc_oracle as B
+    from oracle.pnpg_oracle import PnpgWaveletOracle
+    from imgrecsrc_amd.phantom import ellipse_params, rasterise
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf
+    geo = (64, 91, 72, 72, 1.0, 1.0, 250.0)
+    n = geo[0]
+    knots = B.spline_knots(B.kappa_logspace(E, 0.05, 20.0))
+    Ie = np.exp(-0.5 * ((np.arange(E) - 3.0) / 1.5) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    cf = oracle_mod.conf(*geo)
+    truth = rasterise(ellipse_params(81, n), n)
+    raw = oracle_mod.forward(cf, truth)[0].ravel()
+    Ts = 2.0 / raw.max()
+    BL, _, _ = B.b1_iout_stable(knots, Ts * raw, Ie)
+    counts = np.random.default_rng(82).poisson(1e5 * BL)
+    Imea = counts / counts.max()
+
+    def func(x, grad=True):
+        s = oracle_mod.forward(cf, x.reshape(n, n))[0].ravel()
+        BLx, sBL, _ = B.b1_iout_stable(knots, Ts * s, Ie)
+        f = B._nll(BLx, Imea)
+        if not grad:
+            return f, None
+        w = Ts * (1 - Imea / BLx) * sBL
+        return f, oracle_mod.adjoint(cf, w.reshape(geo[2], geo[1]))[0].ravel()
+
+    h = daubcqf(4)[0]
+    L = 3
+    proj = Projector([geo])
+    alpha0 = np.full(n * n, 0.05)
+    alpha = torch.from_numpy(alpha0.astype(np.float32)).cuda()
+    solver = PNPG(proj, torch.from_numpy(Imea.astype(np.float32)).cuda(), knots, Ie, Ts, alpha)
+    u = 2e-3 * solver.t0
+    wv = Wavelet(n, n, h, L)
+    solver.set_wavelet(wv, u)
+    ref = PnpgWaveletOracle(func, alpha0, solver.t0, u, h, L, (n, n))
+    log = []
+    for k in range(10):
+        solver.step(alpha)
+        st = solver.stats()
+        ref.trial()
+        log.append((ref.log[-1][0], st["last_accepted"], st["cost"], ref.cost, st["innerSearch"],
+                    ref.innerSearch))
+        assert (st["last_accepted"] == 1) == (ref.log[-1][0] == "accept"), log
+        if np.isfinite(ref.cost):
+            assert abs(st["cost"] - ref.cost) <= 1e-5 * abs(ref.cost), log
+    print("\n".join(map(str, log)))
+    assert st["iterations"] == ref.p and st["iterations"] >= 5
+    got = alpha.cpu().numpy().astype(np.float64)
+    assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()
+    assert 0 < st["innerSearch"] <= st["maxInnerItr"]
