@@ -4,7 +4,9 @@ npg/PNPG.m:50-167 with forcePositive and u = 0 (prox = projection on alpha >= 0)
 majorization test of utils/Utils.m:470-480, stepShrnk 0.5, stepIncre 0.9, gamma 2,
 a 1/4, adaptive steps and restarts as in the reference class defaults; one call of
 ``trial`` is one pass of the line-search loop (a failed test shrinks the step and keeps
-the iterate; a restart resets theta).  ``func(x) -> (f, grad)``.
+the iterate; a restart resets theta).  ``func(x) -> (f, grad)``.  ``PnpgWaveletOracle``
+adds the 'wvltADMM' prox (npg/admm.m via oracle/wavelet_oracle.py) with the admmTol /
+maxInnerItr schedule of PNPG.m:113-146.
 """
 from __future__ import annotations
 
