@@ -106,10 +106,16 @@ SIGNATURES = [
     ("ctprj_wavelet_l1", _i, [_vp, _vp, _dp, _vp]),
     ("ctprj_prox_wavelet_admm", _i, [_vp, _vp, ctypes.c_double, ctypes.c_double, _i, _vp, _vp,
                                      ctypes.POINTER(ctypes.c_int), _vp]),
+    ("ctprj_tv_create", _i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
+    ("ctprj_tv_destroy", _i, [_vp]),
+    ("ctprj_tv_value", _i, [_vp, _vp, _dp, _vp]),
+    ("ctprj_tv_denoise", _i, [_vp, _vp, ctypes.c_double, ctypes.c_double, _i, _vp,
+                              ctypes.POINTER(ctypes.c_int), _vp]),
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),
     ("ctprj_pnpg_set_wavelet", _i, [_vp, _vp, ctypes.c_double]),
+    ("ctprj_pnpg_set_tv", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),
     ("ctprj_pnpg_destroy", _i, [_vp]),
     ("ctprj_last_error", ctypes.c_char_p, []),

@@ -19,7 +19,7 @@ LIB = os.path.join(LIBDIR, "libctprj.so")
 ROOT = os.path.dirname(PKG)
 
 SOURCES = ["ctprj_kernels.hip", "ctprj_sym.hip", "bhc_kernels.hip", "ctprj_api.hip", "bhc_api.hip",
-           "bhc_pnpg.hip", "bhc_sim.hip", "wavelet.hip"]
+           "bhc_pnpg.hip", "bhc_sim.hip", "wavelet.hip", "tv.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

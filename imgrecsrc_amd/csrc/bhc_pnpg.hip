@@ -251,7 +251,8 @@ struct ctprj_pnpg {
     double* d_ray_part = nullptr; // NLL partials
     long long nimg = 0, nsino = 0, prox_blocks = 0, ray_blocks = 0;
     // wavelet prox (PNPG.m properties admmTol, maxInnerItr, maxPossibleInnerItr)
-    ctprj_wavelet* wv = nullptr;
+    ctprj_wavelet* wv = nullptr;  // at most one of wv / tv is set
+    ctprj_tv* tv = nullptr;
     double u = 0.0, admmTol = 1e-2, difAlpha = 1.0;
     int maxInnerItr = 100, maxPossibleInnerItr = 1000, innerSearch = 0;
     double* d_wpart = nullptr;    // 5 partials per work-group
@@ -350,9 +351,24 @@ static int state_put(ctprj_pnpg* s, const PnpgState* h, hipStream_t st)
     return e == hipSuccess ? 0 : hip_fail(e, "pnpg state write");
 }
 
-// One trial with the wavelet prox: the same sequence as the device trial, with the
-// decisions of PNPG.m:64-160 taken on the host (the ADMM inner loop needs it anyway).
-static int pnpg_step_wavelet(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
+// The iterative prox of the solver: wavelet ADMM (npg/admm.m) or TV FGP
+// (utils/TV.m:56-80), and its penalty ||W x||_1 or TV(x).
+static int prox_solve(ctprj_pnpg* s, const float* a, double u, double tol, const float* init,
+                      float* out, int* iters, hipStream_t st)
+{
+    if (s->wv) return ctprj_prox_wavelet_admm(s->wv, a, u, tol, s->maxInnerItr, init, out, iters, st);
+    return ctprj_tv_denoise(s->tv, a, u, tol, s->maxInnerItr, out, iters, st);
+}
+
+static int prox_penalty(ctprj_pnpg* s, const float* x, double* v, hipStream_t st)
+{
+    if (s->wv) return ctprj_wavelet_l1(s->wv, x, v, st);
+    return ctprj_tv_value(s->tv, x, v, st);
+}
+
+// One trial with an iterative prox: the same sequence as the device trial, with the
+// decisions of PNPG.m:64-160 taken on the host (the prox's inner loop needs it anyway).
+static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
 {
     const PnpgParams& P = s->pp;
     PnpgState h;
@@ -379,9 +395,9 @@ static int pnpg_step_wavelet(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     if ((rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_gradstep, dim3(gi), dim3(256), 0, st, S, s->d_xbar, s->d_grad,
                        s->d_newx, s->nimg);
-    // newX = admm(xbar - grad/t, u/t, admmTol*difAlpha, maxInnerItr, alpha)  (PNPG.m:78-82)
-    if ((rc = ctprj_prox_wavelet_admm(s->wv, s->d_newx, s->u / h.t, s->admmTol * s->difAlpha,
-                                      s->maxInnerItr, d_alpha, s->d_newx, &s->innerSearch, st)))
+    // newX = prox(xbar - grad/t, u/t, admmTol*difAlpha, maxInnerItr, alpha)  (PNPG.m:78-82)
+    if ((rc = prox_solve(s, s->d_newx, s->u / h.t, s->admmTol * s->difAlpha, d_alpha,
+                         s->d_newx, &s->innerSearch, st)))
         return rc;
     if ((rc = pnpg_func(s, s->d_newx, &S->f_new, nullptr, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_wv_sums, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, s->d_xbar,
@@ -417,8 +433,8 @@ static int pnpg_step_wavelet(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
         goodMM = false;
     }
     double pen_new, pen_bar;
-    if ((rc = ctprj_wavelet_l1(s->wv, s->d_newx, &pen_new, st)) ||
-        (rc = ctprj_wavelet_l1(s->wv, s->d_xbar, &pen_bar, st)))
+    if ((rc = prox_penalty(s, s->d_newx, &pen_new, st)) ||
+        (rc = prox_penalty(s, s->d_xbar, &pen_bar, st)))
         return rc;
     double newObj = h.f_new + s->u * pen_new;
     const double objBar = h.f_old + s->u * pen_bar;
@@ -465,30 +481,42 @@ static int pnpg_step_wavelet(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     return e == hipSuccess ? 0 : hip_fail(e, "pnpg wavelet commit");
 }
 
-extern "C" int ctprj_pnpg_set_wavelet(ctprj_pnpg* s, ctprj_wavelet* w, double u)
+static int set_prox(ctprj_pnpg* s, ctprj_wavelet* w, ctprj_tv* tv, double u)
 {
-    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_wavelet: NULL solver");
-    if (w && wavelet_elems(w) != s->nimg)
-        return fail(CTPRJ_EINVAL, "wavelet context holds %lld pixels, the plan %lld",
-                    wavelet_elems(w), s->nimg);
+    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_wavelet/tv: NULL solver");
+    const long long held = w ? wavelet_elems(w) : (tv ? tv_elems(tv) : s->nimg);
+    if (held != s->nimg)
+        return fail(CTPRJ_EINVAL, "prox context holds %lld pixels, the plan %lld", held, s->nimg);
     if (!std::isfinite(u) || u < 0) return fail(CTPRJ_EINVAL, "u must be finite and >= 0");
+    if (tv && !(u > 0)) return fail(CTPRJ_EINVAL, "the TV prox needs u > 0");
     PnpgState h;
     int rc = state_get(s, &h, nullptr);
     if (rc) return rc;
-    if (h.trials) return fail(CTPRJ_ESTATE, "set_wavelet after the first trial");
-    if (w && !s->d_wpart &&
+    if (h.trials) return fail(CTPRJ_ESTATE, "prox change after the first trial");
+    if ((w || tv) && !s->d_wpart &&
         (rc = pnpg_alloc((void**)&s->d_wpart, sizeof(double) * 5 * s->prox_blocks, "pnpg sums")))
         return rc;
     s->wv = w;
-    s->u = w ? u : 0.0;
+    s->tv = tv;
+    s->u = (w || tv) ? u : 0.0;
     return 0;
+}
+
+extern "C" int ctprj_pnpg_set_wavelet(ctprj_pnpg* s, ctprj_wavelet* w, double u)
+{
+    return set_prox(s, w, nullptr, u);
+}
+
+extern "C" int ctprj_pnpg_set_tv(ctprj_pnpg* s, ctprj_tv* tv, double u)
+{
+    return set_prox(s, nullptr, tv, u);
 }
 
 extern "C" int ctprj_pnpg_step(ctprj_pnpg* s, float* d_alpha, void* stream)
 {
     if (!s || !d_alpha) return fail(CTPRJ_EINVAL, "ctprj_pnpg_step: NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    if (s->wv) return pnpg_step_wavelet(s, d_alpha, st);
+    if (s->wv || s->tv) return pnpg_step_iterative(s, d_alpha, st);
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
     hipLaunchKernelGGL(k_pnpg_begin, dim3(1), dim3(64), 0, st, S, s->pp);

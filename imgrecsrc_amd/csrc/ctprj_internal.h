@@ -102,5 +102,7 @@ void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* 
 
 // Images held by a wavelet context (rows * cols * batch).
 long long wavelet_elems(const ctprj_wavelet* w);
+// Images held by a TV context (rows * cols * batch).
+long long tv_elems(const ctprj_tv* w);
 
 }  // namespace ctprj

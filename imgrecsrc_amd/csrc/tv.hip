@@ -1,0 +1,264 @@
+// Total-variation proximal step on gfx950: argmin 0.5||D - X||^2 + lambda TV(D) over
+// D >= 0 by the fast gradient projection on the dual (utils/denoise_bound_mod.m, as
+// utils/TV.m:56-80 calls it), isotropic or l1 TV (utils/tlv.m), and the TV value.
+//
+// One iteration is two launches over the pixels: the primal step (C = X - lambda
+// Lforward(R), D = max(C, 0), with the FP64 partial sums of ||D - Dold||^2, ||D||^2,
+// ||C - D||^2 and ||C||^2) and the dual step (P = proj(R + Ltrans(D)/(8 lambda)), then
+// the momentum R = P + (tk - 1)(P - Pold)/tkp1).  The dual fields P1 (vertical
+// differences, rows 0..m-2) and P2 (horizontal, columns 0..n-2) are stored padded to
+// m x n.  The host reads the four sums once per iteration for the stopping test and the
+// momentum restart.  All arithmetic is FP64 in the reference's order; images are FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <new>
+#include <vector>
+
+#include "ctprj_internal.h"
+
+namespace ctprj {
+
+constexpr int kTvBlock = 256;
+
+__device__ __forceinline__ void tv_block_sum4(const double v[4], double* part)
+{
+    __shared__ double sh[4][kTvBlock];
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[k][t] = v[k];
+    __syncthreads();
+    for (int off = kTvBlock / 2; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + off];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) part[4 * (long long)blockIdx.x + k] = sh[k][0];
+    }
+}
+
+// primal step (utils/denoise_bound_mod.m:118-119 with utils/Lforward.m)
+__global__ __launch_bounds__(kTvBlock) void k_tv_primal(const double* __restrict__ X,
+                                                        double* __restrict__ D,
+                                                        const double* __restrict__ R1,
+                                                        const double* __restrict__ R2, int m,
+                                                        int n, long long total, double lam,
+                                                        double* __restrict__ part)
+{
+    const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < total) {
+        const int j = (int)(t % n);
+        const int i = (int)((t / n) % m);
+        const double x = (i == 0) ? R1[t] : (i == m - 1 ? -R1[t - n] : R1[t] - R1[t - n]);
+        const double y = (j == 0) ? R2[t] : (j == n - 1 ? -R2[t - 1] : R2[t] - R2[t - 1]);
+        const double c = __dsub_rn(X[t], __dmul_rn(lam, x + y));
+        const double d = fmax(c, 0.0);
+        const double e = d - D[t], f = c - d;
+        D[t] = d;
+        v[0] = e * e;
+        v[1] = d * d;
+        v[2] = f * f;
+        v[3] = c * c;
+    }
+    tv_block_sum4(v, part);
+}
+
+// dual step (utils/denoise_bound_mod.m:120-144 with utils/Ltrans.m)
+__global__ __launch_bounds__(kTvBlock) void k_tv_dual(const double* __restrict__ D,
+                                                      double* __restrict__ P1,
+                                                      double* __restrict__ P2,
+                                                      double* __restrict__ R1,
+                                                      double* __restrict__ R2, int m, int n,
+                                                      long long total, double lam, int iso,
+                                                      double tk, double tkp1)
+{
+    const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    if (t >= total) return;
+    const int j = (int)(t % n);
+    const int i = (int)((t / n) % m);
+    const bool has1 = i < m - 1, has2 = j < n - 1;
+    const double d = D[t];
+    const double l8 = 8.0 * lam;
+    double p1 = has1 ? R1[t] + (d - D[t + n]) / l8 : 0.0;
+    double p2 = has2 ? R2[t] + (d - D[t + 1]) / l8 : 0.0;
+    if (iso) {
+        const double a = sqrt(fmax(__dadd_rn(__dmul_rn(p1, p1), __dmul_rn(p2, p2)), 1.0));
+        p1 = p1 / a;
+        p2 = p2 / a;
+    } else {
+        p1 = fmax(fmin(p1, 1.0), -1.0);
+        p2 = fmax(fmin(p2, 1.0), -1.0);
+    }
+    if (has1) {
+        const double o = P1[t];
+        P1[t] = p1;
+        R1[t] = p1 + (tk - 1.0) * (p1 - o) / tkp1;
+    }
+    if (has2) {
+        const double o = P2[t];
+        P2[t] = p2;
+        R2[t] = p2 + (tk - 1.0) * (p2 - o) / tkp1;
+    }
+}
+
+// TV of FP32 images (utils/tlv.m): iso sum sqrt(P1^2 + P2^2), l1 sum |P1| + |P2|
+__global__ __launch_bounds__(kTvBlock) void k_tv_value(const float* __restrict__ X, int m, int n,
+                                                       long long total, int iso,
+                                                       double* __restrict__ part)
+{
+    const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < total) {
+        const int j = (int)(t % n);
+        const int i = (int)((t / n) % m);
+        const double x = X[t];
+        const double p1 = i < m - 1 ? x - (double)X[t + n] : 0.0;
+        const double p2 = j < n - 1 ? x - (double)X[t + 1] : 0.0;
+        v[0] = iso ? sqrt(p1 * p1 + p2 * p2) : fabs(p1) + fabs(p2);
+    }
+    tv_block_sum4(v, part);
+}
+
+__global__ void k_tv_in(const float* __restrict__ in, double* __restrict__ out, long long n)
+{
+    const long long i = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    if (i < n) out[i] = (double)in[i];
+}
+
+__global__ void k_tv_out(const double* __restrict__ in, float* __restrict__ out, long long n)
+{
+    const long long i = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    if (i < n) out[i] = (float)in[i];
+}
+
+}  // namespace ctprj
+
+using namespace ctprj;
+
+struct ctprj_tv {
+    int rows = 0, cols = 0, batch = 0, iso = 1;
+    long long n = 0, blocks = 0;
+    double *d_X = nullptr, *d_D = nullptr, *d_P1 = nullptr, *d_P2 = nullptr, *d_R1 = nullptr,
+           *d_R2 = nullptr, *d_part = nullptr;
+    std::vector<double> part;
+    ~ctprj_tv()
+    {
+        for (void* p : {(void*)d_X, (void*)d_D, (void*)d_P1, (void*)d_P2, (void*)d_R1, (void*)d_R2,
+                        (void*)d_part})
+            if (p) (void)hipFree(p);
+    }
+};
+
+namespace {
+
+int tv_sums(ctprj_tv* w, hipStream_t st, double s[4])
+{
+    hipError_t e = hipMemcpyAsync(w->part.data(), w->d_part, sizeof(double) * 4 * w->blocks,
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "tv reduction");
+    for (int k = 0; k < 4; ++k) s[k] = 0.0;
+    for (long long b = 0; b < w->blocks; ++b)
+        for (int k = 0; k < 4; ++k) s[k] += w->part[4 * b + k];
+    return 0;
+}
+
+}  // namespace
+
+long long ctprj::tv_elems(const ctprj_tv* w) { return w ? w->n : 0; }
+
+extern "C" int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv** out)
+{
+    if (!out) return fail(CTPRJ_EINVAL, "tv_create: NULL argument");
+    *out = nullptr;
+    if (rows < 2 || cols < 2 || batch < 1)
+        return fail(CTPRJ_EINVAL, "tv_create: bad sizes (%d x %d x %d)", rows, cols, batch);
+    const long long n = (long long)rows * cols * batch;
+    if (n >= (1LL << 31)) return fail(CTPRJ_EINVAL, "tv_create: %lld elements is too many", n);
+    ctprj_tv* w = new (std::nothrow) ctprj_tv();
+    if (!w) return fail(CTPRJ_ENOMEM, "host allocation");
+    w->rows = rows;
+    w->cols = cols;
+    w->batch = batch;
+    w->iso = iso ? 1 : 0;
+    w->n = n;
+    w->blocks = (n + kTvBlock - 1) / kTvBlock;
+    w->part.assign((size_t)4 * w->blocks, 0.0);
+    hipError_t e = hipSuccess;
+    for (double** p : {&w->d_X, &w->d_D, &w->d_P1, &w->d_P2, &w->d_R1, &w->d_R2})
+        if (e == hipSuccess) e = hipMalloc((void**)p, sizeof(double) * (size_t)n);
+    if (e == hipSuccess) e = hipMalloc((void**)&w->d_part, sizeof(double) * 4 * (size_t)w->blocks);
+    if (e != hipSuccess) {
+        delete w;
+        return fail(CTPRJ_ENOMEM, "tv workspace: %s", hipGetErrorString(e));
+    }
+    *out = w;
+    return 0;
+}
+
+extern "C" int ctprj_tv_destroy(ctprj_tv* w)
+{
+    delete w;
+    return 0;
+}
+
+extern "C" int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* stream)
+{
+    if (!w || !d_x || !tv) return fail(CTPRJ_EINVAL, "tv_value: NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_tv_value, dim3((unsigned)w->blocks), dim3(kTvBlock), 0, st, d_x, w->rows,
+                       w->cols, w->n, w->iso, w->d_part);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "tv value launch");
+    double s[4];
+    int rc = tv_sums(w, st, s);
+    if (rc) return rc;
+    *tv = s[0];
+    return 0;
+}
+
+extern "C" int ctprj_tv_denoise(ctprj_tv* w, const float* d_x, double lambda, double epsilon,
+                                int maxIter, float* d_out, int* iterations, void* stream)
+{
+    if (!w || !d_x || !d_out) return fail(CTPRJ_EINVAL, "tv_denoise: NULL argument");
+    if (!(lambda > 0.0) || !std::isfinite(lambda) || !std::isfinite(epsilon) || maxIter < 0)
+        return fail(CTPRJ_EINVAL, "tv_denoise: lambda must be > 0, epsilon finite, maxIter >= 0");
+    hipStream_t st = (hipStream_t)stream;
+    const long long n = w->n;
+    const unsigned g = (unsigned)w->blocks;
+    hipLaunchKernelGGL(k_tv_in, dim3(g), dim3(kTvBlock), 0, st, d_x, w->d_X, n);
+    // D = 0, P = proj(Ltrans(0)) = 0, R = P   (utils/denoise_bound_mod.m:60-86)
+    hipError_t e = hipSuccess;
+    for (double* p : {w->d_D, w->d_P1, w->d_P2, w->d_R1, w->d_R2})
+        if (e == hipSuccess) e = hipMemsetAsync(p, 0, sizeof(double) * (size_t)n, st);
+    if (e != hipSuccess) return hip_fail(e, "tv init");
+    double tk = 1.0, tkp1 = 1.0, fval = INFINITY;
+    int count = 0, i = 0, rc;
+    while (i < maxIter && count < 5) {
+        const double fold = fval;
+        ++i;
+        tk = tkp1;
+        hipLaunchKernelGGL(k_tv_primal, dim3(g), dim3(kTvBlock), 0, st, w->d_X, w->d_D, w->d_R1,
+                           w->d_R2, w->rows, w->cols, n, lambda, w->d_part);
+        tkp1 = (1.0 + std::sqrt(1.0 + 4.0 * tk * tk)) / 2.0;
+        hipLaunchKernelGGL(k_tv_dual, dim3(g), dim3(kTvBlock), 0, st, w->d_D, w->d_P1, w->d_P2,
+                           w->d_R1, w->d_R2, w->rows, w->cols, n, lambda, w->iso, tk, tkp1);
+        e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "tv iteration launch");
+        double s[4];
+        if ((rc = tv_sums(w, st, s))) return rc;
+        const double re = std::sqrt(s[0]) / std::sqrt(s[1]);
+        count = (re < epsilon) ? count + 1 : 0;
+        fval = -s[2] + s[3];
+        if (fval > fold) tkp1 = 1.0;
+    }
+    hipLaunchKernelGGL(k_tv_out, dim3(g), dim3(kTvBlock), 0, st, w->d_D, d_out, n);
+    if (iterations) *iterations = i;
+    e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "tv output");
+}

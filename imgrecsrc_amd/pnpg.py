@@ -6,7 +6,8 @@ b1 / b0 / dis spectrum basis.
 run of trials can be captured once in a HIP graph (``torch.cuda.CUDAGraph``) and
 replayed; ``stats()`` reads the device scalars.  ``PNPG.set_wavelet(w, u)`` switches to
 the reference's default prox ('wvltADMM': non-negativity plus u ||W alpha||_1 solved by
-npg/admm.m), whose trials synchronise.
+npg/admm.m), ``PNPG.set_tv(tv, u)`` to 'tvl1' / 'tviso' (utils/TV.m); their trials
+synchronise.
 """
 from __future__ import annotations
 
@@ -62,6 +63,13 @@ class PNPG:
         self._wavelet = wavelet  # keeps the context alive as long as the solver
         check(lib().ctprj_pnpg_set_wavelet(self._h, wavelet.handle if wavelet is not None
                                            else None, ctypes.c_double(float(u))))
+
+    def set_tv(self, tv, u: float):
+        """TV prox u TV(alpha) (opt.proximal = 'tvl1' / 'tviso'); ``tv`` is an
+        imgrecsrc_amd.tv.TV over the plan's images (None: non-negativity only)."""
+        self._tv = tv
+        check(lib().ctprj_pnpg_set_tv(self._h, tv.handle if tv is not None else None,
+                                      ctypes.c_double(float(u))))
 
     def step(self, alpha: torch.Tensor, stream=None):
         check(lib().ctprj_pnpg_step(self._h, _ptr(alpha), _stream_ptr(stream)))

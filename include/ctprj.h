@@ -281,6 +281,26 @@ int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u, double
                             void* stream);
 
 /* ------------------------------------------------------------------------------
+ * Total-variation proximal step (opt.proximal = 'tvl1' / 'tviso',
+ * bhc/beamhardenSpline.m:215-224 -> utils/TV.m:56-80 -> utils/denoise_bound_mod.m): a
+ * context holds `batch` images of rows x cols (rows, cols >= 2) and the TV type
+ * (iso != 0: isotropic, else l1).  FP64 state (six image-sized buffers), FP32 images.
+ * ------------------------------------------------------------------------------ */
+typedef struct ctprj_tv ctprj_tv;
+int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv** out);
+int ctprj_tv_destroy(ctprj_tv* w);
+/* *tv = TV(x) summed over the images (utils/tlv.m; synchronises `stream`). */
+int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* stream);
+/* x = argmin 0.5||x - a||^2 + lambda TV(x) over x >= 0 by FGP on the dual
+ * (utils/denoise_bound_mod.m with the bounds [0, inf) of TV.denoise): stops after
+ * maxIter iterations or when ||D - Dold|| / ||D|| < epsilon five times in a row; the
+ * dual starts at zero (TV.denoise does not forward its init).  Norms and the stopping
+ * test run over all images of the context together.  *iterations gets the count.
+ * Host-driven: one small device-to-host copy per iteration. */
+int ctprj_tv_denoise(ctprj_tv* w, const float* d_a, double lambda, double epsilon,
+                     int maxIter, float* d_out, int* iterations, void* stream);
+
+/* ------------------------------------------------------------------------------
  * On-device PNPG alpha step (npg/PNPG.m:50-167 with the non-negativity prox, on the
  * poissLAlpha objective, or gaussLAlpha under the plan's Gaussian model): no reference
  * counterpart as a C call -- the reference runs this loop in MATLAB
@@ -325,6 +345,10 @@ int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
  * failed line search (PNPG.m:94-103) theta restarts at 1; the reference's reset()
  * additionally re-estimates t from the Hessian, which is not done here. */
 int ctprj_pnpg_set_wavelet(ctprj_pnpg* solver, ctprj_wavelet* w, double u);
+/* The same with the TV prox (opt.proximal = 'tvl1' / 'tviso', penalty u TV(alpha),
+ * utils/TV.m denoise with innerThresh = admmTol * difAlpha and maxInnerItr).  tv = NULL
+ * returns to the non-negativity prox. */
+int ctprj_pnpg_set_tv(ctprj_pnpg* solver, ctprj_tv* tv, double u);
 int ctprj_pnpg_destroy(ctprj_pnpg* solver);
 
 /* ------------------------------------------------------------------------------

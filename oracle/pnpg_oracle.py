@@ -90,6 +90,10 @@ class PnpgWaveletOracle(PnpgOracle):
     def _pen(self, x):
         return self.W.l1_penalty(x.reshape(self.shape), self.h, self.L)
 
+    def _prox(self, a, u, tol):
+        return self.W.admm(a, u, self.h, self.L, tol, self.maxInnerItr,
+                           self.alpha.reshape(self.shape))
+
     def trial(self):
         if self.ppp == 0:
             self.incStep = False
@@ -104,9 +108,7 @@ class PnpgWaveletOracle(PnpgOracle):
         xbar = np.maximum(self.alpha + coef * (self.alpha - self.pre), 0.0)
         f_old, g = self.func(xbar)
         a = (xbar - g / self.t).reshape(self.shape)
-        newx, self.innerSearch = self.W.admm(a, self.u / self.t, self.h, self.L,
-                                             self.admmTol * self.difAlpha, self.maxInnerItr,
-                                             self.alpha.reshape(self.shape))
+        newx, self.innerSearch = self._prox(a, self.u / self.t, self.admmTol * self.difAlpha)
         newx = newx.ravel()
         f_new, _ = self.func(newx, grad=False)
         d = newx - xbar
@@ -156,4 +158,21 @@ class PnpgWaveletOracle(PnpgOracle):
         self.pre = self.alpha
         self.alpha = newx
         self.log.append(("accept", newObj))
+
+
+class PnpgTVOracle(PnpgWaveletOracle):
+    """The same trials with the TV prox ('tvl1' / 'tviso': utils/TV.m denoise with
+    innerThresh = admmTol*difAlpha, penalty utils/tlv.m)."""
+
+    def __init__(self, func, alpha0, t0, u, tv, shape):
+        super().__init__(func, alpha0, t0, u, None, 0, shape)
+        from oracle import tv_oracle as T
+        self.T = T
+        self.tv = tv
+
+    def _pen(self, x):
+        return self.T.tlv(x.reshape(self.shape), self.tv)
+
+    def _prox(self, a, u, tol):
+        return self.T.denoise(a, u, self.tv, tol, self.maxInnerItr)
 
