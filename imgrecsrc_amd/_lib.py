@@ -114,6 +114,7 @@ SIGNATURES = [
     ("ctprj_pnpg_create", _i, [_vp, _vp, _dp, _i, _dp, _f, _vp, ctypes.c_double,
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),
+    ("ctprj_pnpg_set_method", _i, [_vp, _i, _i]),
     ("ctprj_pnpg_set_wavelet", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_set_tv", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),

@@ -1,5 +1,6 @@
 // On-device PNPG for the alpha step of the beam-hardening reconstruction: the projected
-// Nesterov proximal-gradient method of npg/PNPG.m:50-167 with the non-negativity prox
+// Nesterov proximal-gradient method of npg/PNPG.m:50-167 (or its NPG variant,
+// npg/NPG.m:45-148, the reference's default alphaStep) with the non-negativity prox
 // (forcePositive, u = 0), on the polychromatic Poisson objective of bhc/poissLAlpha.m.
 //
 // Every decision of the method (momentum theta, the majorization test of
@@ -40,34 +41,98 @@ struct PnpgState {
     int incStep;
     int accept;       // this trial accepted
     int restarts, rejects;
+    int outer;        // NPG: outer iterations begun (NPG.m obj.p, retries included)
+    int goodStep;     // NPG: no failed test in this iteration yet
 };
 
-// PNPG.m properties (defaults of the reference class)
+// PNPG.m / NPG.m properties (defaults of the reference classes)
 struct PnpgParams {
     double stepShrnk;  // 0.5: t <- t/stepShrnk on a failed test
-    double stepIncre;  // 0.9: t <- t*stepIncre after cumuTol good steps
-    double gamma, a;   // theta growth: 1/gamma + sqrt(a + B theta^2)
+    double stepIncre;  // 0.9: PNPG t <- t*stepIncre after cumuTol good steps
+    double gamma, a;   // PNPG theta growth: 1/gamma + sqrt(a + B theta^2)
     int adaptive;      // adaptiveStep
     int restart;       // restart >= 0 enables restarts
     int incCumuTol;
+    int npg;           // 1: the NPG variant (npg/NPG.m:45-148)
+    int preSteps;      // NPG: iterations whose line search also grows the step
 };
+
+// Start of a trial.  PNPG (PNPG.m:56-77): theta and xbar are recomputed each trial
+// from B = t/preT.  NPG (NPG.m:48-66): theta = (1 + sqrt(1 + 4 theta^2))/2 and xbar once
+// per outer iteration, the adaptive increase divides t by 4 (stepShrnk applied twice).
+__host__ __device__ inline void pnpg_begin(PnpgState& S, const PnpgParams& P)
+{
+    if (S.ppp == 0) {
+        S.incStep = 0;
+        if (P.npg) {
+            S.outer += 1;
+            S.goodStep = 1;
+            S.newTheta = (1.0 + sqrt(1.0 + 4.0 * S.theta * S.theta)) / 2.0;
+            S.coef = (S.theta - 1.0) / S.newTheta;
+        }
+        if (P.adaptive && S.cumu >= S.cumuTol) {
+            if (P.npg) {
+                S.t = S.t * P.stepShrnk;
+                S.t = S.t * P.stepShrnk;
+            } else {
+                S.t *= P.stepIncre;
+            }
+            S.cumu = 0;
+            S.incStep = 1;
+        }
+    }
+    S.ppp += 1;
+    S.trials += 1;
+    if (!P.npg) {
+        const double B = S.t / S.preT;
+        S.newTheta = (S.p == 0) ? 1.0 : 1.0 / P.gamma + sqrt(P.a + B * S.theta * S.theta);
+        S.coef = (S.theta - 1.0) / S.newTheta;
+    }
+}
+
+// Line-search outcome of a trial: 0 = another trial follows (shrunk or, NPG's first
+// preSteps iterations, grown step), 1 = the test holds, 2 = given up (goodMM false).
+// PNPG.m:83-103, NPG.m:71-96.
+__host__ __device__ inline int pnpg_linesearch(PnpgState& S, const PnpgParams& P, bool holds)
+{
+    if (holds) {
+        if (P.npg && S.outer <= P.preSteps && S.ppp < 18 && S.goodStep && S.t > 0) {
+            S.t = S.t * P.stepShrnk;
+            return 0;
+        }
+        return 1;
+    }
+    if (S.ppp <= 20 && S.t > 0) {
+        S.t = S.t / P.stepShrnk;
+        S.goodStep = 0;
+        if (S.incStep) {
+            if (P.incCumuTol) S.cumuTol += 4;
+            S.incStep = 0;
+        }
+        S.rejects += 1;
+        return 0;
+    }
+    return 2;
+}
+
+// Accepting newX with objective newObj (PNPG.m:147-159, NPG.m:137-146).
+__host__ __device__ inline void pnpg_accept(PnpgState& S, const PnpgParams& P, double newObj)
+{
+    S.accept = 1;
+    S.cumu = (S.ppp == 1 && P.adaptive) ? S.cumu + 1 : 0;
+    S.theta = S.newTheta;
+    S.cost = newObj;
+    S.preT = S.t;
+    S.p += 1;
+    S.ppp = 0;
+}
 
 __global__ void k_pnpg_begin(PnpgState* S, PnpgParams P)
 {
     if (threadIdx.x | blockIdx.x) return;
-    if (S->ppp == 0) {  // first trial of an iteration (PNPG.m:58-63)
-        S->incStep = 0;
-        if (P.adaptive && S->cumu >= S->cumuTol) {
-            S->t *= P.stepIncre;
-            S->cumu = 0;
-            S->incStep = 1;
-        }
-    }
-    S->ppp += 1;
-    S->trials += 1;
-    const double B = S->t / S->preT;
-    S->newTheta = (S->p == 0) ? 1.0 : 1.0 / P.gamma + sqrt(P.a + B * S->theta * S->theta);
-    S->coef = (S->theta - 1.0) / S->newTheta;
+    PnpgState s = *S;
+    pnpg_begin(s, P);
+    *S = s;
 }
 
 // xbar = max(alpha + coef (alpha - preAlpha), 0)   (PNPG.m:73-77)
@@ -146,32 +211,19 @@ __global__ void k_pnpg_dot_reduce(const double* __restrict__ partials, long long
 __global__ void k_pnpg_decide(PnpgState* S, PnpgParams P)
 {
     if (threadIdx.x | blockIdx.x) return;
-    const bool holds = (S->f_new - S->f_old) <= S->dg + 0.5 * S->t * S->dd;
-    S->accept = 0;
-    if (!holds) {
-        if (S->ppp <= 20 && S->t > 0) {
-            S->t = S->t / P.stepShrnk;
-            if (S->incStep) {
-                if (P.incCumuTol) S->cumuTol += 4;
-                S->incStep = 0;
-            }
-            S->rejects += 1;
-            return;  // retry the same iteration on the next call
-        }
-        // PNPG.m:94-103 gives up on the line search: accept as it is (goodMM = false)
-    } else if (S->p > 0 && S->f_new > S->cost && S->coef != 0.0 && P.restart >= 0) {
-        S->theta = 1.0;  // restart: drop the momentum and retry (PNPG.m:103-112)
-        S->restarts += 1;
-        S->ppp = 0;
-        return;
+    PnpgState s = *S;
+    const bool holds = (s.f_new - s.f_old) <= s.dg + 0.5 * s.t * s.dd;
+    s.accept = 0;
+    const int ls = pnpg_linesearch(s, P, holds);
+    if (ls == 1 && s.p > 0 && s.f_new > s.cost && s.coef != 0.0 && P.restart >= 0) {
+        s.theta = 1.0;  // restart: drop the momentum and retry (PNPG.m:103-112)
+        s.restarts += 1;
+        s.ppp = 0;
+    } else if (ls != 0) {
+        // a given-up line search (ls == 2) is accepted as it is (goodMM = false)
+        pnpg_accept(s, P, s.f_new);
     }
-    S->accept = 1;
-    S->cumu = (S->ppp == 1 && P.adaptive) ? S->cumu + 1 : 0;
-    S->theta = S->newTheta;
-    S->cost = S->f_new;
-    S->preT = S->t;
-    S->p += 1;
-    S->ppp = 0;
+    *S = s;
 }
 
 // wavelet prox: the ADMM input xbar - grad/t   (PNPG.m:78-79)
@@ -288,7 +340,7 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
     s->d_Imea = d_Imea;
     s->Ts = Ts;
     s->bp = bp;
-    s->pp = PnpgParams{0.5, 0.9, 2.0, 0.25, 1, 0, 1};
+    s->pp = PnpgParams{0.5, 0.9, 2.0, 0.25, 1, 0, 1, 0, 2};
     s->nimg = plan->img_elems;
     s->nsino = plan->sino_elems;
     s->prox_blocks = (s->nimg + 255) / 256;
@@ -374,19 +426,7 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     PnpgState h;
     int rc = state_get(s, &h, st);
     if (rc) return rc;
-    if (h.ppp == 0) {  // PNPG.m:56-63
-        h.incStep = 0;
-        if (P.adaptive && h.cumu >= h.cumuTol) {
-            h.t *= P.stepIncre;
-            h.cumu = 0;
-            h.incStep = 1;
-        }
-    }
-    h.ppp += 1;
-    h.trials += 1;
-    const double B = h.t / h.preT;
-    h.newTheta = (h.p == 0) ? 1.0 : 1.0 / P.gamma + std::sqrt(P.a + B * h.theta * h.theta);
-    h.coef = (h.theta - 1.0) / h.newTheta;
+    pnpg_begin(h, P);
     h.accept = 0;
     if ((rc = state_put(s, &h, st))) return rc;
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
@@ -419,19 +459,9 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     h.dd = sum[1];
     // majorization test of the smooth part (utils/Utils.m:470-480)
     const bool holds = (h.f_new - h.f_old) <= h.dg + 0.5 * h.t * h.dd;
-    bool goodMM = true;
-    if (!holds) {
-        if (h.ppp <= 20 && h.t > 0) {
-            h.t = h.t / P.stepShrnk;
-            if (h.incStep) {
-                if (P.incCumuTol) h.cumuTol += 4;
-                h.incStep = 0;
-            }
-            h.rejects += 1;
-            return state_put(s, &h, st);
-        }
-        goodMM = false;
-    }
+    const int ls = pnpg_linesearch(h, P, holds);
+    if (ls == 0) return state_put(s, &h, st);
+    const bool goodMM = ls == 1;
     double pen_new, pen_bar;
     if ((rc = prox_penalty(s, s->d_newx, &pen_new, st)) ||
         (rc = prox_penalty(s, s->d_xbar, &pen_bar, st)))
@@ -462,14 +492,8 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
         }
     }
     // accept (PNPG.m:147-160)
-    h.theta = h.newTheta;
-    h.cost = newObj;
+    pnpg_accept(h, P, newObj);
     s->difAlpha = keep ? 0.0 : std::sqrt(sum[3]) / std::max(std::sqrt(sum[4]), 1e-10);
-    h.preT = h.t;
-    h.cumu = (h.ppp == 1 && P.adaptive) ? h.cumu + 1 : 0;
-    h.p += 1;
-    h.ppp = 0;
-    h.accept = 1;
     if (keep) {
         e = hipMemcpyAsync(s->d_newx, d_alpha, sizeof(float) * (size_t)s->nimg,
                            hipMemcpyDeviceToDevice, st);
@@ -499,6 +523,19 @@ static int set_prox(ctprj_pnpg* s, ctprj_wavelet* w, ctprj_tv* tv, double u)
     s->wv = w;
     s->tv = tv;
     s->u = (w || tv) ? u : 0.0;
+    return 0;
+}
+
+extern "C" int ctprj_pnpg_set_method(ctprj_pnpg* s, int npg, int preSteps)
+{
+    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_method: NULL solver");
+    if (preSteps < 0) return fail(CTPRJ_EINVAL, "preSteps must be >= 0");
+    PnpgState h;
+    int rc = state_get(s, &h, nullptr);
+    if (rc) return rc;
+    if (h.trials) return fail(CTPRJ_ESTATE, "method change after the first trial");
+    s->pp.npg = npg ? 1 : 0;
+    s->pp.preSteps = preSteps;
     return 0;
 }
 

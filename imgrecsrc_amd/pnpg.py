@@ -56,6 +56,14 @@ class PNPG:
                                       ctypes.c_double(self.t0), ctypes.byref(h)))
         self._h = h
 
+    def set_method(self, method: str = "PNPG", preSteps: int = 2):
+        """'PNPG' (npg/PNPG.m) or 'NPG' (npg/NPG.m, opt.alphaStep = 'NPG'); before the
+        first step."""
+        if method.upper() not in ("PNPG", "NPG"):
+            raise ValueError("method must be 'PNPG' or 'NPG'")
+        check(lib().ctprj_pnpg_set_method(self._h, 1 if method.upper() == "NPG" else 0,
+                                          int(preSteps)))
+
     def set_wavelet(self, wavelet, u: float):
         """Wavelet l1 prox u ||mdwt(alpha)||_1 (opt.proximal = 'wvltADMM'); ``wavelet`` is an
         imgrecsrc_amd.wavelet.Wavelet over the plan's images (None: non-negativity only).

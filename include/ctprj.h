@@ -334,6 +334,12 @@ int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const double* knots
 int ctprj_pnpg_step(ctprj_pnpg* solver, float* d_alpha, void* stream);
 /* Copies the device scalars to `info` (synchronises `stream`). */
 int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
+/* Selects the method before the first trial: npg = 0 PNPG (npg/PNPG.m, the default
+ * here), npg = 1 NPG (npg/NPG.m:45-148, opt.alphaStep = 'NPG', the reference's default):
+ * FISTA theta (1 + sqrt(1 + 4 theta^2))/2, xbar and its gradient once per iteration (a
+ * trial that retries recomputes the same values), the adaptive increase t/4, and in
+ * the first preSteps iterations (opt.preSteps = 2) the step grows while the test holds. */
+int ctprj_pnpg_set_method(ctprj_pnpg* solver, int npg, int preSteps);
 /* Switches the prox to the wavelet l1 penalty u ||mdwt(alpha)||_1 with non-negativity
  * (opt.proximal = 'wvltADMM', bhc/beamhardenSpline.m:204-209), solved per trial by
  * ctprj_prox_wavelet_admm with tolerance admmTol * difAlpha and the reference's

@@ -4,9 +4,11 @@ npg/PNPG.m:50-167 with forcePositive and u = 0 (prox = projection on alpha >= 0)
 majorization test of utils/Utils.m:470-480, stepShrnk 0.5, stepIncre 0.9, gamma 2,
 a 1/4, adaptive steps and restarts as in the reference class defaults; one call of
 ``trial`` is one pass of the line-search loop (a failed test shrinks the step and keeps
-the iterate; a restart resets theta).  ``func(x) -> (f, grad)``.  ``PnpgWaveletOracle``
-adds the 'wvltADMM' prox (npg/admm.m via oracle/wavelet_oracle.py) with the admmTol /
-maxInnerItr schedule of PNPG.m:113-146.
+the iterate; a restart resets theta).  ``func(x) -> (f, grad)``.  With ``npg=True`` the
+NPG variant (npg/NPG.m:45-148): FISTA theta once per iteration, adaptive increase t/4,
+and in the first ``pre_steps`` iterations the step grows while the test holds.
+``PnpgWaveletOracle`` adds the 'wvltADMM' prox (npg/admm.m via oracle/wavelet_oracle.py)
+with the admmTol / maxInnerItr schedule of PNPG.m:113-146, ``PnpgTVOracle`` the TV prox.
 """
 from __future__ import annotations
 
@@ -14,7 +16,7 @@ import numpy as np
 
 
 class PnpgOracle:
-    def __init__(self, func, alpha0, t0):
+    def __init__(self, func, alpha0, t0, npg=False, pre_steps=2):
         self.func = func
         self.alpha = np.array(alpha0, np.float64)
         self.pre = self.alpha.copy()
@@ -27,46 +29,84 @@ class PnpgOracle:
         self.cumu = 0
         self.cumuTol = 4
         self.incStep = False
+        self.npg = npg
+        self.pre_steps = pre_steps
+        self.outer = 0
+        self.goodStep = True
+        self.newTheta = 1.0
+        self.coef = 0.0
         self.log = []
 
-    def trial(self):
+    def _begin(self):
+        """PNPG.m:56-77 / NPG.m:48-66: theta, the adaptive step, xbar."""
         if self.ppp == 0:
             self.incStep = False
+            if self.npg:
+                self.outer += 1
+                self.goodStep = True
+                self.newTheta = (1 + np.sqrt(1 + 4 * self.theta ** 2)) / 2
+                self.coef = (self.theta - 1.0) / self.newTheta
             if self.cumu >= self.cumuTol:
-                self.t *= 0.9
+                if self.npg:
+                    self.t = self.t * 0.5
+                    self.t = self.t * 0.5
+                else:
+                    self.t *= 0.9
                 self.cumu = 0
                 self.incStep = True
         self.ppp += 1
-        B = self.t / self.preT
-        newTheta = 1.0 if self.p == 0 else 0.5 + np.sqrt(0.25 + B * self.theta ** 2)
-        coef = (self.theta - 1.0) / newTheta
-        xbar = np.maximum(self.alpha + coef * (self.alpha - self.pre), 0.0)
-        f_old, g = self.func(xbar)
-        newx = np.maximum(xbar - g / self.t, 0.0)
-        f_new, _ = self.func(newx, grad=False)
-        d = newx - xbar
-        holds = (f_new - f_old) <= d @ g + 0.5 * self.t * (d @ d)
-        if not holds and self.ppp <= 20 and self.t > 0:
+        if not self.npg:
+            B = self.t / self.preT
+            self.newTheta = 1.0 if self.p == 0 else 0.5 + np.sqrt(0.25 + B * self.theta ** 2)
+            self.coef = (self.theta - 1.0) / self.newTheta
+        return np.maximum(self.alpha + self.coef * (self.alpha - self.pre), 0.0)
+
+    def _linesearch(self, holds, f_new):
+        """0: another trial follows, 1: the test holds, 2: given up (goodMM false)."""
+        if holds:
+            if self.npg and self.outer <= self.pre_steps and self.ppp < 18 and \
+                    self.goodStep and self.t > 0:
+                self.t = self.t * 0.5
+                self.log.append(("grow", f_new))
+                return 0
+            return 1
+        if self.ppp <= 20 and self.t > 0:
             self.t /= 0.5
+            self.goodStep = False
             if self.incStep:
                 self.cumuTol += 4
                 self.incStep = False
             self.log.append(("reject", f_new))
-            return
-        if holds and self.p > 0 and f_new > self.cost and coef != 0.0:
-            self.theta = 1.0
-            self.ppp = 0
-            self.log.append(("restart", f_new))
-            return
+            return 0
+        return 2
+
+    def _accept(self, newx, newObj):
         self.cumu = self.cumu + 1 if self.ppp == 1 else 0
-        self.theta = newTheta
-        self.cost = f_new
+        self.theta = self.newTheta
+        self.cost = newObj
         self.preT = self.t
         self.p += 1
         self.ppp = 0
         self.pre = self.alpha
         self.alpha = newx
-        self.log.append(("accept", f_new))
+        self.log.append(("accept", newObj))
+
+    def trial(self):
+        xbar = self._begin()
+        f_old, g = self.func(xbar)
+        newx = np.maximum(xbar - g / self.t, 0.0)
+        f_new, _ = self.func(newx, grad=False)
+        d = newx - xbar
+        holds = (f_new - f_old) <= d @ g + 0.5 * self.t * (d @ d)
+        ls = self._linesearch(holds, f_new)
+        if ls == 0:
+            return
+        if ls == 1 and self.p > 0 and f_new > self.cost and self.coef != 0.0:
+            self.theta = 1.0
+            self.ppp = 0
+            self.log.append(("restart", f_new))
+            return
+        self._accept(newx, f_new)
 
 
 class PnpgWaveletOracle(PnpgOracle):
@@ -76,8 +116,8 @@ class PnpgWaveletOracle(PnpgOracle):
     PNPG.m:113-146.  A failed line search sets theta = 1 (reset() without re-estimating
     t).  ``h``, ``L``: the wavelet filter and levels."""
 
-    def __init__(self, func, alpha0, t0, u, h, L, shape):
-        super().__init__(func, alpha0, t0)
+    def __init__(self, func, alpha0, t0, u, h, L, shape, npg=False, pre_steps=2):
+        super().__init__(func, alpha0, t0, npg, pre_steps)
         from oracle import wavelet_oracle as W
         self.W = W
         self.u, self.h, self.L, self.shape = float(u), h, L, shape
@@ -95,17 +135,7 @@ class PnpgWaveletOracle(PnpgOracle):
                            self.alpha.reshape(self.shape))
 
     def trial(self):
-        if self.ppp == 0:
-            self.incStep = False
-            if self.cumu >= self.cumuTol:
-                self.t *= 0.9
-                self.cumu = 0
-                self.incStep = True
-        self.ppp += 1
-        B = self.t / self.preT
-        newTheta = 1.0 if self.p == 0 else 0.5 + np.sqrt(0.25 + B * self.theta ** 2)
-        coef = (self.theta - 1.0) / newTheta
-        xbar = np.maximum(self.alpha + coef * (self.alpha - self.pre), 0.0)
+        xbar = self._begin()
         f_old, g = self.func(xbar)
         a = (xbar - g / self.t).reshape(self.shape)
         newx, self.innerSearch = self._prox(a, self.u / self.t, self.admmTol * self.difAlpha)
@@ -113,16 +143,10 @@ class PnpgWaveletOracle(PnpgOracle):
         f_new, _ = self.func(newx, grad=False)
         d = newx - xbar
         holds = (f_new - f_old) <= d @ g + 0.5 * self.t * (d @ d)
-        goodMM = True
-        if not holds:
-            if self.ppp <= 20 and self.t > 0:
-                self.t /= 0.5
-                if self.incStep:
-                    self.cumuTol += 4
-                    self.incStep = False
-                self.log.append(("reject", f_new))
-                return
-            goodMM = False
+        ls = self._linesearch(holds, f_new)
+        if ls == 0:
+            return
+        goodMM = ls == 1
         newObj = f_new + self.u * self._pen(newx)
         objBar = f_old + self.u * self._pen(xbar)
         if newObj - self.cost > 0:
@@ -147,25 +171,17 @@ class PnpgWaveletOracle(PnpgOracle):
                     return
                 newObj = self.cost
                 newx = self.alpha
-        self.theta = newTheta
-        self.cost = newObj
         nb = np.sqrt(newx @ newx)
         self.difAlpha = np.sqrt((self.alpha - newx) @ (self.alpha - newx)) / max(nb, 1e-10)
-        self.cumu = self.cumu + 1 if self.ppp == 1 else 0
-        self.preT = self.t
-        self.p += 1
-        self.ppp = 0
-        self.pre = self.alpha
-        self.alpha = newx
-        self.log.append(("accept", newObj))
+        self._accept(newx, newObj)
 
 
 class PnpgTVOracle(PnpgWaveletOracle):
     """The same trials with the TV prox ('tvl1' / 'tviso': utils/TV.m denoise with
     innerThresh = admmTol*difAlpha, penalty utils/tlv.m)."""
 
-    def __init__(self, func, alpha0, t0, u, tv, shape):
-        super().__init__(func, alpha0, t0, u, None, 0, shape)
+    def __init__(self, func, alpha0, t0, u, tv, shape, npg=False, pre_steps=2):
+        super().__init__(func, alpha0, t0, u, None, 0, shape, npg, pre_steps)
         from oracle import tv_oracle as T
         self.T = T
         self.tv = tv

@@ -57,7 +57,8 @@ def _oracle_func(oracle_mod, P):
     return func
 
 
-def test_pnpg_matches_fp64_restatement(problem, oracle_mod):
+@pytest.mark.parametrize("method", ["PNPG", "NPG"])
+def test_pnpg_matches_fp64_restatement(problem, oracle_mod, method):
     import torch
     from oracle.pnpg_oracle import PnpgOracle
     from imgrecsrc_amd.pnpg import PNPG
@@ -66,7 +67,8 @@ def test_pnpg_matches_fp64_restatement(problem, oracle_mod):
     proj = Projector(GEOMS)
     alpha = torch.from_numpy(P["alpha0"].astype(np.float32)).cuda()
     solver = PNPG(proj, P["Imea_t"], P["knots"], P["Ie"], P["Ts"], alpha)
-    ref = PnpgOracle(_oracle_func(oracle_mod, P), P["alpha0"], solver.t0)
+    solver.set_method(method)
+    ref = PnpgOracle(_oracle_func(oracle_mod, P), P["alpha0"], solver.t0, npg=method == "NPG")
     costs = []
     for k in range(12):
         solver.step(alpha)
@@ -114,7 +116,8 @@ def test_pnpg_graph_replay_matches_eager(problem):
     assert s_graph.stats()["cost"] == s_eager.stats()["cost"]
 
 
-def test_pnpg_wavelet_prox_matches_fp64_restatement(oracle_mod):
+@pytest.mark.parametrize("method", ["PNPG", "NPG"])
+def test_pnpg_wavelet_prox_matches_fp64_restatement(oracle_mod, method):
     """The default BHC prox ('wvltADMM'): trials against the FP64 restatement with the
     wavelet ADMM of npg/admm.m; decisions identical, objective (NLL + u ||W alpha||_1) to
     1e-5, iterate to 1e-4 of its scale."""
@@ -155,20 +158,23 @@ def test_pnpg_wavelet_prox_matches_fp64_restatement(oracle_mod):
     solver = PNPG(proj, torch.from_numpy(Imea.astype(np.float32)).cuda(), knots, Ie, Ts, alpha)
     u = 2e-3 * solver.t0
     wv = Wavelet(n, n, h, L)
+    solver.set_method(method)
     solver.set_wavelet(wv, u)
-    ref = PnpgWaveletOracle(func, alpha0, solver.t0, u, h, L, (n, n))
+    ref = PnpgWaveletOracle(func, alpha0, solver.t0, u, h, L, (n, n), npg=method == "NPG")
     log = []
-    for k in range(10):
+    for k in range(12):
         solver.step(alpha)
         st = solver.stats()
         ref.trial()
         log.append((ref.log[-1][0], st["last_accepted"], st["cost"], ref.cost, st["innerSearch"],
-                    ref.innerSearch))
+                    ref.innerSearch, st["t"], ref.t))
+        assert st["t"] == pytest.approx(ref.t, rel=1e-12), log
         assert (st["last_accepted"] == 1) == (ref.log[-1][0] == "accept"), log
         if np.isfinite(ref.cost):
             assert abs(st["cost"] - ref.cost) <= 1e-5 * abs(ref.cost), log
     print("\n".join(map(str, log)))
-    assert st["iterations"] == ref.p and st["iterations"] >= 5
+    # NPG spends its first iterations growing the step (preSteps)
+    assert st["iterations"] == ref.p and st["iterations"] >= (3 if method == "NPG" else 5)
     got = alpha.cpu().numpy().astype(np.float64)
     assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()
     assert 0 < st["innerSearch"] <= st["maxInnerItr"]
