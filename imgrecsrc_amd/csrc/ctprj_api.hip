@@ -681,7 +681,8 @@ int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool
     if (n_syma_items) {
         for (int g = 1; g < 4; ++g)
             if (syma_count[g])
-                launch_adjoint_sym(fbp, g, tune.adj_seg, syma_count[g], d_desc,
+                launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4, syma_count[g],
+                                   d_desc,
                                    d_syma_items + syma_start[g], d_views, d_rays, d_sino,
                                    d_zero_row, d_symtab, d_img, st);
         launch_zero_border(n_border, d_desc, d_border_ids, d_img, st);
@@ -779,7 +780,7 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
         t->fwd_seg != 16)
         return fail(CTPRJ_EINVAL, "fwd_seg must be 1, 2, 4, 8 or 16");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
-    if (t->flags & ~15) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
+    if (t->flags & ~31) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
     const bool tables = (t->flags != p->tune.flags);
@@ -804,8 +805,8 @@ static std::string tune_key(const ctprj_plan* p, const char* arch)
 
 // Times the kernel shapes on this plan's geometry: general forward threads per group,
 // symmetric forward waves per ray run, general adjoint tile edge, symmetric adjoint waves
-// per tile, and (several slices) the work order of tune flag bit 3; keeps the symmetry
-// flags (bits 0-2).
+// per tile and bin window width (flag bit 4), and (several slices) the work order of
+// tune flag bit 3; keeps the symmetry flags (bits 0-2).
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -893,6 +894,11 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     if (!rc && p->n_symf_items) rc = search(&ctprj_tune::fwd_seg, fsegs, 5, true);
     if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);
     if (!rc && p->n_syma_items) rc = search(&ctprj_tune::adj_seg, asegs, 4, false);
+    if (!rc && p->n_syma_items) {
+        // symmetric adjoint bin windows: 4 wide, or 3 wide (flag bit 4)
+        const int wins[2] = {best.flags & ~16, best.flags | 16};
+        rc = search(&ctprj_tune::flags, wins, 2, false);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     free_dev(d_img);

@@ -93,7 +93,7 @@ void launch_scale(float* p, long long n, float s, hipStream_t st);
 void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
                         const float4* views, const Ray* rays, const float* img,
                         const float* imgT, float* sino, hipStream_t st);
-void launch_adjoint_sym(bool fbp, int grp, int seg, int nitems, const SliceDesc* desc,
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nitems, const SliceDesc* desc,
                         const AdjItem* items, const float4* views, const Ray* rays,
                         const float* sino, const float* zero_row, const int* symtab, float* img,
                         hipStream_t st);

@@ -115,6 +115,28 @@ struct Quad {
     float q[4];
 };
 
+// N (3 or 4) consecutive floats at a 4-byte aligned offset: one 12- or 16-byte load.
+template <int N>
+struct Win {
+    float q[N];
+};
+
+template <int N>
+__device__ __forceinline__ Win<N> ldbw(__amdgpu_buffer_rsrc_t r, int voff, int soff)
+{
+    Win<N> w;
+    if constexpr (N == 4) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) w.q[k] = __uint_as_float(v[k]);
+    } else {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b96(r, voff, soff, 0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) w.q[k] = __uint_as_float(v[k]);
+    }
+    return w;
+}
+
 __device__ __forceinline__ Quad ldb4(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 {
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
@@ -318,7 +340,7 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
 // `tab` holds the view-row entries of (a0, g') for the eight h (wave-uniform).  The
 // bins [blo, bhi] of the pixel are read as 4-bin windows (one 16-byte load per image),
 // clamped into [0, W - 4] so they stay inside the row, reflected rows at 2pC - b.
-template <bool FAN, bool FBP, unsigned HMASK>
+template <bool FAN, bool FBP, unsigned HMASK, int WIN>
 __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __restrict__ rview,
                                              __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
                                              const int* __restrict__ tab, float cT, float sT,
@@ -348,16 +370,17 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
     float part[8];
 #pragma unroll
     for (int h = 0; h < 8; ++h) part[h] = 0.0f;
-    const int two_pc = 2 * S.pC, smax = S.W - 4;
-    for (int s0 = blo; s0 <= bhi; s0 += 4) {
+    const int two_pc = 2 * S.pC, smax = S.W - WIN;
+    for (int s0 = blo; s0 <= bhi; s0 += WIN) {
         const int s = min(s0, smax);
-        const int ob = s * 4, of = (two_pc - s - 3) * 4;  // windows [s, s+3], [2pC-s-3, 2pC-s]
-        Quad q[8];
+        // windows [s, s+WIN-1] and the reflected [2pC-s-WIN+1, 2pC-s]
+        const int ob = s * 4, of = (two_pc - s - (WIN - 1)) * 4;
+        Win<WIN> q[8];
 #pragma unroll
         for (int h = 0; h < 8; ++h)
-            if (HMASK >> h & 1u) q[h] = ldb4(rs[h], flip[h] ? of : ob, soff[h]);
+            if (HMASK >> h & 1u) q[h] = ldbw<WIN>(rs[h], flip[h] ? of : ob, soff[h]);
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < WIN; ++j) {
             const int b = s + j;
             if (b < s0 || b > bhi) continue;
             Ray r;
@@ -371,7 +394,7 @@ __device__ __forceinline__ void adj_sym_elem(const SliceDesc& S, const Ray* __re
 #pragma unroll
             for (int h = 0; h < 8; ++h)
                 if (HMASK >> h & 1u)
-                    part[h] = fmaf(w, flip[h] ? q[h].q[3 - j] : q[h].q[j], part[h]);
+                    part[h] = fmaf(w, flip[h] ? q[h].q[WIN - 1 - j] : q[h].q[j], part[h]);
         }
     }
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258); qe of g'^-1 q0 at a0 is the depth of
@@ -408,7 +431,7 @@ __device__ __forceinline__ Ray ldray(__amdgpu_buffer_rsrc_t r, int voff, int sof
 // (a pixel meets at most three bins when the detector pitch is at least the pixel's;
 // the fourth is loaded where needed), RR being the slice's ray table and rrow the
 // view's row.
-template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD>
+template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD, int WIN>
 __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* __restrict__ rview,
                                                 __amdgpu_buffer_rsrc_t RS,
                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
@@ -435,26 +458,27 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
     const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
-    const int two_pc = 2 * S.pC, smax = S.W - 4;
-    for (int s0 = blo; s0 <= bhi; s0 += 4) {
+    const int two_pc = 2 * S.pC, smax = S.W - WIN;
+    for (int s0 = blo; s0 <= bhi; s0 += WIN) {
         const int s = min(s0, smax);
-        const int ob = s * 4, of = (two_pc - s - 3) * 4;  // windows [s, s+3], [2pC-s-3, 2pC-s]
+        // windows [s, s+WIN-1] and the reflected [2pC-s-WIN+1, 2pC-s]
+        const int ob = s * 4, of = (two_pc - s - (WIN - 1)) * 4;
         Ray rq[3];
         if (FAN) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) rq[j] = ldray(RR, (s + j) * (int)sizeof(Ray), rrow);
         }
-        Quad q[8];
+        Win<WIN> q[8];
 #pragma unroll
         for (int h = 0; h < 8; ++h)
-            if (HMASK >> h & 1u) q[h] = ldb4(RS, elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
+            if (HMASK >> h & 1u) q[h] = ldbw<WIN>(RS, elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
         if (none) {  // wave-uniform
 #pragma unroll
             for (int h = 0; h < 8; ++h)
-                if ((HMASK >> h & 1u) && (none >> h & 1u)) q[h] = Quad{{0.0f, 0.0f, 0.0f, 0.0f}};
+                if ((HMASK >> h & 1u) && (none >> h & 1u)) q[h] = Win<WIN>{};
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < WIN; ++j) {
             const int b = s + j;
             if (b < s0 || b > bhi) continue;
             Ray r;
@@ -469,13 +493,13 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
 #pragma unroll
             for (int h = 0; h < 8; ++h)
                 if (HMASK >> h & 1u)
-                    vpart[h] = fmaf(w, q[h].q[elem_flip(h, E2, FOLD) ? 3 - j : j], vpart[h]);
+                    vpart[h] = fmaf(w, q[h].q[elem_flip(h, E2, FOLD) ? WIN - 1 - j : j], vpart[h]);
         }
     }
 }
 
 // All elements of a canonical view off the mirror lines, unrolled at compile time.
-template <bool FAN, bool FBP, int GRP, bool FOLD>
+template <bool FAN, bool FBP, int GRP, bool FOLD, int WIN>
 __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* __restrict__ rview,
                                                 __amdgpu_buffer_rsrc_t RS,
                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
@@ -486,14 +510,14 @@ __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* _
     constexpr unsigned GSET = FOLD ? G.fold : G.all;
 #define CTPRJ_E(E2)                                                                        \
     if constexpr ((GSET >> E2) & 1u)                                                        \
-        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD>(S, rview, RS, RR, rrow, tv + 8 * E2, cT, \
-                                                   sT, x0, y0, vpart);
+        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RR, rrow, tv + 8 * E2, \
+                                                        cT, sT, x0, y0, vpart);
     CTPRJ_E(0) CTPRJ_E(1) CTPRJ_E(2) CTPRJ_E(3) CTPRJ_E(4) CTPRJ_E(5) CTPRJ_E(6) CTPRJ_E(7)
 #undef CTPRJ_E
 }
 
 // Canonical views i = i0, i0 + step, ... for the orbit of (x0, y0) under group GRP.
-template <bool FAN, bool FBP, int GRP>
+template <bool FAN, bool FBP, int GRP, int WIN>
 __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* __restrict__ views,
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ sino,
@@ -525,21 +549,21 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
 #pragma unroll 1
             for (int e = 0; e < 8; ++e)
                 if (grot >> e & 1u)
-                    adj_sym_elem<FAN, FBP, G.rot>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
-                                                  e >> 2, x0, y0, vpart);
+                    adj_sym_elem<FAN, FBP, G.rot, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
+                                                       e & 3, e >> 2, x0, y0, vpart);
 #pragma unroll 1
             for (int e = 0; e < 8; ++e)
                 if (gref >> e & 1u)
-                    adj_sym_elem<FAN, FBP, REF>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y, e & 3,
-                                                e >> 2, x0, y0, vpart);
+                    adj_sym_elem<FAN, FBP, REF, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
+                                                     e & 3, e >> 2, x0, y0, vpart);
         } else if (GRP != kGrpZ2 && S.fold) {
             // a half-turn parallel scan takes the half set that maps the canonical range
             // into [0, 180) and reads the rest folded
-            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2>(S, rview, RS, RR, rrow, tv, T.x, T.y,
-                                                         x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2, WIN>(S, rview, RS, RR, rrow, tv, T.x,
+                                                              T.y, x0, y0, vpart);
         } else {
-            adj_sym_view_ct<FAN, FBP, GRP, false>(S, rview, RS, RR, rrow, tv, T.x, T.y, x0, y0,
-                                                  vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, false, WIN>(S, rview, RS, RR, rrow, tv, T.x, T.y, x0,
+                                                       y0, vpart);
         }
 #pragma unroll
         for (int h = 0; h < 8; ++h)
@@ -550,7 +574,7 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
 // Orbit base pixels of a work item: an 8x8 tile of the fundamental domain of the group,
 //   D4: 0 <= y0 <= x0 <= H,  V4: 0 <= x0, y0 <= H,  Z2: 0 <= x0 <= H, -H <= y0 <= H
 // (tiles start inside it; the upper ends are checked here).
-template <bool FBP, int SEG, int GRP>
+template <bool FBP, int SEG, int GRP, int WIN>
 __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& it,
                                              const float4* __restrict__ views,
                                              const Ray* __restrict__ rays,
@@ -569,11 +593,11 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
     KahanAcc acc[8];
     if (valid) {
         if (S.fan)
-            adj_sym_pixel<true, FBP, GRP>(S, views, rays, sino, zero_row, symtab, x0, y0, wave,
-                                          SEG, acc);
+            adj_sym_pixel<true, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
+                                               wave, SEG, acc);
         else
-            adj_sym_pixel<false, FBP, GRP>(S, views, rays, sino, zero_row, symtab, x0, y0, wave,
-                                           SEG, acc);
+            adj_sym_pixel<false, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
+                                                wave, SEG, acc);
     }
     if (SEG > 1) {
 #pragma unroll
@@ -609,7 +633,7 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 
 // One launch per symmetry group; SEG waves split the canonical views (wave w takes
 // i = w, w + SEG, ...) and then share the final sums.
-template <bool FBP, int SEG, int GRP>
+template <bool FBP, int SEG, int GRP, int WIN>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
@@ -623,7 +647,7 @@ __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __res
     const AdjItem it = items[blockIdx.x];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
-    adj_sym_tile<FBP, SEG, GRP>(S, it, views, rays, sino, zero_row, symtab, img, red);
+    adj_sym_tile<FBP, SEG, GRP, WIN>(S, it, views, rays, sino, zero_row, symtab, img, red);
 }
 
 // Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the
@@ -671,15 +695,15 @@ void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, con
         launch_forward_sym_g<kGrpZ2>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
 }
 
-template <bool FBP, int GRP>
+template <bool FBP, int GRP, int WIN>
 static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, const AdjItem* items,
                                  const float4* views, const Ray* rays, const float* sino,
                                  const float* zero_row, const int* symtab, float* img,
                                  hipStream_t st)
 {
 #define CTPRJ_LAUNCH(SEG)                                                                   \
-    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP>), dim3(nitems), dim3(64 * SEG), 0, st, \
-                       desc, items, views, rays, sino, zero_row, symtab, img)
+    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN>), dim3(nitems), dim3(64 * SEG), 0, \
+                       st, desc, items, views, rays, sino, zero_row, symtab, img)
     switch (seg) {
         case 1: CTPRJ_LAUNCH(1); break;
         case 2: CTPRJ_LAUNCH(2); break;
@@ -689,34 +713,40 @@ static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, con
 #undef CTPRJ_LAUNCH
 }
 
-template <bool FBP>
+template <bool FBP, int WIN>
 static void launch_adjoint_sym_f(int grp, int seg, int nitems, const SliceDesc* desc,
                                  const AdjItem* items, const float4* views, const Ray* rays,
                                  const float* sino, const float* zero_row, const int* symtab,
                                  float* img, hipStream_t st)
 {
     if (grp == kGrpD4)
-        launch_adjoint_sym_t<FBP, kGrpD4>(seg, nitems, desc, items, views, rays, sino, zero_row,
+        launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
                                           symtab, img, st);
     else if (grp == kGrpV4)
-        launch_adjoint_sym_t<FBP, kGrpV4>(seg, nitems, desc, items, views, rays, sino, zero_row,
+        launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
                                           symtab, img, st);
     else
-        launch_adjoint_sym_t<FBP, kGrpZ2>(seg, nitems, desc, items, views, rays, sino, zero_row,
+        launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
                                           symtab, img, st);
 }
 
-void launch_adjoint_sym(bool fbp, int grp, int seg, int nitems, const SliceDesc* desc,
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nitems, const SliceDesc* desc,
                         const AdjItem* items, const float4* views, const Ray* rays,
                         const float* sino, const float* zero_row, const int* symtab, float* img,
                         hipStream_t st)
 {
-    if (fbp)
-        launch_adjoint_sym_f<true>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                   symtab, img, st);
+    if (fbp && win == 3)
+        launch_adjoint_sym_f<true, 3>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                      symtab, img, st);
+    else if (fbp)
+        launch_adjoint_sym_f<true, 4>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                      symtab, img, st);
+    else if (win == 3)
+        launch_adjoint_sym_f<false, 3>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                       symtab, img, st);
     else
-        launch_adjoint_sym_f<false>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                    symtab, img, st);
+        launch_adjoint_sym_f<false, 4>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
+                                       symtab, img, st);
 }
 
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,

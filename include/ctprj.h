@@ -111,7 +111,8 @@ typedef struct ctprj_tune {
     int flags;       /* bit0: use the symmetric kernels where the geometry allows;
                         bit1: not the 8-fold group; bit2: not the 4-fold group;
                         bit3: work order keeps small slices whole on one XCD (else every
-                        slice is spread over all XCDs) */
+                        slice is spread over all XCDs);
+                        bit4: the symmetric adjoint reads 3-bin windows (else 4) */
     int adj_seg;     /* symmetric adjoint: waves per 8x8 tile of orbit base pixels
                         splitting the views: 1, 2, 4 or 8 */
     int fwd_seg;     /* symmetric forward: waves per 64 canonical rays splitting the

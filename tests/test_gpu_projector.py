@@ -224,6 +224,29 @@ def test_work_order_does_not_change_results(torch_dev):
     assert torch.equal(out[0][0], out[1][0]) and torch.equal(out[0][1], out[1][1])
 
 
+WINDOW_CASES = [(g, 1) for g in TINY] + [
+    ((12, 17, 24, 24, 1.0, 1.0, 30.0), 3), ((12, 17, 24, 24, 1.0, 1.0, 30.0), 7),
+    ((9, 14, 20, 40, 1.0, 1.0, 0.0), 3), ((9, 14, 20, 40, 1.0, 1.0, 0.0), 7),
+    ((256, 367, 180, 360, 1.0, 1.0, 0.0), 1), ((257, 365, 360, 360, 1.0, 1.0, 1100.0), 1),
+    ((300, 427, 298, 298, 1.0, 1.0, 1200.0), 1), ((255, 363, 361, 361, 1.0, 1.0, 1000.0), 1),
+    ((64, 150, 60, 60, 0.45, 1.0, 200.0), 1)]  # fine pitch: several windows per pixel
+
+
+@pytest.mark.parametrize("geo,flags", WINDOW_CASES)
+def test_adjoint_window_width_is_bitwise_neutral(torch_dev, geo, flags):
+    """The symmetric adjoint's bin windows (4 wide, or 3 wide with tune flag bit 4) only
+    change how bins are loaded: every pixel still sums its bins in increasing order, so
+    the adjoint and FBP are bit-identical."""
+    import torch
+    from imgrecsrc_amd.phantom import gaussian
+    P = _proj(geo, flags)
+    y = torch.from_numpy(gaussian((geo[2], geo[1]), 13)).cuda().reshape(-1)
+    ref, fbp = P.adjoint(y).clone(), P.fbp(y).clone()
+    P.set_tune(flags=flags | 16)
+    assert torch.equal(P.adjoint(y), ref)
+    assert torch.equal(P.fbp(y), fbp)
+
+
 def test_nnz_matches_oracle(torch_dev, oracle_mod):
     """Device count of non-zero FP32 weights vs the FP64 count: overlaps of a pixel corner
     with a strip edge below ~1e-9 px^2 round to exactly 0 in FP32, so the FP32 count is
