@@ -101,6 +101,7 @@ SIGNATURES = [
                                    _dp, _dp, _vp]),
     ("ctprj_wavelet_create", _i, [_i, _i, _i, _dp, _i, _i, ctypes.POINTER(_vp)]),
     ("ctprj_wavelet_destroy", _i, [_vp]),
+    ("ctprj_wavelet_set_masks", _i, [_vp, _vp, _vp]),
     ("ctprj_dwt", _i, [_vp, _vp, _vp, _vp]),
     ("ctprj_idwt", _i, [_vp, _vp, _vp, _vp]),
     ("ctprj_wavelet_l1", _i, [_vp, _vp, _dp, _vp]),
@@ -108,6 +109,7 @@ SIGNATURES = [
                                      ctypes.POINTER(ctypes.c_int), _vp]),
     ("ctprj_tv_create", _i, [_i, _i, _i, _i, ctypes.POINTER(_vp)]),
     ("ctprj_tv_destroy", _i, [_vp]),
+    ("ctprj_tv_set_mask", _i, [_vp, _vp]),
     ("ctprj_tv_value", _i, [_vp, _vp, _dp, _vp]),
     ("ctprj_tv_denoise", _i, [_vp, _vp, ctypes.c_double, ctypes.c_double, _i, _vp,
                               ctypes.POINTER(ctypes.c_int), _vp]),
@@ -115,6 +117,7 @@ SIGNATURES = [
                                ctypes.POINTER(_vp)]),
     ("ctprj_pnpg_step", _i, [_vp, _vp, _vp]),
     ("ctprj_pnpg_set_method", _i, [_vp, _i, _i]),
+    ("ctprj_pnpg_set_mask", _i, [_vp, _vp]),
     ("ctprj_pnpg_set_wavelet", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_set_tv", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),

@@ -136,13 +136,21 @@ __global__ void k_pnpg_begin(PnpgState* S, PnpgParams P)
 }
 
 // xbar = max(alpha + coef (alpha - preAlpha), 0)   (PNPG.m:73-77)
+// Support mask: alpha is the vector of the mask's pixels (utils/maskFunc.m), so pixels
+// outside it (mask 0) stay zero; mask == NULL keeps every pixel.
+__device__ __forceinline__ float pm(const float* mask, long long i, float v)
+{
+    return (!mask || mask[i] != 0.0f) ? v : 0.0f;
+}
+
 __global__ void k_pnpg_xbar(const PnpgState* __restrict__ S, const float* __restrict__ alpha,
-                            const float* __restrict__ pre, float* __restrict__ xbar, long long n)
+                            const float* __restrict__ pre, float* __restrict__ xbar, long long n,
+                            const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float c = (float)S->coef;
-    xbar[i] = fmaxf(fmaf(c, alpha[i] - pre[i], alpha[i]), 0.0f);
+    xbar[i] = pm(mask, i, fmaxf(fmaf(c, alpha[i] - pre[i], alpha[i]), 0.0f));
 }
 
 // newX = max(xbar - grad/t, 0) and the FP64 partials of <newX-xbar, grad>, ||newX-xbar||^2
@@ -150,6 +158,7 @@ __global__ __launch_bounds__(256) void k_pnpg_prox(const PnpgState* __restrict__
                                                    const float* __restrict__ xbar,
                                                    const float* __restrict__ grad,
                                                    float* __restrict__ newx, long long n,
+                                                   const float* __restrict__ mask,
                                                    double* __restrict__ partials)
 {
     __shared__ double sg[256], sd[256];
@@ -157,10 +166,11 @@ __global__ __launch_bounds__(256) void k_pnpg_prox(const PnpgState* __restrict__
     double dg = 0.0, dd = 0.0;
     if (i < n) {
         const float it = (float)(1.0 / S->t);
-        const float x = fmaxf(fmaf(-it, grad[i], xbar[i]), 0.0f);
+        const float g = pm(mask, i, grad[i]);
+        const float x = fmaxf(fmaf(-it, g, xbar[i]), 0.0f);
         newx[i] = x;
         const double d = (double)x - (double)xbar[i];
-        dg = d * (double)grad[i];
+        dg = d * (double)g;
         dd = d * d;
     }
     const int t = threadIdx.x;
@@ -229,12 +239,12 @@ __global__ void k_pnpg_decide(PnpgState* S, PnpgParams P)
 // wavelet prox: the ADMM input xbar - grad/t   (PNPG.m:78-79)
 __global__ void k_pnpg_gradstep(const PnpgState* __restrict__ S, const float* __restrict__ xbar,
                                 const float* __restrict__ grad, float* __restrict__ a,
-                                long long n)
+                                long long n, const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float it = (float)(1.0 / S->t);
-    a[i] = fmaf(-it, grad[i], xbar[i]);
+    a[i] = fmaf(-it, pm(mask, i, grad[i]), xbar[i]);
 }
 
 // wavelet prox: FP64 partials of <newX - xbar, grad>, ||newX - xbar||^2,
@@ -308,10 +318,12 @@ struct ctprj_pnpg {
     double u = 0.0, admmTol = 1e-2, difAlpha = 1.0;
     int maxInnerItr = 100, maxPossibleInnerItr = 1000, innerSearch = 0;
     double* d_wpart = nullptr;    // 5 partials per work-group
+    float* d_mask = nullptr;      // support of alpha (NULL: every pixel)
     ~ctprj_pnpg()
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
-                        (void*)d_s, (void*)d_part, (void*)d_ray_part, (void*)d_wpart})
+                        (void*)d_s, (void*)d_part, (void*)d_ray_part, (void*)d_wpart,
+                        (void*)d_mask})
             if (p) (void)hipFree(p);
     }
 };
@@ -431,10 +443,11 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     if ((rc = state_put(s, &h, st))) return rc;
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
-    hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg);
+    hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg,
+                       s->d_mask);
     if ((rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_gradstep, dim3(gi), dim3(256), 0, st, S, s->d_xbar, s->d_grad,
-                       s->d_newx, s->nimg);
+                       s->d_newx, s->nimg, s->d_mask);
     // newX = prox(xbar - grad/t, u/t, admmTol*difAlpha, maxInnerItr, alpha)  (PNPG.m:78-82)
     if ((rc = prox_solve(s, s->d_newx, s->u / h.t, s->admmTol * s->difAlpha, d_alpha,
                          s->d_newx, &s->innerSearch, st)))
@@ -526,6 +539,18 @@ static int set_prox(ctprj_pnpg* s, ctprj_wavelet* w, ctprj_tv* tv, double u)
     return 0;
 }
 
+extern "C" int ctprj_pnpg_set_mask(ctprj_pnpg* s, const float* d_mask)
+{
+    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_mask: NULL solver");
+    if (s->d_mask) (void)hipFree(s->d_mask);
+    s->d_mask = nullptr;
+    if (!d_mask) return 0;
+    const size_t bytes = sizeof(float) * (size_t)s->nimg;
+    hipError_t e = hipMalloc((void**)&s->d_mask, bytes);
+    if (e == hipSuccess) e = hipMemcpy(s->d_mask, d_mask, bytes, hipMemcpyDeviceToDevice);
+    return e == hipSuccess ? 0 : hip_fail(e, "pnpg mask");
+}
+
 extern "C" int ctprj_pnpg_set_method(ctprj_pnpg* s, int npg, int preSteps)
 {
     if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_method: NULL solver");
@@ -557,11 +582,12 @@ extern "C" int ctprj_pnpg_step(ctprj_pnpg* s, float* d_alpha, void* stream)
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
     hipLaunchKernelGGL(k_pnpg_begin, dim3(1), dim3(64), 0, st, S, s->pp);
-    hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg);
+    hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg,
+                       s->d_mask);
     int rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pnpg_prox, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, S, s->d_xbar,
-                       s->d_grad, s->d_newx, s->nimg, s->d_part);
+                       s->d_grad, s->d_newx, s->nimg, s->d_mask, s->d_part);
     hipLaunchKernelGGL(k_pnpg_dot_reduce, dim3(1), dim3(256), 0, st, s->d_part, s->prox_blocks, S);
     if ((rc = pnpg_func(s, s->d_newx, &S->f_new, nullptr, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_decide, dim3(1), dim3(64), 0, st, S, s->pp);

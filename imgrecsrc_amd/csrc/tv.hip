@@ -106,9 +106,16 @@ __global__ __launch_bounds__(kTvBlock) void k_tv_dual(const double* __restrict__
     }
 }
 
+// Support mask (utils/maskFunc.m): pixels where mask is 0 are zero; NULL keeps all.
+__device__ __forceinline__ double tv_masked(const float* mask, long long i, double v)
+{
+    return (!mask || mask[i] != 0.0f) ? v : 0.0;
+}
+
 // TV of FP32 images (utils/tlv.m): iso sum sqrt(P1^2 + P2^2), l1 sum |P1| + |P2|
 __global__ __launch_bounds__(kTvBlock) void k_tv_value(const float* __restrict__ X, int m, int n,
                                                        long long total, int iso,
+                                                       const float* __restrict__ mask,
                                                        double* __restrict__ part)
 {
     const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
@@ -116,24 +123,26 @@ __global__ __launch_bounds__(kTvBlock) void k_tv_value(const float* __restrict__
     if (t < total) {
         const int j = (int)(t % n);
         const int i = (int)((t / n) % m);
-        const double x = X[t];
-        const double p1 = i < m - 1 ? x - (double)X[t + n] : 0.0;
-        const double p2 = j < n - 1 ? x - (double)X[t + 1] : 0.0;
+        const double x = tv_masked(mask, t, X[t]);
+        const double p1 = i < m - 1 ? x - tv_masked(mask, t + n, X[t + n]) : 0.0;
+        const double p2 = j < n - 1 ? x - tv_masked(mask, t + 1, X[t + 1]) : 0.0;
         v[0] = iso ? sqrt(p1 * p1 + p2 * p2) : fabs(p1) + fabs(p2);
     }
     tv_block_sum4(v, part);
 }
 
-__global__ void k_tv_in(const float* __restrict__ in, double* __restrict__ out, long long n)
+__global__ void k_tv_in(const float* __restrict__ in, double* __restrict__ out, long long n,
+                        const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * kTvBlock + threadIdx.x;
-    if (i < n) out[i] = (double)in[i];
+    if (i < n) out[i] = tv_masked(mask, i, (double)in[i]);
 }
 
-__global__ void k_tv_out(const double* __restrict__ in, float* __restrict__ out, long long n)
+__global__ void k_tv_out(const double* __restrict__ in, float* __restrict__ out, long long n,
+                         const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * kTvBlock + threadIdx.x;
-    if (i < n) out[i] = (float)in[i];
+    if (i < n) out[i] = (float)tv_masked(mask, i, in[i]);
 }
 
 }  // namespace ctprj
@@ -145,11 +154,12 @@ struct ctprj_tv {
     long long n = 0, blocks = 0;
     double *d_X = nullptr, *d_D = nullptr, *d_P1 = nullptr, *d_P2 = nullptr, *d_R1 = nullptr,
            *d_R2 = nullptr, *d_part = nullptr;
+    float* d_mask = nullptr;  // support (NULL: every pixel)
     std::vector<double> part;
     ~ctprj_tv()
     {
         for (void* p : {(void*)d_X, (void*)d_D, (void*)d_P1, (void*)d_P2, (void*)d_R1, (void*)d_R2,
-                        (void*)d_part})
+                        (void*)d_part, (void*)d_mask})
             if (p) (void)hipFree(p);
     }
 };
@@ -201,6 +211,18 @@ extern "C" int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv*
     return 0;
 }
 
+extern "C" int ctprj_tv_set_mask(ctprj_tv* w, const float* d_mask)
+{
+    if (!w) return fail(CTPRJ_EINVAL, "tv_set_mask: NULL context");
+    if (w->d_mask) (void)hipFree(w->d_mask);
+    w->d_mask = nullptr;
+    if (!d_mask) return 0;
+    const size_t bytes = sizeof(float) * (size_t)w->n;
+    hipError_t e = hipMalloc((void**)&w->d_mask, bytes);
+    if (e == hipSuccess) e = hipMemcpy(w->d_mask, d_mask, bytes, hipMemcpyDeviceToDevice);
+    return e == hipSuccess ? 0 : hip_fail(e, "tv mask");
+}
+
 extern "C" int ctprj_tv_destroy(ctprj_tv* w)
 {
     delete w;
@@ -212,7 +234,7 @@ extern "C" int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* s
     if (!w || !d_x || !tv) return fail(CTPRJ_EINVAL, "tv_value: NULL argument");
     hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_tv_value, dim3((unsigned)w->blocks), dim3(kTvBlock), 0, st, d_x, w->rows,
-                       w->cols, w->n, w->iso, w->d_part);
+                       w->cols, w->n, w->iso, w->d_mask, w->d_part);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "tv value launch");
     double s[4];
@@ -231,7 +253,7 @@ extern "C" int ctprj_tv_denoise(ctprj_tv* w, const float* d_x, double lambda, do
     hipStream_t st = (hipStream_t)stream;
     const long long n = w->n;
     const unsigned g = (unsigned)w->blocks;
-    hipLaunchKernelGGL(k_tv_in, dim3(g), dim3(kTvBlock), 0, st, d_x, w->d_X, n);
+    hipLaunchKernelGGL(k_tv_in, dim3(g), dim3(kTvBlock), 0, st, d_x, w->d_X, n, w->d_mask);
     // D = 0, P = proj(Ltrans(0)) = 0, R = P   (utils/denoise_bound_mod.m:60-86)
     hipError_t e = hipSuccess;
     for (double* p : {w->d_D, w->d_P1, w->d_P2, w->d_R1, w->d_R2})
@@ -257,7 +279,7 @@ extern "C" int ctprj_tv_denoise(ctprj_tv* w, const float* d_x, double lambda, do
         fval = -s[2] + s[3];
         if (fval > fold) tkp1 = 1.0;
     }
-    hipLaunchKernelGGL(k_tv_out, dim3(g), dim3(kTvBlock), 0, st, w->d_D, d_out, n);
+    hipLaunchKernelGGL(k_tv_out, dim3(g), dim3(kTvBlock), 0, st, w->d_D, d_out, n, w->d_mask);
     if (iterations) *iterations = i;
     e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "tv output");

@@ -129,17 +129,25 @@ __global__ __launch_bounds__(kWvBlock) void k_idwt_rows(WvTaps T, const double* 
     dst[base + 2 * i + 1] = o;
 }
 
-__global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ out, long long n,
-                             double scale)
+// Support masks (utils/maskFunc.m with tomography/ConfigCT.m:85-86): an entry outside a
+// mask (mask value 0) is dropped, i.e. read as zero; mask == NULL keeps every entry.
+__device__ __forceinline__ double masked(const float* mask, long long i, double v)
 {
-    const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
-    if (i < n) out[i] = (double)in[i] * scale;
+    return (!mask || mask[i] != 0.0f) ? v : 0.0;
 }
 
-__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, long long n)
+__global__ void k_f32_to_f64(const float* __restrict__ in, double* __restrict__ out, long long n,
+                             double scale, const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
-    if (i < n) out[i] = (float)in[i];
+    if (i < n) out[i] = masked(mask, i, (double)in[i] * scale);
+}
+
+__global__ void k_f64_to_f32(const double* __restrict__ in, float* __restrict__ out, long long n,
+                             const float* __restrict__ mask)
+{
+    const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
+    if (i < n) out[i] = (float)masked(mask, i, in[i]);
 }
 
 // Per-group FP64 partial sums of up to two values, summed in a fixed LDS tree.
@@ -165,13 +173,15 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* part)
 
 // sum |x| and sum x^2 of an FP64 array
 __global__ __launch_bounds__(kWvBlock) void k_norms(const double* __restrict__ x, long long n,
+                                                    const float* __restrict__ mask,
                                                     double* __restrict__ part)
 {
     const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
     double a = 0.0, b = 0.0;
     if (i < n) {
-        a = fabs(x[i]);
-        b = x[i] * x[i];
+        const double v = masked(mask, i, x[i]);
+        a = fabs(v);
+        b = v * v;
     }
     block_sum2(a, b, part);
 }
@@ -186,10 +196,13 @@ __global__ void k_admm_sum(const double* __restrict__ s, const double* __restric
 
 // alpha = max((a + rho Psi(s + nu)) / (1 + rho), 0)   (npg/admm.m:26)
 __global__ void k_admm_alpha(const double* __restrict__ a, const double* __restrict__ psi,
-                             double rho, double* __restrict__ alpha, long long n)
+                             double rho, double* __restrict__ alpha, long long n,
+                             const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
-    if (i < n) alpha[i] = fmax(__dadd_rn(a[i], __dmul_rn(rho, psi[i])) / (1.0 + rho), 0.0);
+    if (i < n)
+        alpha[i] = fmax(__dadd_rn(a[i], __dmul_rn(rho, masked(mask, i, psi[i]))) / (1.0 + rho),
+                        0.0);
 }
 
 // s = softThresh(Psit(alpha) - nu, u/rho); nu += s - Psit(alpha); partials of
@@ -198,12 +211,14 @@ __global__ __launch_bounds__(kWvBlock) void k_admm_shrink(const double* __restri
                                                           double* __restrict__ s,
                                                           double* __restrict__ nu,
                                                           double* __restrict__ pre, double thr,
-                                                          long long n, double* __restrict__ part)
+                                                          long long n,
+                                                          const float* __restrict__ cmask,
+                                                          double* __restrict__ part)
 {
     const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
     double ds = 0.0, rs = 0.0;
     if (i < n) {
-        const double p = pa[i];
+        const double p = masked(cmask, i, pa[i]);
         const double x = p - nu[i];
         const double t = fabs(thr);
         const double si = x > t ? x - t : (x < -t ? x + t : 0.0);
@@ -218,6 +233,14 @@ __global__ __launch_bounds__(kWvBlock) void k_admm_shrink(const double* __restri
     block_sum2(ds, rs, part);
 }
 
+// x = mask ? x : 0 in place
+__global__ void k_f64_to_f64_masked(double* __restrict__ x, long long n,
+                                    const float* __restrict__ mask)
+{
+    const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
+    if (i < n) x[i] = masked(mask, i, x[i]);
+}
+
 // rho changes (npg/admm.m:43-49): nu *= f
 __global__ void k_scale64(double* __restrict__ x, double f, long long n)
 {
@@ -227,11 +250,15 @@ __global__ void k_scale64(double* __restrict__ x, double f, long long n)
 
 // out = max((a + rho Psi(s + nu)) / (1 + rho), 0) * scale   (npg/admm.m:52), FP32
 __global__ void k_admm_final(const double* __restrict__ a, const double* __restrict__ psi,
-                             double rho, double scale, float* __restrict__ out, long long n)
+                             double rho, double scale, float* __restrict__ out, long long n,
+                             const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * kWvBlock + threadIdx.x;
     if (i < n)
-        out[i] = (float)(fmax(__dadd_rn(a[i], __dmul_rn(rho, psi[i])) / (1.0 + rho), 0.0) * scale);
+        out[i] = (float)(fmax(__dadd_rn(a[i], __dmul_rn(rho, masked(mask, i, psi[i]))) /
+                                  (1.0 + rho),
+                              0.0) *
+                         scale);
 }
 
 }  // namespace ctprj
@@ -245,12 +272,14 @@ struct ctprj_wavelet {
     double *d_x = nullptr, *d_y = nullptr;                  // transform ping-pong
     double *d_a = nullptr, *d_s = nullptr, *d_nu = nullptr, *d_pre = nullptr, *d_pa = nullptr;
     double* d_part = nullptr;
+    float* d_mask = nullptr;   // image support (NULL: every pixel)
+    float* d_cmask = nullptr;  // coefficient support (NULL: every coefficient)
     long long blocks = 0;
     std::vector<double> part;
     ~ctprj_wavelet()
     {
         for (void* p : {(void*)d_x, (void*)d_y, (void*)d_a, (void*)d_s, (void*)d_nu,
-                        (void*)d_pre, (void*)d_pa, (void*)d_part})
+                        (void*)d_pre, (void*)d_pa, (void*)d_part, (void*)d_mask, (void*)d_cmask})
             if (p) (void)hipFree(p);
     }
 };
@@ -364,6 +393,24 @@ extern "C" int ctprj_wavelet_create(int rows, int cols, int batch, const double*
 
 long long ctprj::wavelet_elems(const ctprj_wavelet* w) { return w ? w->n : 0; }
 
+extern "C" int ctprj_wavelet_set_masks(ctprj_wavelet* w, const float* d_mask,
+                                       const float* d_cmask)
+{
+    if (!w) return fail(CTPRJ_EINVAL, "wavelet_set_masks: NULL context");
+    const size_t bytes = sizeof(float) * (size_t)w->n;
+    float** dst[2] = {&w->d_mask, &w->d_cmask};
+    const float* src[2] = {d_mask, d_cmask};
+    for (int k = 0; k < 2; ++k) {
+        if (*dst[k]) (void)hipFree(*dst[k]);
+        *dst[k] = nullptr;
+        if (!src[k]) continue;
+        hipError_t e = hipMalloc((void**)dst[k], bytes);
+        if (e == hipSuccess) e = hipMemcpy(*dst[k], src[k], bytes, hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) return hip_fail(e, "wavelet masks");
+    }
+    return 0;
+}
+
 extern "C" int ctprj_wavelet_destroy(ctprj_wavelet* w)
 {
     delete w;
@@ -374,10 +421,12 @@ extern "C" int ctprj_dwt(ctprj_wavelet* w, const float* d_x, float* d_y, void* s
 {
     if (!w || !d_x || !d_y) return fail(CTPRJ_EINVAL, "dwt: NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_x, w->d_x, w->n, 1.0);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_x, w->d_x, w->n, 1.0,
+                       w->d_mask);
     int rc = dwt64(w, w->d_x, w->d_y, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, d_y, w->n);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, d_y, w->n,
+                       w->d_cmask);
     return last("dwt output");
 }
 
@@ -385,10 +434,12 @@ extern "C" int ctprj_idwt(ctprj_wavelet* w, const float* d_y, float* d_x, void* 
 {
     if (!w || !d_x || !d_y) return fail(CTPRJ_EINVAL, "idwt: NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_y, w->d_x, w->n, 1.0);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_y, w->d_x, w->n, 1.0,
+                       w->d_cmask);
     int rc = idwt64(w, w->d_x, w->d_y, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, d_x, w->n);
+    hipLaunchKernelGGL(k_f64_to_f32, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, d_x, w->n,
+                       w->d_mask);
     return last("idwt output");
 }
 
@@ -396,10 +447,12 @@ extern "C" int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, 
 {
     if (!w || !d_x || !l1) return fail(CTPRJ_EINVAL, "wavelet_l1: NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_x, w->d_x, w->n, 1.0);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_x, w->d_x, w->n, 1.0,
+                       w->d_mask);
     int rc = dwt64(w, w->d_x, w->d_y, st);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_norms, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, w->n, w->d_part);
+    hipLaunchKernelGGL(k_norms, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_x, w->n, w->d_cmask,
+                       w->d_part);
     if ((rc = last("wavelet l1 launch"))) return rc;
     double sq;
     return fetch2(w, st, l1, &sq);
@@ -418,8 +471,8 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
     relTol = std::min(1e-3, relTol);  // npg/admm.m:14
     int rc;
     // scale = ||a||  (npg/admm.m:17-18)
-    hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(kWvBlock), 0, st, d_a, w->d_a, n, 1.0);
-    hipLaunchKernelGGL(k_norms, dim3(g), dim3(kWvBlock), 0, st, w->d_a, n, w->d_part);
+    hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(kWvBlock), 0, st, d_a, w->d_a, n, 1.0, w->d_mask);
+    hipLaunchKernelGGL(k_norms, dim3(g), dim3(kWvBlock), 0, st, w->d_a, n, nullptr, w->d_part);
     if ((rc = last("admm scale launch"))) return rc;
     double l1, sq;
     if ((rc = fetch2(w, st, &l1, &sq))) return rc;
@@ -434,8 +487,11 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
     hipLaunchKernelGGL(k_scale64, dim3(g), dim3(kWvBlock), 0, st, w->d_a, inv, n);
     // preS = s = Psit(init / scale), nu = 0
     hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(kWvBlock), 0, st, d_init ? d_init : d_a,
-                       w->d_pre, n, inv);
+                       w->d_pre, n, inv, w->d_mask);
     if ((rc = dwt64(w, w->d_pre, w->d_y, st))) return rc;
+    if (w->d_cmask)
+        hipLaunchKernelGGL(k_f64_to_f64_masked, dim3(g), dim3(kWvBlock), 0, st, w->d_pre, n,
+                           w->d_cmask);
     hipError_t e = hipMemcpyAsync(w->d_s, w->d_pre, sizeof(double) * (size_t)n,
                                   hipMemcpyDeviceToDevice, st);
     if (e == hipSuccess) e = hipMemsetAsync(w->d_nu, 0, sizeof(double) * (size_t)n, st);
@@ -449,11 +505,11 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
         hipLaunchKernelGGL(k_admm_sum, dim3(g), dim3(kWvBlock), 0, st, w->d_s, w->d_nu, w->d_x, n);
         if ((rc = idwt64(w, w->d_x, w->d_y, st))) return rc;
         hipLaunchKernelGGL(k_admm_alpha, dim3(g), dim3(kWvBlock), 0, st, w->d_a, w->d_x, rho,
-                           w->d_pa, n);
+                           w->d_pa, n, w->d_mask);
         // Psit(alpha) in d_pa
         if ((rc = dwt64(w, w->d_pa, w->d_y, st))) return rc;
         hipLaunchKernelGGL(k_admm_shrink, dim3(g), dim3(kWvBlock), 0, st, w->d_pa, w->d_s, w->d_nu,
-                           w->d_pre, u / rho, n, w->d_part);
+                           w->d_pre, u / rho, n, w->d_cmask, w->d_part);
         if ((rc = last("admm launch"))) return rc;
         double ds2, rs2;
         if ((rc = fetch2(w, st, &ds2, &rs2))) return rc;
@@ -475,7 +531,7 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
     hipLaunchKernelGGL(k_admm_sum, dim3(g), dim3(kWvBlock), 0, st, w->d_s, w->d_nu, w->d_x, n);
     if ((rc = idwt64(w, w->d_x, w->d_y, st))) return rc;
     hipLaunchKernelGGL(k_admm_final, dim3(g), dim3(kWvBlock), 0, st, w->d_a, w->d_x, rho, scale,
-                       d_out, n);
+                       d_out, n, w->d_mask);
     if (iterations) *iterations = itr;
     return last("admm output");
 }

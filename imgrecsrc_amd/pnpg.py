@@ -64,6 +64,19 @@ class PNPG:
         check(lib().ctprj_pnpg_set_method(self._h, 1 if method.upper() == "NPG" else 0,
                                           int(preSteps)))
 
+    def set_mask(self, mask=None):
+        """Support of alpha (opt.mask): pixels outside stay zero; None = no mask.  Give a
+        wavelet or TV prior the same mask."""
+        if mask is None:
+            check(lib().ctprj_pnpg_set_mask(self._h, None))
+            return
+        m = mask if isinstance(mask, torch.Tensor) else torch.from_numpy(np.asarray(mask))
+        m = (m != 0).to(device="cuda", dtype=torch.float32).reshape(-1).contiguous()
+        if m.numel() != self.proj.img_elems:
+            raise ValueError("mask must have the plan's image size")
+        self._mask = m
+        check(lib().ctprj_pnpg_set_mask(self._h, _ptr(m)))
+
     def set_wavelet(self, wavelet, u: float):
         """Wavelet l1 prox u ||mdwt(alpha)||_1 (opt.proximal = 'wvltADMM'); ``wavelet`` is an
         imgrecsrc_amd.wavelet.Wavelet over the plan's images (None: non-negativity only).

@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from ._lib import check, lib
@@ -50,6 +51,15 @@ class TV:
         if t.numel() != self.elems:
             raise ValueError(f"{what} has {t.numel()} elements, expected {self.elems}")
         return t.contiguous()
+
+    def set_mask(self, mask=None):
+        """Support (opt.mask of TV.denoise); nonzero = inside; None = no mask."""
+        if mask is None:
+            check(lib().ctprj_tv_set_mask(self.handle, None))
+            return
+        m = mask if isinstance(mask, torch.Tensor) else torch.from_numpy(np.asarray(mask))
+        m = self._check((m != 0).to(device="cuda", dtype=torch.float32).reshape(-1), "mask")
+        check(lib().ctprj_tv_set_mask(self.handle, _ptr(m)))
 
     def tlv(self, x: torch.Tensor) -> float:
         v = ctypes.c_double()

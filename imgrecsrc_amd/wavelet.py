@@ -10,6 +10,8 @@ Mirrors of the reference's MATLAB contract, computed by libctprj:
   argmin 0.5||x - a||^2 + I(x >= 0) + u ||mdwt(x)||_1
 * ``Wavelet.penalty(x)``: ||mdwt(x)||_1 (the wvltADMM penalty of
   bhc/beamhardenSpline.m:208)
+* ``Wavelet.set_masks(mask, coef_mask)`` and ``wvlt_mask`` (utils/wvltMask.m): the
+  supports of ConfigCT's Psi / Psit
 
 Images are float32 CUDA tensors (a stack of ``batch`` images of rows x cols, row-major
 with the MATLAB row index as row); the transforms and the ADMM state are FP64 inside.
@@ -59,6 +61,42 @@ def daubcqf(N: int, TYPE: str = "min"):
     return h0, h1
 
 
+def _as_f32(x, device) -> torch.Tensor:
+    if isinstance(x, torch.Tensor):
+        return (x != 0).to(device=device, dtype=torch.float32).reshape(-1)
+    return torch.from_numpy((np.asarray(x) != 0).astype(np.float32).ravel()).to(device)
+
+
+def wvlt_mask(mask, h, L) -> np.ndarray:
+    """The coefficients that depend on a pixel of ``mask`` (utils/wvltMask.m: the union
+    of the supports of mdwt(e_i) over the mask's pixels), from one transform of the
+    mask with every tap replaced by its magnitude, so that no contribution cancels (the
+    reference's test `Wt(e_i) ~= 0` differs only where contributions cancel exactly)."""
+    m = (np.asarray(mask) != 0).astype(np.float64)
+    R, C = m.shape
+    a = np.abs(np.asarray(h, np.float64).ravel())
+    lh = a.size
+    lo, hi = a, a[::-1]  # tap magnitudes of the lowpass and the highpass analysis filters
+
+    def analysis(x):
+        n = x.shape[1]
+        p2 = np.arange(0, n, 2)
+        al = np.zeros((x.shape[0], n // 2))
+        ah = np.zeros((x.shape[0], n // 2))
+        for j in range(lh):
+            col = x[:, (p2 + j) % n]
+            al = al + col * lo[j]
+            ah = ah + col * hi[j]
+        return np.concatenate([al, ah], axis=1)
+
+    y = m.copy()
+    for lev in range(L):
+        bm, bn = R >> lev, C >> lev
+        y[:bm, :bn] = analysis(y[:bm, :bn])
+        y[:bm, :bn] = analysis(y[:bm, :bn].T).T
+    return y > 0
+
+
 class Wavelet:
     """Device wavelet context: rows x cols images (both divisible by 2^L), ``batch``
     images per call, scaling filter h (even length <= 32), L levels."""
@@ -93,6 +131,16 @@ class Wavelet:
             self.close()
         except Exception:
             pass
+
+    def set_masks(self, mask=None, coef_mask=None):
+        """Supports of Psi / Psit (tomography/ConfigCT.m:85-86): ``mask`` the image pixels
+        (opt.mask), ``coef_mask`` the coefficients (maskk, see ``wvlt_mask``); arrays of
+        the context's size, nonzero = inside; None = no mask."""
+        m = None if mask is None else self._check(_as_f32(mask, self.device), "mask")
+        c = None if coef_mask is None else self._check(_as_f32(coef_mask, self.device),
+                                                       "coef_mask")
+        check(lib().ctprj_wavelet_set_masks(self.handle, _ptr(m) if m is not None else None,
+                                            _ptr(c) if c is not None else None))
 
     def _check(self, t: torch.Tensor, what: str) -> torch.Tensor:
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda" or t.dtype != torch.float32:
@@ -131,4 +179,4 @@ class Wavelet:
         return out, itr.value
 
 
-__all__ = ["daubcqf", "Wavelet"]
+__all__ = ["daubcqf", "wvlt_mask", "Wavelet"]

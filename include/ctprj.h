@@ -264,6 +264,13 @@ typedef struct ctprj_wavelet ctprj_wavelet;
 int ctprj_wavelet_create(int rows, int cols, int batch, const double* h, int lh, int L,
                          ctprj_wavelet** out);
 int ctprj_wavelet_destroy(ctprj_wavelet* w);
+/* Support masks of Psi / Psit (tomography/ConfigCT.m:85-86, utils/maskFunc.m): d_mask
+ * (image pixels, the reconstruction support opt.mask) and d_cmask (coefficients, maskk of
+ * utils/wvltMask.m), float device arrays of the context's size, 0 = outside; NULL = no
+ * mask.  Copied.  Afterwards ctprj_dwt reads masked pixels and returns masked
+ * coefficients, ctprj_idwt the reverse, and the l1 value and the ADMM work on the masked
+ * vectors exactly as the reference's closures. */
+int ctprj_wavelet_set_masks(ctprj_wavelet* w, const float* d_mask, const float* d_cmask);
 /* y = mdwt(x, h, L) (rwt/mdwt_r.c MDWT) of each image: periodised two-channel analysis,
  * rows then columns, L levels on the top-left block; lowpass first.  x may equal y. */
 int ctprj_dwt(ctprj_wavelet* w, const float* d_x, float* d_y, void* stream);
@@ -290,6 +297,9 @@ int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u, double
 typedef struct ctprj_tv ctprj_tv;
 int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv** out);
 int ctprj_tv_destroy(ctprj_tv* w);
+/* Support mask (opt.mask of TV.denoise, utils/TV.m:68-76): pixels outside it (0) are zero
+ * in the input, the output and the TV value; NULL = no mask.  Copied. */
+int ctprj_tv_set_mask(ctprj_tv* w, const float* d_mask);
 /* *tv = TV(x) summed over the images (utils/tlv.m; synchronises `stream`). */
 int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* stream);
 /* x = argmin 0.5||x - a||^2 + lambda TV(x) over x >= 0 by FGP on the dual
@@ -341,6 +351,11 @@ int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
  * trial that retries recomputes the same values), the adaptive increase t/4, and in
  * the first preSteps iterations (opt.preSteps = 2) the step grows while the test holds. */
 int ctprj_pnpg_set_method(ctprj_pnpg* solver, int npg, int preSteps);
+/* Support of alpha (the reference's alpha is the vector of opt.mask's pixels): pixels
+ * outside d_mask (0) stay zero (xbar and the gradient are masked); NULL = no mask.
+ * Copied.  A wavelet or TV prior should carry the same mask (ctprj_wavelet_set_masks,
+ * ctprj_tv_set_mask). */
+int ctprj_pnpg_set_mask(ctprj_pnpg* solver, const float* d_mask);
 /* Switches the prox to the wavelet l1 penalty u ||mdwt(alpha)||_1 with non-negativity
  * (opt.proximal = 'wvltADMM', bhc/beamhardenSpline.m:204-209), solved per trial by
  * ctprj_prox_wavelet_admm with tolerance admmTol * difAlpha and the reference's

@@ -16,8 +16,10 @@ import numpy as np
 
 
 class PnpgOracle:
-    def __init__(self, func, alpha0, t0, npg=False, pre_steps=2):
-        self.func = func
+    def __init__(self, func, alpha0, t0, npg=False, pre_steps=2, mask=None):
+        """mask: support of alpha (flattened bool), pixels outside stay zero."""
+        self.mask = None if mask is None else np.asarray(mask, bool).ravel()
+        self.func = func if mask is None else self._masked(func)
         self.alpha = np.array(alpha0, np.float64)
         self.pre = self.alpha.copy()
         self.t = float(t0)
@@ -36,6 +38,12 @@ class PnpgOracle:
         self.newTheta = 1.0
         self.coef = 0.0
         self.log = []
+
+    def _masked(self, func):
+        def f(x, grad=True):
+            v, g = func(x, grad)
+            return v, (g * self.mask if g is not None else None)
+        return f
 
     def _begin(self):
         """PNPG.m:56-77 / NPG.m:48-66: theta, the adaptive step, xbar."""
@@ -59,7 +67,8 @@ class PnpgOracle:
             B = self.t / self.preT
             self.newTheta = 1.0 if self.p == 0 else 0.5 + np.sqrt(0.25 + B * self.theta ** 2)
             self.coef = (self.theta - 1.0) / self.newTheta
-        return np.maximum(self.alpha + self.coef * (self.alpha - self.pre), 0.0)
+        xbar = np.maximum(self.alpha + self.coef * (self.alpha - self.pre), 0.0)
+        return xbar if self.mask is None else xbar * self.mask
 
     def _linesearch(self, holds, f_new):
         """0: another trial follows, 1: the test holds, 2: given up (goodMM false)."""
@@ -116,10 +125,15 @@ class PnpgWaveletOracle(PnpgOracle):
     PNPG.m:113-146.  A failed line search sets theta = 1 (reset() without re-estimating
     t).  ``h``, ``L``: the wavelet filter and levels."""
 
-    def __init__(self, func, alpha0, t0, u, h, L, shape, npg=False, pre_steps=2):
-        super().__init__(func, alpha0, t0, npg, pre_steps)
+    def __init__(self, func, alpha0, t0, u, h, L, shape, npg=False, pre_steps=2, mask=None,
+                 coef_mask=None):
+        super().__init__(func, alpha0, t0, npg, pre_steps, mask)
         from oracle import wavelet_oracle as W
         self.W = W
+        self.ops = None
+        if mask is not None:  # Psi / Psit on the supports' vectors (ConfigCT.m:85-86)
+            ck = np.ones(shape, bool) if coef_mask is None else np.asarray(coef_mask, bool)
+            self.ops = W.masked_ops(h, L, np.asarray(mask, bool).reshape(shape), ck)
         self.u, self.h, self.L, self.shape = float(u), h, L, shape
         self.admmTol = 1e-2
         self.maxInnerItr = 100
@@ -128,9 +142,17 @@ class PnpgWaveletOracle(PnpgOracle):
         self.innerSearch = 0
 
     def _pen(self, x):
+        if self.ops is not None:
+            return float(np.abs(self.ops[1](x[self.mask])).sum())
         return self.W.l1_penalty(x.reshape(self.shape), self.h, self.L)
 
     def _prox(self, a, u, tol):
+        if self.ops is not None:
+            v, it = self.W.admm(a.ravel()[self.mask], u, self.h, self.L, tol, self.maxInnerItr,
+                                self.alpha[self.mask], ops=self.ops)
+            x = np.zeros(self.mask.size)
+            x[self.mask] = v
+            return x.reshape(self.shape), it
         return self.W.admm(a, u, self.h, self.L, tol, self.maxInnerItr,
                            self.alpha.reshape(self.shape))
 

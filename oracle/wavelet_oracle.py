@@ -8,6 +8,8 @@ Follows, line by line:
   * ``admm``  -- npg/admm.m:1-57 (ADMM for 0.5||x - a||^2 + I(x >= 0) + u||W x||_1 with
                  the adaptive penalty rho), with Utils.softThresh (utils/Utils.m:13-20)
   * ``l1_penalty`` -- pNorm(Psit(x), 1) (bhc/beamhardenSpline.m:208, utils/pNorm.m)
+  * ``masked_ops`` -- Psi / Psit with supports (tomography/ConfigCT.m:85-86,
+                      utils/maskFunc.m); ``wvlt_mask`` -- utils/wvltMask.m
 
 Images are 2-D arrays whose row index is the MATLAB row index (the C row-major layout of
 the same matrix), so ``mdwt(x)`` here equals ``mdwt(x)`` of the MATLAB matrix.
@@ -108,8 +110,38 @@ def l1_penalty(x, h, L):
     return float(np.abs(mdwt(x, h, L)).sum())
 
 
-def admm(a, u, h, L, relative_tol=1e-6, max_itr=1000, init=None):
-    """npg/admm.m: argmin 0.5||x-a||^2 + I(x>=0) + u||mdwt(x)||_1; returns (x, iterations)."""
+def masked_ops(h, L, mask, maskk):
+    """Psi / Psit of tomography/ConfigCT.m:85-86 on the vectors of the supports mask
+    (pixels) and maskk (coefficients), with utils/maskFunc.m's embedding."""
+    mask = np.asarray(mask, bool)
+    maskk = np.asarray(maskk, bool)
+
+    def embed(v, m):
+        x = np.zeros(m.shape)
+        x[m] = v
+        return x
+
+    Psi = lambda s: midwt(embed(s, maskk), h, L)[mask]   # noqa: E731
+    Psit = lambda x: mdwt(embed(x, mask), h, L)[maskk]   # noqa: E731
+    return Psi, Psit
+
+
+def wvlt_mask(mask, h, L):
+    """utils/wvltMask.m by its definition: the union over the mask's pixels i of the
+    nonzero coefficients of mdwt(e_i)."""
+    mask = np.asarray(mask) != 0
+    out = np.zeros(mask.shape, bool)
+    e = np.zeros(mask.shape)
+    for i in zip(*np.nonzero(mask)):
+        e[i] = 1.0
+        out |= mdwt(e, h, L) != 0
+        e[i] = 0.0
+    return out
+
+
+def admm(a, u, h, L, relative_tol=1e-6, max_itr=1000, init=None, ops=None):
+    """npg/admm.m: argmin 0.5||x-a||^2 + I(x>=0) + u||Psit(x)||_1; returns (x, iterations).
+    ops = (Psi, Psit) (e.g. masked_ops), default the full transforms."""
     a = np.asarray(a, np.float64)
     init = a if init is None else np.asarray(init, np.float64)
     relative_tol = min(1e-3, relative_tol)
@@ -119,8 +151,11 @@ def admm(a, u, h, L, relative_tol=1e-6, max_itr=1000, init=None):
     init = init / scale
     a = a / scale
     u = u / scale
-    Psi = lambda s: midwt(s, h, L)   # noqa: E731
-    Psit = lambda x: mdwt(x, h, L)   # noqa: E731
+    if ops is None:
+        Psi = lambda s: midwt(s, h, L)   # noqa: E731
+        Psit = lambda x: mdwt(x, h, L)   # noqa: E731
+    else:
+        Psi, Psit = ops
     nu = 0.0
     rho = 1.0
     cnt = 0

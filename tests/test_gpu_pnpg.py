@@ -178,3 +178,70 @@ def test_pnpg_wavelet_prox_matches_fp64_restatement(oracle_mod, method):
     got = alpha.cpu().numpy().astype(np.float64)
     assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()
     assert 0 < st["innerSearch"] <= st["maxInnerItr"]
+
+
+def test_pnpg_masked_wavelet_prior_matches_restatement(oracle_mod):
+    """alpha restricted to a disc support (opt.mask) with the wavelet prior on the
+    supports (tomography/ConfigCT.m:85-86): pixels outside stay zero, the trials match
+    the restatement on the supports' vectors."""
+    import torch
+    from oracle import bhc_oracle as B
+    from oracle.pnpg_oracle import PnpgWaveletOracle
+    from imgrecsrc_amd.phantom import ellipse_params, rasterise
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
+    geo = (64, 91, 72, 72, 1.0, 1.0, 250.0)
+    n = geo[0]
+    c = np.arange(n) - (n - 1) / 2
+    X, Y = np.meshgrid(c, c)
+    mask = X ** 2 + Y ** 2 <= 30 ** 2
+    knots = B.spline_knots(B.kappa_logspace(E, 0.05, 20.0))
+    Ie = np.exp(-0.5 * ((np.arange(E) - 3.0) / 1.5) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    cf = oracle_mod.conf(*geo)
+    truth = rasterise(ellipse_params(81, n), n) * mask
+    raw = oracle_mod.forward(cf, truth)[0].ravel()
+    Ts = 2.0 / raw.max()
+    BL, _, _ = B.b1_iout_stable(knots, Ts * raw, Ie)
+    counts = np.random.default_rng(83).poisson(1e5 * BL)
+    Imea = counts / counts.max()
+
+    def func(x, grad=True):
+        s = oracle_mod.forward(cf, x.reshape(n, n))[0].ravel()
+        BLx, sBL, _ = B.b1_iout_stable(knots, Ts * s, Ie)
+        f = B._nll(BLx, Imea)
+        if not grad:
+            return f, None
+        w = Ts * (1 - Imea / BLx) * sBL
+        return f, oracle_mod.adjoint(cf, w.reshape(geo[2], geo[1]))[0].ravel()
+
+    h = daubcqf(4)[0]
+    L = 3
+    mk = wvlt_mask(mask, h, L)
+    proj = Projector([geo])
+    alpha0 = np.where(mask, 0.05, 0.0).ravel()
+    alpha = torch.from_numpy(alpha0.astype(np.float32)).cuda()
+    solver = PNPG(proj, torch.from_numpy(Imea.astype(np.float32)).cuda(), knots, Ie, Ts, alpha)
+    u = 2e-3 * solver.t0
+    wv = Wavelet(n, n, h, L)
+    wv.set_masks(mask, mk)
+    solver.set_mask(mask)
+    solver.set_wavelet(wv, u)
+    ref = PnpgWaveletOracle(func, alpha0, solver.t0, u, h, L, (n, n), mask=mask.ravel(),
+                            coef_mask=mk)
+    log = []
+    for k in range(10):
+        solver.step(alpha)
+        st = solver.stats()
+        ref.trial()
+        log.append((ref.log[-1][0], st["last_accepted"], st["cost"], ref.cost, st["innerSearch"],
+                    ref.innerSearch))
+        assert (st["last_accepted"] == 1) == (ref.log[-1][0] == "accept"), log
+        if np.isfinite(ref.cost):
+            assert abs(st["cost"] - ref.cost) <= 1e-5 * abs(ref.cost), log
+    print("\n".join(map(str, log)))
+    got = alpha.cpu().numpy().astype(np.float64)
+    assert not got[~mask.ravel()].any()
+    assert st["iterations"] == ref.p and st["iterations"] >= 5
+    assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()

@@ -82,3 +82,25 @@ def test_pnpg_tv_prox_matches_fp64_restatement(oracle_mod):
     assert st["iterations"] == ref.p and st["iterations"] >= 5
     got = alpha.cpu().numpy().astype(np.float64)
     assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()
+
+
+def test_masked_denoise_matches_oracle():
+    """TV.denoise with opt.mask (utils/TV.m:68-79): the masked vector embedded in a zero
+    image, denoised, and read back on the mask."""
+    import torch
+    from imgrecsrc_amd.tv import TV
+    n = 40
+    c = np.arange(n) - (n - 1) / 2
+    X, Y = np.meshgrid(c, c)
+    m = X ** 2 + Y ** 2 <= 16 ** 2
+    x = (np.random.default_rng(8).standard_normal((n, n)) + 0.5).astype(np.float32)
+    w = TV(n, n, tv="l1")
+    w.set_mask(m)
+    out, itr = w.denoise(torch.from_numpy(x).cuda().reshape(-1), 0.4, 1e-6, 500)
+    ref, itr_ref = T.denoise(np.where(m, x, 0).astype(np.float64), 0.4, "l1", 1e-6, 500)
+    got = out.cpu().numpy().reshape(n, n)
+    assert itr == itr_ref
+    assert not got[~m].any()
+    assert np.abs(got[m] - ref[m]).max() <= 1e-6 * np.abs(ref).max()
+    assert w.tlv(torch.from_numpy(x).cuda().reshape(-1)) == pytest.approx(
+        T.tlv(np.where(m, x, 0).astype(np.float64), "l1"), rel=1e-12)

@@ -87,3 +87,38 @@ def test_admm_zero_input_and_bad_arguments():
         Wavelet(12, 12, [0.5, 0.5], 3)  # 12 not divisible by 8
     with pytest.raises(CtprjError):
         Wavelet(16, 16, [1.0, 0.5, 0.2], 2)  # odd filter length
+
+
+def _disc(n, r):
+    c = np.arange(n) - (n - 1) / 2
+    X, Y = np.meshgrid(c, c)
+    return X ** 2 + Y ** 2 <= r * r
+
+
+def test_masked_admm_matches_oracle_vectors():
+    """Psi / Psit with the supports opt.mask and maskk (tomography/ConfigCT.m:85-86): the
+    device works on whole images with zeros outside, the restatement on the supports'
+    vectors as the reference does."""
+    import torch
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
+    h = daubcqf(6)[0]
+    L, n = 3, 64
+    m = _disc(n, 26)
+    mk = wvlt_mask(m, h, L)
+    w = Wavelet(n, n, h, L)
+    w.set_masks(m, mk)
+    a = np.random.default_rng(5).standard_normal((n, n)).astype(np.float32)
+    out, itr = w.admm(torch.from_numpy(a).cuda().reshape(-1), 1.5, 1e-7, 3000)
+    ops = W.masked_ops(h, L, m, mk)
+    ref, itr_ref = W.admm(a.astype(np.float64)[m], 1.5, h, L, 1e-7, 3000, ops=ops)
+    got = out.cpu().numpy().reshape(n, n)
+    print(f"masked ADMM: iterations {itr} (oracle {itr_ref})")
+    assert not got[~m].any()
+    assert np.abs(got[m] - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert abs(itr - itr_ref) <= max(2, itr_ref // 50)
+    x = torch.from_numpy(np.where(m, a, 0).astype(np.float32)).cuda().reshape(-1)
+    assert w.penalty(x) == pytest.approx(np.abs(ops[1](a.astype(np.float64)[m])).sum(), rel=1e-9)
+    # Psit returns the masked coefficients, Psi the masked image
+    c = w.mdwt(x).cpu().numpy().reshape(n, n)
+    assert not c[~mk].any()
+    assert not w.midwt(torch.from_numpy(c).cuda().reshape(-1)).cpu().numpy().reshape(n, n)[~m].any()

@@ -113,3 +113,34 @@ def test_admm_satisfies_the_prox_optimality_conditions():
 def test_admm_of_zero_is_zero():
     x, itr = W.admm(np.zeros((8, 8)), 1.0, daubcqf(2)[0], 2)
     assert itr == 0 and not x.any()
+
+
+def _disc(n, r):
+    c = np.arange(n) - (n - 1) / 2
+    X, Y = np.meshgrid(c, c)
+    return X ** 2 + Y ** 2 <= r * r
+
+
+@pytest.mark.parametrize("N,L,n", [(2, 2, 16), (4, 3, 32), (6, 2, 16)])
+def test_wvlt_mask_host_helper_matches_the_definition(N, L, n):
+    from imgrecsrc_amd.wavelet import wvlt_mask
+    h = daubcqf(N)[0]
+    m = _disc(n, n / 3)
+    np.testing.assert_array_equal(wvlt_mask(m, h, L), W.wvlt_mask(m, h, L))
+
+
+def test_masked_admm_is_the_prox_on_the_support():
+    # the masked ADMM works on the disc's vector and satisfies the prox optimality there
+    h = daubcqf(4)[0]
+    L, n = 2, 16
+    m = _disc(n, 6)
+    mk = W.wvlt_mask(m, h, L)
+    a = (10.0 + np.random.default_rng(2).standard_normal((n, n)))[m]
+    ops = W.masked_ops(h, L, m, mk)
+    x, itr = W.admm(a, 0.3, h, L, 1e-11, 20000, ops=ops)
+    assert x.shape == a.shape and itr > 1
+    f = lambda z: 0.5 * ((z - a) ** 2).sum() + 0.3 * np.abs(ops[1](z)).sum()  # noqa: E731
+    rng = np.random.default_rng(3)
+    for _ in range(20):
+        z = np.maximum(x + 1e-4 * rng.standard_normal(x.shape), 0)
+        assert f(z) >= f(x) - 1e-9
