@@ -391,6 +391,40 @@ def main():
             "gen_beam_harden_trials": int(round(np.log(sim["Ts"]) / np.log(0.8))) + 1,
             "gen_beam_harden_energies": int(eps_s.size)})
 
+        # ---- the reference's default alpha step: NPG with the wavelet-ADMM prior (Haar,
+        #      6 levels) on a circle support, same 512^2 fan problem; trials synchronise
+        log("npg + wavelet leg")
+        from imgrecsrc_amd.pnpg import PNPG
+        from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
+        nb = bg[0]
+        cc = np.arange(nb) - (nb - 1) / 2
+        XX, YY = np.meshgrid(cc, cc)
+        disc = XX ** 2 + YY ** 2 <= (nb / 2 - 1) ** 2
+        hw = daubcqf(2)[0]
+        wv = Wavelet(nb, nb, hw, 6)
+        wv.set_masks(disc, wvlt_mask(disc, hw, 6))
+        a_w = (alpha * torch.from_numpy(disc.ravel().astype(np.float32)).to(dev)).contiguous()
+        npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
+        npg.set_method("NPG")
+        npg.set_mask(disc)
+        npg.set_wavelet(wv, 1e-3 * npg.t0)
+        ntr, inner = 10, 0
+        npg.step(a_w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ntr):
+            npg.step(a_w)
+            inner += npg.stats()["innerSearch"]
+        torch.cuda.synchronize()
+        t_npg = (time.perf_counter() - t0) * 1e3 / ntr
+        st = npg.stats()
+        out["npg_wavelet"] = {
+            "workload": "NPG + wavelet-ADMM prior (daubcqf(2), 6 levels, circle support), "
+                        "fan 512^2, E=20",
+            "ms_per_trial": round(t_npg, 3), "admm_iterations_per_trial": inner / ntr,
+            "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
+        del npg, wv
+
         # ---- 64 ragged slices: one launch per operator vs one launch per slice
         log("batched leg")
         geoms = ragged_geoms()
