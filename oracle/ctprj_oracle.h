@@ -16,8 +16,8 @@
  *   - FBP: fan cosine pre-weight prj/cpuPrj.c:733-751, Hann-windowed ramp filter
  *     prj/utils.h:94-151, backprojection scale prj/cpuPrj.c:161-163,258,304-306.
  *
- * Parity pins (tests/test_oracle_*.py): the reference's own cpuPrj compiled from
- * /root/reference into oracle/_ref (golden fixtures), exact polygon-clipping areas of
+ * Parity pins (tests/test_oracle_*.py): the reference's own cpuPrj compiled from its
+ * sources into oracle/_ref (golden fixtures), exact polygon-clipping areas of
  * the pixel/strip overlap, analytic ellipse line integrals, brute-force dense A on
  * tiny images, and <Ax,y> = <x,A^T y>.
  */

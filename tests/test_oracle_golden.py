@@ -1,5 +1,5 @@
 """Pins the FP64 oracle against the reference's own cpuPrj (golden fixtures written by
-tests/golden/make_golden.py from prj/cpuPrj.c compiled out of /root/reference).
+tests/golden/make_golden.py from the reference's prj/cpuPrj.c, compiled by oracle/Makefile).
 
 The reference is FP32; the oracle FP64.  Bars: forward / adjoint of smooth data 1e-6
 relative L2; adjoint of Gaussian noise 5e-6 relative to the magnitude of the summed
