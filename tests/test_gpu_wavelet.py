@@ -10,8 +10,11 @@ from oracle import wavelet_oracle as W
 
 pytestmark = pytest.mark.gpu
 
+# levels whose block holds at most 4096 coefficients run in one LDS-resident work-group;
+# 128^2, 256 x 64 and 256 x 128 start with one launch per level; 8 x 4096 is wider than
+# the one-launch level kernels and runs two passes per level
 CASES = [(2, 3, 16, 16), (4, 3, 32, 16), (6, 4, 64, 64), (8, 2, 8, 12), (20, 2, 16, 16),
-         (6, 6, 128, 128)]
+         (6, 6, 128, 128), (4, 5, 256, 64), (2, 7, 256, 128), (4, 2, 8, 4096)]
 
 
 def _wv(N, L, R, C, batch=1):
@@ -37,20 +40,25 @@ def test_dwt_idwt_bitwise(N, L, R, C):
     np.testing.assert_array_equal(xt.cpu().numpy().reshape(R, C), y)
 
 
-def test_batched_images_transform_independently():
+@pytest.mark.parametrize("n,L", [(32, 3), (128, 4)])
+def test_batched_images_transform_independently(n, L):
     import torch
-    w, h = _wv(4, 3, 32, 32, batch=3)
-    x = np.random.default_rng(9).standard_normal((3, 32, 32)).astype(np.float32)
-    y = w.mdwt(torch.from_numpy(x).cuda().reshape(-1)).cpu().numpy().reshape(3, 32, 32)
+    w, h = _wv(4, L, n, n, batch=3)
+    x = np.random.default_rng(9).standard_normal((3, n, n)).astype(np.float32)
+    y = w.mdwt(torch.from_numpy(x).cuda().reshape(-1)).cpu().numpy().reshape(3, n, n)
     for b in range(3):
-        np.testing.assert_array_equal(y[b], W.mdwt(x[b].astype(np.float64), h, 3).astype(np.float32))
+        np.testing.assert_array_equal(y[b], W.mdwt(x[b].astype(np.float64), h, L).astype(np.float32))
+    z = w.midwt(torch.from_numpy(y).cuda().reshape(-1)).cpu().numpy().reshape(3, n, n)
+    for b in range(3):
+        np.testing.assert_array_equal(z[b], W.midwt(y[b].astype(np.float64), h, L).astype(np.float32))
     assert w.penalty(torch.from_numpy(x).cuda().reshape(-1)) == pytest.approx(
-        sum(W.l1_penalty(x[b].astype(np.float64), h, 3) for b in range(3)), rel=1e-12)
+        sum(W.l1_penalty(x[b].astype(np.float64), h, L) for b in range(3)), rel=1e-12)
 
 
 @pytest.mark.parametrize("N,L,shape,u,nonneg_active",
                          [(4, 3, (32, 32), 0.5, False), (6, 3, (64, 64), 2.0, True),
-                          (2, 4, (32, 48), 1.0, True)])
+                          (2, 4, (32, 48), 1.0, True), (4, 4, (128, 128), 1.0, True),
+                          (2, 2, (8, 4096), 1.0, True)])
 def test_admm_matches_oracle(N, L, shape, u, nonneg_active):
     import torch
     w, h = _wv(N, L, *shape)
@@ -77,10 +85,17 @@ def test_admm_matches_oracle(N, L, shape, u, nonneg_active):
 
 def test_admm_zero_input_and_bad_arguments():
     import torch
-    w, _ = _wv(2, 2, 8, 8)
+    w, h = _wv(2, 2, 8, 8)
     z = torch.zeros(64, dtype=torch.float32, device="cuda")
     out, itr = w.admm(z, 1.0)
     assert itr == 0 and not torch.any(out)
+    # the loop state starts afresh on the next call
+    a = np.random.default_rng(3).standard_normal((8, 8))
+    out, itr = w.admm(torch.from_numpy(a.astype(np.float32)).cuda().reshape(-1), 0.3, 1e-7, 3000)
+    ref, itr_ref = W.admm(a.astype(np.float32).astype(np.float64), 0.3, h, 2,
+                          relative_tol=1e-7, max_itr=3000)
+    assert np.abs(out.cpu().numpy().reshape(8, 8) - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert abs(itr - itr_ref) <= 2
     from imgrecsrc_amd._lib import CtprjError
     from imgrecsrc_amd.wavelet import Wavelet
     with pytest.raises(CtprjError):
