@@ -341,6 +341,7 @@ def main():
         log("bhc leg")
         bg = BHC512
         PB = Projector([bg])
+        PB.autotune(reps=3)
         knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
         xs = np.arange(20)
         Ie = normalize_ie(np.exp(-0.5 * ((xs - 7.5) / 3.0) ** 2) + 0.05, knots)
@@ -365,7 +366,8 @@ def main():
                       "grad_alpha_ms": round(t_ga, 4),
                       "grad_ie_evals_per_s": round(1e3 / t_gi, 1), "grad_ie_ms": round(t_gi, 4),
                       "roofline_evals_per_s": round(roof_evals, 1),
-                      "frac": round((1e3 / t_ga) / roof_evals, 4), "nnz": bnnz}
+                      "frac": round((1e3 / t_ga) / roof_evals, 4), "nnz": bnnz,
+                      "tune": PB.get_tune()}
 
         # ---- I-epsilon step (20 iterations) and the measurement simulator; both block
         from imgrecsrc_amd.bhc import gen_beam_harden, ie_solve

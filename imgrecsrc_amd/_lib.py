@@ -63,7 +63,8 @@ class IeInfo(ctypes.Structure):
 
 class Tune(ctypes.Structure):
     _fields_ = [("fwd_block", ctypes.c_int), ("adj_tile", ctypes.c_int),
-                ("flags", ctypes.c_int), ("adj_seg", ctypes.c_int), ("fwd_seg", ctypes.c_int)]
+                ("flags", ctypes.c_int), ("adj_seg", ctypes.c_int), ("fwd_seg", ctypes.c_int),
+                ("adj_split", ctypes.c_int), ("fwd_split", ctypes.c_int)]
 
 
 # (name, restype, argtypes) of every entry point declared in include/ctprj.h

@@ -159,6 +159,7 @@ Plan::~Plan()
     free_ptr(d_red);
     free_ptr(d_red_out);
     free_ptr(d_sim);
+    free_ptr(d_split);
 }
 
 template <class T>
@@ -565,6 +566,8 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
     p->tune.flags = flags;
     p->tune.adj_seg = 8;
     p->tune.fwd_seg = 8;
+    p->tune.adj_split = 1;
+    p->tune.fwd_split = 1;
     std::vector<long long> kern_off(nslices), prew_off(nslices);
     std::vector<float> kern, prew;
     long long img = 0, sino = 0;
@@ -667,8 +670,9 @@ int Plan::launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool
                               d_rays, d_img, d_imgT, d_sino, st);
     for (int g = 1; g < 4; ++g)
         if (symf_count[g])
-            launch_forward_sym(g, tune.fwd_seg, symf_count[g], d_desc, d_symf_items + symf_start[g],
-                               d_views, d_rays, d_img, d_imgT, d_sino, st);
+            launch_forward_sym(g, tune.fwd_seg, tune.fwd_split, d_split, symf_count[g], d_desc,
+                               d_symf_items + symf_start[g], d_views, d_rays, d_img, d_imgT,
+                               d_sino, st);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
 }
@@ -681,10 +685,9 @@ int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool
     if (n_syma_items) {
         for (int g = 1; g < 4; ++g)
             if (syma_count[g])
-                launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4, syma_count[g],
-                                   d_desc,
-                                   d_syma_items + syma_start[g], d_views, d_rays, d_sino,
-                                   d_zero_row, d_symtab, d_img, st);
+                launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4, tune.adj_split,
+                                   d_split, syma_count[g], d_desc, d_syma_items + syma_start[g],
+                                   d_views, d_rays, d_sino, d_zero_row, d_symtab, d_img, st);
         launch_zero_border(n_border, d_desc, d_border_ids, d_img, st);
     }
     hipError_t e = hipGetLastError();
@@ -783,15 +786,47 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     if (t->flags & ~31) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
+    if (t->adj_split < 1 || t->adj_split > 4 || t->fwd_split < 1 || t->fwd_split > 4)
+        return fail(CTPRJ_EINVAL, "adj_split and fwd_split must be 1, 2, 3 or 4");
     const bool tables = (t->flags != p->tune.flags);
     const bool fwd = t->fwd_block != p->tune.fwd_block, adj = t->adj_tile != p->tune.adj_tile;
     p->tune = *t;
     int rc = 0;
-    if (tables) return p->build_tables();
-    if (fwd && (rc = p->build_fwd_items())) return rc;
-    if (adj && (rc = p->build_adj_items())) return rc;
+    if (tables) rc = p->build_tables();
+    else if (fwd && (rc = p->build_fwd_items())) return rc;
+    else if (adj && (rc = p->build_adj_items())) return rc;
+    if (rc) return rc;
+    if ((rc = p->ensure_split())) {
+        p->tune.adj_split = p->tune.fwd_split = 1;
+        return rc;
+    }
     return 0;
 }
+
+namespace ctprj {
+
+// Partial-sum space of the split symmetric launches: per group launch nitems * nsplit
+// work-groups of 512 floats; the launches of one call run in stream order and share it.
+int Plan::ensure_split()
+{
+    size_t need = 0;
+    for (int g = 1; g < 4; ++g) {
+        if (tune.fwd_split > 1) need = std::max(need, (size_t)symf_count[g] * tune.fwd_split);
+        if (tune.adj_split > 1) need = std::max(need, (size_t)syma_count[g] * tune.adj_split);
+    }
+    need *= 512;
+    if (need > kSplitMaxFloats)
+        return fail(CTPRJ_EINVAL, "split work items need %zu MiB of partial sums (at most %zu)",
+                    need * sizeof(float) >> 20, kSplitMaxFloats * sizeof(float) >> 20);
+    if (need <= split_cap) return 0;
+    free_ptr(d_split);
+    split_cap = 0;
+    int rc = dev_alloc((void**)&d_split, sizeof(float) * need, "split partial sums");
+    if (!rc) split_cap = need;
+    return rc;
+}
+
+}  // namespace ctprj
 
 static std::string tune_key(const ctprj_plan* p, const char* arch)
 {
@@ -804,9 +839,9 @@ static std::string tune_key(const ctprj_plan* p, const char* arch)
 }
 
 // Times the kernel shapes on this plan's geometry: general forward threads per group,
-// symmetric forward waves per ray run, general adjoint tile edge, symmetric adjoint waves
-// per tile and bin window width (flag bit 4), and (several slices) the work order of
-// tune flag bit 3; keeps the symmetry flags (bits 0-2).
+// symmetric forward waves and work-groups per ray run, general adjoint tile edge,
+// symmetric adjoint waves and work-groups per tile and bin window width (flag bit 4), and
+// (several slices) the work order of tune flag bit 3; keeps the symmetry flags (bits 0-2).
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -822,8 +857,10 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
             const size_t sp = line.rfind(' ');
             if (sp == std::string::npos || line.substr(0, sp) != key) continue;
             ctprj_tune t{};
-            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d,%d,%d", &t.fwd_block, &t.adj_tile,
-                            &t.flags, &t.adj_seg, &t.fwd_seg) == 5)
+            t.adj_split = t.fwd_split = 1;  // entries without the split fields
+            if (std::sscanf(line.c_str() + sp + 1, "%d,%d,%d,%d,%d,%d,%d", &t.fwd_block,
+                            &t.adj_tile, &t.flags, &t.adj_seg, &t.fwd_seg, &t.adj_split,
+                            &t.fwd_split) >= 5)
                 return ctprj_plan_set_tune(p, &t);
         }
     }
@@ -871,6 +908,33 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
         }
         return ctprj_plan_set_tune(p, &best);
     };
+    // waves per item and work-groups per item together (both cut an item's work); more
+    // than one work-group per item only while the items are too few to fill the device
+    // several times over
+    auto search_split = [&](int ctprj_tune::*seg, int ctprj_tune::*split, const int* segs,
+                            int ns, bool fwd, int nitems) -> int {
+        float best_ms = 1e30f;
+        const int max_split = nitems < 8192 ? 4 : 1;
+        for (int i = 0; i < ns; ++i)
+            for (int k = 1; k <= max_split; ++k) {
+                ctprj_tune t = best;
+                t.*seg = segs[i];
+                t.*split = k;
+                if (ctprj_plan_set_tune(p, &t)) {
+                    if (k > 1) continue;  // partial sums beyond the plan's cap
+                    return CTPRJ_EINVAL;
+                }
+                float ms = 0;
+                int rc2 = time_dir(fwd, &ms);
+                if (rc2) return rc2;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    best.*seg = segs[i];
+                    best.*split = k;
+                }
+            }
+        return ctprj_plan_set_tune(p, &best);
+    };
     const int blocks[3] = {64, 128, 256}, tiles[2] = {8, 16};
     const int fsegs[5] = {1, 2, 4, 8, 16}, asegs[4] = {1, 2, 4, 8};
     if (!rc && p->nslices > 1) {
@@ -891,9 +955,13 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
         if (!rc) rc = ctprj_plan_set_tune(p, &best);
     }
     if (!rc && p->n_fwd_items) rc = search(&ctprj_tune::fwd_block, blocks, 3, true);
-    if (!rc && p->n_symf_items) rc = search(&ctprj_tune::fwd_seg, fsegs, 5, true);
+    if (!rc && p->n_symf_items)
+        rc = search_split(&ctprj_tune::fwd_seg, &ctprj_tune::fwd_split, fsegs, 5, true,
+                          p->n_symf_items);
     if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);
-    if (!rc && p->n_syma_items) rc = search(&ctprj_tune::adj_seg, asegs, 4, false);
+    if (!rc && p->n_syma_items)
+        rc = search_split(&ctprj_tune::adj_seg, &ctprj_tune::adj_split, asegs, 4, false,
+                          p->n_syma_items);
     if (!rc && p->n_syma_items) {
         // symmetric adjoint bin windows: 4 wide, or 3 wide (flag bit 4)
         const int wins[2] = {best.flags & ~16, best.flags | 16};
@@ -908,7 +976,8 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     if (!rc && cache_path) {
         std::ofstream f(cache_path, std::ios::app);
         f << key << " " << best.fwd_block << "," << best.adj_tile << "," << best.flags << ","
-          << best.adj_seg << "," << best.fwd_seg << "\n";
+          << best.adj_seg << "," << best.fwd_seg << "," << best.adj_split << ","
+          << best.fwd_split << "\n";
     }
     return rc;
 }

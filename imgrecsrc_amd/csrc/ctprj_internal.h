@@ -14,6 +14,8 @@ namespace ctprj {
 struct FwdItem {
     int slice, view, bin0, pad;
 };
+// partial-sum space of split symmetric work items: at most 256 MiB per plan
+constexpr size_t kSplitMaxFloats = (size_t)64 << 20;
 struct AdjItem {
     int slice, row0, col0, pad;
 };
@@ -29,7 +31,7 @@ struct Plan {
     std::vector<int> slice_sym;  // per slice: symmetry group (kGrp*), 0 = general kernels
     long long img_elems = 0, sino_elems = 0;
     int max_W = 0;
-    ctprj_tune tune{64, 16, 1, 8, 8};
+    ctprj_tune tune{64, 16, 1, 8, 8, 1, 1};
 
     SliceDesc* d_desc = nullptr;
     float4* d_views = nullptr;
@@ -62,11 +64,14 @@ struct Plan {
     int bhc_basis = CTPRJ_BASIS_B1, bhc_noise = CTPRJ_NOISE_POISSON;  // spectrum model
     void* d_sim = nullptr;       // measurement simulator workspace
     size_t sim_cap = 0;          // bytes held by d_sim
+    float* d_split = nullptr;    // partial sums of split symmetric work items
+    size_t split_cap = 0;        // floats held by d_split
 
     ~Plan();
     int build_tables();          // geometry tables for the current tune.flags
     int build_fwd_items();
     int build_adj_items();
+    int ensure_split();          // d_split sized for tune.fwd_split / adj_split
     int launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count);
     int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp);
     int launch_fbp(const float* d_sino, float* d_img, hipStream_t st);
@@ -89,14 +94,17 @@ void launch_fbp_filter(int nitems, size_t shmem, const SliceDesc* desc, const in
                        const long long* prew_off, const float* sino, float* out,
                        hipStream_t st);
 void launch_scale(float* p, long long n, float s, hipStream_t st);
-// symmetric kernels (ctprj_sym.hip), one launch per symmetry group
-void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
-                        const float4* views, const Ray* rays, const float* img,
-                        const float* imgT, float* sino, hipStream_t st);
-void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nitems, const SliceDesc* desc,
-                        const AdjItem* items, const float4* views, const Ray* rays,
-                        const float* sino, const float* zero_row, const int* symtab, float* img,
+// symmetric kernels (ctprj_sym.hip), one launch per symmetry group; nsplit > 1 work-groups
+// per item leave partial sums in part (nitems * nsplit * 512 floats), added by a second
+// launch
+void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
+                        const SliceDesc* desc, const FwdItem* items, const float4* views,
+                        const Ray* rays, const float* img, const float* imgT, float* sino,
                         hipStream_t st);
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* part, int nitems,
+                        const SliceDesc* desc, const AdjItem* items, const float4* views,
+                        const Ray* rays, const float* sino, const float* zero_row,
+                        const int* symtab, float* img, hipStream_t st);
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
                         hipStream_t st);
 

@@ -224,14 +224,35 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
     }
 }
 
+// The sum t of image e of canonical ray (it, lane) written to its view g_e a0.
+__device__ __forceinline__ void fwd_sym_store(const SliceDesc& S, const FwdItem& it, int a0, int e,
+                                              int lane, float t, float* __restrict__ sino)
+{
+    const int b = it.bin0 + lane;
+    const int bmax = S.fold ? S.pC : S.bhi;
+    const bool active = b >= S.blo && b <= bmax;
+    float* out = sino + S.sino_off;
+    int sg, off;
+    const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
+    if (v >= S.np) return;
+    // folded scans: images e and e ^ 2 (R^2 apart) meet at the centre bin, whose two sums
+    // are equal up to rounding; one writer (k < 2) keeps the result deterministic
+    if (S.fold && (e & 2) && b == S.pC) return;
+    if (b < S.blo) out[(long long)v * S.W] = 0.0f;  // even detector: bin 0 carries no ray
+    else if (active) out[(long long)v * S.W + off + sg * b] = t;
+}
+
 // One canonical ray run over the images in MASK (the group, or its rotations on a
-// mirror view).
+// mirror view).  The image rows are cut into nsplit * SEG segments: this work-group
+// walks segments split * SEG + wave.  nsplit == 1 writes the sinogram; otherwise the
+// work-group's sums go to part[e * 64 + lane] for k_forward_sym_combine.
 template <bool FAN, unsigned MASK, bool COLS, int SEG>
 __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& it, float cT,
                                             float sT, int a0, const Ray* __restrict__ rays,
                                             const float* __restrict__ img,
                                             const float* __restrict__ imgT,
-                                            float* __restrict__ sino, float (*red)[8][64])
+                                            float* __restrict__ sino, float (*red)[8][64],
+                                            int split, int nsplit, float* __restrict__ part)
 {
     // the wave index is uniform over the wave: the row loop then runs on scalar registers
     const int lane = (int)threadIdx.x & 63;
@@ -247,8 +268,8 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
             r = rays[S.ray_off + it.view];
             r.dc = (float)(b - S.pC) * S.dSize;
         }
-        const int nrow = 2 * S.H + 1, per = (nrow + SEG - 1) / SEG;
-        const int vlo = -S.H + seg * per, vhi = min(S.H, vlo + per - 1);
+        const int nrow = 2 * S.H + 1, nseg = SEG * nsplit, per = (nrow + nseg - 1) / nseg;
+        const int vlo = -S.H + (split * SEG + seg) * per, vhi = min(S.H, vlo + per - 1);
         fwd_sym_rows<FAN, MASK, COLS>(S, r, cT, sT, img, imgT, vlo, vhi, acc);
     }
     if (SEG > 1) {
@@ -258,8 +279,7 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
         __syncthreads();
     }
     // wave w finalises the images e = w (mod SEG): the sum of the waves' partials in wave
-    // order, written to view g_e a0
-    float* out = sino + S.sino_off;
+    // order
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         if (!(MASK >> e & 1u) || (SEG > 1 && e % SEG != seg)) continue;
@@ -269,14 +289,8 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
 #pragma unroll
             for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
         }
-        int sg, off;
-        const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
-        if (v >= S.np) continue;
-        // folded scans: images e and e ^ 2 (R^2 apart) meet at the centre bin, whose two
-        // sums are equal up to rounding; one writer (k < 2) keeps the result deterministic
-        if (S.fold && (e & 2) && b == S.pC) continue;
-        if (b < S.blo) out[(long long)v * S.W] = 0.0f;  // even detector: bin 0 carries no ray
-        else if (active) out[(long long)v * S.W + off + sg * b] = t;
+        if (part) part[e * 64 + lane] = t;
+        else fwd_sym_store(S, it, a0, e, lane, t, sino);
     }
 }
 
@@ -286,14 +300,16 @@ __device__ __forceinline__ void fwd_sym_group(const SliceDesc& S, const FwdItem&
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ img,
                                               const float* __restrict__ imgT,
-                                              float* __restrict__ sino, float (*red)[8][64])
+                                              float* __restrict__ sino, float (*red)[8][64],
+                                              int split, int nsplit, float* __restrict__ part)
 {
     // D4's canonical views lie in [0, 45] degrees and always walk rows; V4's reach 90 and
     // Z2's 180 degrees, and the steep ones walk columns
     constexpr GroupMasks GM = group_masks(GRP);
     const bool cols = GRP != kGrpD4 && fabsf(cT) < fabsf(sT);
-#define CTPRJ_RUN(MASK, COLS) \
-    fwd_sym_run<FAN, MASK, COLS, SEG>(S, it, cT, sT, a0, rays, img, imgT, sino, red)
+#define CTPRJ_RUN(MASK, COLS)                                                                 \
+    fwd_sym_run<FAN, MASK, COLS, SEG>(S, it, cT, sT, a0, rays, img, imgT, sino, red, split, \
+                                      nsplit, part)
     if (GRP != kGrpD4 && cols) {
         if (mirror) CTPRJ_RUN(GM.rot, GRP != kGrpD4);
         else CTPRJ_RUN(GM.all, GRP != kGrpD4);
@@ -304,10 +320,19 @@ __device__ __forceinline__ void fwd_sym_group(const SliceDesc& S, const FwdItem&
 #undef CTPRJ_RUN
 }
 
+// Work-group -> (split, item): the first nitems work-groups take split 0 of every item,
+// the next nitems split 1, and so on.
+__device__ __forceinline__ void split_item(int nitems, int& split, int& idx)
+{
+    split = (int)blockIdx.x / nitems;
+    idx = (int)blockIdx.x - split * nitems;
+}
+
 // One work item: (slice, canonical view index, first bin of 64) of a slice under group
 // GRP (one launch per group keeps each variant's registers to its own images).  SEG
-// waves share the 64 rays, each walking 1/SEG of the image rows, and then share the
-// final sums.
+// waves share the 64 rays, each walking 1/SEG of the work-group's image rows, and then
+// share the final sums; with nsplit > 1, nsplit work-groups share the rows of an item
+// and leave their sums in part (512 floats per work-group) for k_forward_sym_combine.
 template <int SEG, int GRP>
 __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __restrict__ slices,
                                                           const FwdItem* __restrict__ items,
@@ -315,19 +340,55 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
                                                           const Ray* __restrict__ rays,
                                                           const float* __restrict__ img,
                                                           const float* __restrict__ imgT,
-                                                          float* __restrict__ sino)
+                                                          float* __restrict__ sino, int nitems,
+                                                          int nsplit, float* __restrict__ part)
 {
     __shared__ float red[SEG][8][64];
-    const FwdItem it = items[blockIdx.x];
+    int split, idx;
+    split_item(nitems, split, idx);
+    const FwdItem it = items[idx];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
     const float4 T = views[S.view_off + it.view];
     const int a0 = __float_as_int(T.z);
     const bool mirror = a0 == 0 || group_masks(GRP).q * a0 == S.P;
+    float* mine = part ? part + (long long)blockIdx.x * 512 : nullptr;
     // (fan, mirror, walk) are uniform over the work-group: the barrier inside is reached
     // by all of its waves
-    if (S.fan) fwd_sym_group<true, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
-    else fwd_sym_group<false, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red);
+    if (S.fan)
+        fwd_sym_group<true, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red,
+                                      split, nsplit, mine);
+    else
+        fwd_sym_group<false, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red,
+                                       split, nsplit, mine);
+}
+
+// The nsplit partial sums of each canonical ray of an item, added in split order and
+// written like the one-work-group path (64 threads, one ray each).
+template <int GRP>
+__global__ __launch_bounds__(64) void k_forward_sym_combine(const SliceDesc* __restrict__ slices,
+                                                            const FwdItem* __restrict__ items,
+                                                            const float4* __restrict__ views,
+                                                            int nitems, int nsplit,
+                                                            const float* __restrict__ part,
+                                                            float* __restrict__ sino)
+{
+    const FwdItem it = items[blockIdx.x];
+    if (it.slice < 0) return;
+    const SliceDesc S = slices[it.slice];
+    const int a0 = __float_as_int(views[S.view_off + it.view].z);
+    constexpr GroupMasks GM = group_masks(GRP);
+    const unsigned mask = (a0 == 0 || GM.q * a0 == S.P) ? GM.rot : GM.all;
+    const int lane = threadIdx.x;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        if (!(mask >> e & 1u)) continue;
+        float t = 0.0f;
+        for (int k = 0; k < nsplit; ++k)
+            t = k ? t + part[((long long)k * nitems + blockIdx.x) * 512 + e * 64 + lane]
+                  : part[(long long)blockIdx.x * 512 + e * 64 + lane];
+        fwd_sym_store(S, it, a0, e, lane, t, sino);
+    }
 }
 
 // ------------------------------------------------------------------ adjoint ---
@@ -574,6 +635,38 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
 // Orbit base pixels of a work item: an 8x8 tile of the fundamental domain of the group,
 //   D4: 0 <= y0 <= x0 <= H,  V4: 0 <= x0, y0 <= H,  Z2: 0 <= x0 <= H, -H <= y0 <= H
 // (tiles start inside it; the upper ends are checked here).
+template <int GRP>
+__device__ __forceinline__ bool adj_sym_base(const SliceDesc& S, const AdjItem& it, int lane,
+                                             int& x0, int& y0)
+{
+    x0 = it.col0 + (lane & 7);
+    y0 = it.row0 + (lane >> 3);
+    return x0 <= S.H && y0 <= (GRP == kGrpD4 ? x0 : S.H);
+}
+
+// The sum t of orbit image h of (x0, y0) written to the image, unless an earlier element
+// (in element order) reaches the same pixel: each distinct orbit pixel is written once.
+template <bool FBP, int GRP>
+__device__ __forceinline__ void adj_sym_store(const SliceDesc& S, int x0, int y0, int h, float t,
+                                              float* __restrict__ img)
+{
+    constexpr GroupMasks G = group_masks(GRP);
+    int ox, oy;
+    apply_px(h & 3, h >> 2, x0, y0, ox, oy);
+#pragma unroll
+    for (int g = 0; g < 8; ++g) {
+        if (g >= h) break;
+        if (!(G.all >> g & 1u)) continue;
+        int gx, gy;
+        apply_px(g & 3, g >> 2, x0, y0, gx, gy);
+        if (gx == ox && gy == oy) return;
+    }
+    img[S.img_off + (long long)(oy + S.half) * S.n + ox + S.half] = FBP ? t * S.fbp_scale : t;
+}
+
+// The canonical views are cut into nsplit * SEG strided sets: wave w of this work-group
+// takes i = split * SEG + w + j * nsplit * SEG.  nsplit == 1 writes the image; otherwise
+// the work-group's sums go to part[h * 64 + lane] for k_adjoint_sym_combine.
 template <bool FBP, int SEG, int GRP, int WIN>
 __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& it,
                                              const float4* __restrict__ views,
@@ -581,23 +674,24 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
                                              const float* __restrict__ sino,
                                              const float* __restrict__ zero_row,
                                              const int* __restrict__ symtab,
-                                             float* __restrict__ img, float (*red)[8][64])
+                                             float* __restrict__ img, float (*red)[8][64],
+                                             int split, int nsplit, float* __restrict__ part)
 {
     constexpr GroupMasks G = group_masks(GRP);
     // the wave index is uniform over the wave: the view loop then runs on scalar registers
     const int lane = (int)threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
-    const int x0 = it.col0 + (lane & 7);
-    const int y0 = it.row0 + (lane >> 3);
-    const bool valid = x0 <= S.H && y0 <= (GRP == kGrpD4 ? x0 : S.H);
+    int x0, y0;
+    const bool valid = adj_sym_base<GRP>(S, it, lane, x0, y0);
     KahanAcc acc[8];
     if (valid) {
+        const int i0 = split * SEG + wave, step = SEG * nsplit;
         if (S.fan)
             adj_sym_pixel<true, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
-                                               wave, SEG, acc);
+                                               i0, step, acc);
         else
             adj_sym_pixel<false, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
-                                                wave, SEG, acc);
+                                                i0, step, acc);
     }
     if (SEG > 1) {
 #pragma unroll
@@ -606,33 +700,26 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
         __syncthreads();
     }
     if (!valid) return;
-    float* out = img + S.img_off;
-    const int n = S.n, half = S.half;
-    // each distinct orbit pixel h q0 once, from the first h (in element order) that
-    // reaches it; wave w finalises the elements h = w (mod SEG): the sum of the waves'
-    // partials in wave order
-    int ox[8], oy[8];
+    // wave w finalises the elements h = w (mod SEG): the sum of the waves' partials in
+    // wave order
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
-        if (!(G.all >> h & 1u)) continue;
-        apply_px(h & 3, h >> 2, x0, y0, ox[h], oy[h]);
-        bool fresh = true;
-#pragma unroll
-        for (int g = 0; g < h; ++g)
-            if ((G.all >> g & 1u) && ox[g] == ox[h] && oy[g] == oy[h]) fresh = false;
-        if (!fresh || (SEG > 1 && h % SEG != wave)) continue;
+        if (!(G.all >> h & 1u) || (SEG > 1 && h % SEG != wave)) continue;
         float t = acc[h].s;
         if (SEG > 1) {
             t = red[0][h][lane];
 #pragma unroll
             for (int q = 1; q < SEG; ++q) t += red[q][h][lane];
         }
-        out[(long long)(oy[h] + half) * n + ox[h] + half] = FBP ? t * S.fbp_scale : t;
+        if (part) part[h * 64 + lane] = t;
+        else adj_sym_store<FBP, GRP>(S, x0, y0, h, t, img);
     }
 }
 
 // One launch per symmetry group; SEG waves split the canonical views (wave w takes
-// i = w, w + SEG, ...) and then share the final sums.
+// i = w, w + SEG, ...) and then share the final sums; with nsplit > 1, nsplit
+// work-groups share the views of a tile and leave their sums in part (512 floats per
+// work-group) for k_adjoint_sym_combine.
 template <bool FBP, int SEG, int GRP, int WIN>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
@@ -641,13 +728,44 @@ __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __res
                                                           const float* __restrict__ sino,
                                                           const float* __restrict__ zero_row,
                                                           const int* __restrict__ symtab,
-                                                          float* __restrict__ img)
+                                                          float* __restrict__ img, int nitems,
+                                                          int nsplit, float* __restrict__ part)
 {
     __shared__ float red[SEG][8][64];
-    const AdjItem it = items[blockIdx.x];
+    int split, idx;
+    split_item(nitems, split, idx);
+    const AdjItem it = items[idx];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
-    adj_sym_tile<FBP, SEG, GRP, WIN>(S, it, views, rays, sino, zero_row, symtab, img, red);
+    adj_sym_tile<FBP, SEG, GRP, WIN>(S, it, views, rays, sino, zero_row, symtab, img, red, split,
+                                     nsplit, part ? part + (long long)blockIdx.x * 512 : nullptr);
+}
+
+// The nsplit partial sums of each orbit image of a tile, added in split order and
+// written like the one-work-group path (64 threads, one orbit base pixel each).
+template <bool FBP, int GRP>
+__global__ __launch_bounds__(64) void k_adjoint_sym_combine(const SliceDesc* __restrict__ slices,
+                                                            const AdjItem* __restrict__ items,
+                                                            int nitems, int nsplit,
+                                                            const float* __restrict__ part,
+                                                            float* __restrict__ img)
+{
+    const AdjItem it = items[blockIdx.x];
+    if (it.slice < 0) return;
+    const SliceDesc S = slices[it.slice];
+    constexpr GroupMasks G = group_masks(GRP);
+    const int lane = threadIdx.x;
+    int x0, y0;
+    if (!adj_sym_base<GRP>(S, it, lane, x0, y0)) return;
+#pragma unroll
+    for (int h = 0; h < 8; ++h) {
+        if (!(G.all >> h & 1u)) continue;
+        float t = 0.0f;
+        for (int k = 0; k < nsplit; ++k)
+            t = k ? t + part[((long long)k * nitems + blockIdx.x) * 512 + h * 64 + lane]
+                  : part[(long long)blockIdx.x * 512 + h * 64 + lane];
+        adj_sym_store<FBP, GRP>(S, x0, y0, h, t, img);
+    }
 }
 
 // Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the
@@ -666,13 +784,16 @@ __global__ void k_zero_border(const SliceDesc* __restrict__ slices, const int* _
 // ----------------------------------------------------------------- launchers ---
 
 template <int GRP>
-static void launch_forward_sym_g(int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
-                                 const float4* views, const Ray* rays, const float* img,
-                                 const float* imgT, float* sino, hipStream_t st)
+static void launch_forward_sym_g(int seg, int nsplit, float* part, int nitems,
+                                 const SliceDesc* desc, const FwdItem* items, const float4* views,
+                                 const Ray* rays, const float* img, const float* imgT, float* sino,
+                                 hipStream_t st)
 {
-#define CTPRJ_LAUNCH(SEG)                                                                     \
-    hipLaunchKernelGGL((k_forward_sym<SEG, GRP>), dim3(nitems), dim3(64 * SEG), 0, st, desc,   \
-                       items, views, rays, img, imgT, sino)
+    float* pp = nsplit > 1 ? part : nullptr;
+    const dim3 grid((unsigned)nitems * (unsigned)nsplit);
+#define CTPRJ_LAUNCH(SEG)                                                                      \
+    hipLaunchKernelGGL((k_forward_sym<SEG, GRP>), grid, dim3(64 * SEG), 0, st, desc, items,     \
+                       views, rays, img, imgT, sino, nitems, nsplit, pp)
     switch (seg) {
         case 1: CTPRJ_LAUNCH(1); break;
         case 2: CTPRJ_LAUNCH(2); break;
@@ -681,29 +802,38 @@ static void launch_forward_sym_g(int seg, int nitems, const SliceDesc* desc, con
         default: CTPRJ_LAUNCH(16); break;
     }
 #undef CTPRJ_LAUNCH
+    if (pp)
+        hipLaunchKernelGGL((k_forward_sym_combine<GRP>), dim3((unsigned)nitems), dim3(64), 0, st,
+                           desc, items, views, nitems, nsplit, pp, sino);
 }
 
-void launch_forward_sym(int grp, int seg, int nitems, const SliceDesc* desc, const FwdItem* items,
-                        const float4* views, const Ray* rays, const float* img,
-                        const float* imgT, float* sino, hipStream_t st)
+void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
+                        const SliceDesc* desc, const FwdItem* items, const float4* views,
+                        const Ray* rays, const float* img, const float* imgT, float* sino,
+                        hipStream_t st)
 {
     if (grp == kGrpD4)
-        launch_forward_sym_g<kGrpD4>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
+        launch_forward_sym_g<kGrpD4>(seg, nsplit, part, nitems, desc, items, views, rays, img,
+                                     imgT, sino, st);
     else if (grp == kGrpV4)
-        launch_forward_sym_g<kGrpV4>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
+        launch_forward_sym_g<kGrpV4>(seg, nsplit, part, nitems, desc, items, views, rays, img,
+                                     imgT, sino, st);
     else
-        launch_forward_sym_g<kGrpZ2>(seg, nitems, desc, items, views, rays, img, imgT, sino, st);
+        launch_forward_sym_g<kGrpZ2>(seg, nsplit, part, nitems, desc, items, views, rays, img,
+                                     imgT, sino, st);
 }
 
 template <bool FBP, int GRP, int WIN>
-static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, const AdjItem* items,
-                                 const float4* views, const Ray* rays, const float* sino,
-                                 const float* zero_row, const int* symtab, float* img,
-                                 hipStream_t st)
+static void launch_adjoint_sym_t(int seg, int nsplit, float* part, int nitems,
+                                 const SliceDesc* desc, const AdjItem* items, const float4* views,
+                                 const Ray* rays, const float* sino, const float* zero_row,
+                                 const int* symtab, float* img, hipStream_t st)
 {
-#define CTPRJ_LAUNCH(SEG)                                                                   \
-    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN>), dim3(nitems), dim3(64 * SEG), 0, \
-                       st, desc, items, views, rays, sino, zero_row, symtab, img)
+    float* pp = nsplit > 1 ? part : nullptr;
+    const dim3 grid((unsigned)nitems * (unsigned)nsplit);
+#define CTPRJ_LAUNCH(SEG)                                                                      \
+    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN>), grid, dim3(64 * SEG), 0, st, desc,  \
+                       items, views, rays, sino, zero_row, symtab, img, nitems, nsplit, pp)
     switch (seg) {
         case 1: CTPRJ_LAUNCH(1); break;
         case 2: CTPRJ_LAUNCH(2); break;
@@ -711,42 +841,45 @@ static void launch_adjoint_sym_t(int seg, int nitems, const SliceDesc* desc, con
         default: CTPRJ_LAUNCH(8); break;
     }
 #undef CTPRJ_LAUNCH
+    if (pp)
+        hipLaunchKernelGGL((k_adjoint_sym_combine<FBP, GRP>), dim3((unsigned)nitems), dim3(64), 0,
+                           st, desc, items, nitems, nsplit, pp, img);
 }
 
 template <bool FBP, int WIN>
-static void launch_adjoint_sym_f(int grp, int seg, int nitems, const SliceDesc* desc,
-                                 const AdjItem* items, const float4* views, const Ray* rays,
-                                 const float* sino, const float* zero_row, const int* symtab,
-                                 float* img, hipStream_t st)
+static void launch_adjoint_sym_f(int grp, int seg, int nsplit, float* part, int nitems,
+                                 const SliceDesc* desc, const AdjItem* items, const float4* views,
+                                 const Ray* rays, const float* sino, const float* zero_row,
+                                 const int* symtab, float* img, hipStream_t st)
 {
     if (grp == kGrpD4)
-        launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
-                                          symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                               sino, zero_row, symtab, img, st);
     else if (grp == kGrpV4)
-        launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
-                                          symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                               sino, zero_row, symtab, img, st);
     else
-        launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, nitems, desc, items, views, rays, sino, zero_row,
-                                          symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                               sino, zero_row, symtab, img, st);
 }
 
-void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nitems, const SliceDesc* desc,
-                        const AdjItem* items, const float4* views, const Ray* rays,
-                        const float* sino, const float* zero_row, const int* symtab, float* img,
-                        hipStream_t st)
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* part, int nitems,
+                        const SliceDesc* desc, const AdjItem* items, const float4* views,
+                        const Ray* rays, const float* sino, const float* zero_row,
+                        const int* symtab, float* img, hipStream_t st)
 {
     if (fbp && win == 3)
-        launch_adjoint_sym_f<true, 3>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                      symtab, img, st);
+        launch_adjoint_sym_f<true, 3>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
+                                      sino, zero_row, symtab, img, st);
     else if (fbp)
-        launch_adjoint_sym_f<true, 4>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                      symtab, img, st);
+        launch_adjoint_sym_f<true, 4>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
+                                      sino, zero_row, symtab, img, st);
     else if (win == 3)
-        launch_adjoint_sym_f<false, 3>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                       symtab, img, st);
+        launch_adjoint_sym_f<false, 3>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
+                                       sino, zero_row, symtab, img, st);
     else
-        launch_adjoint_sym_f<false, 4>(grp, seg, nitems, desc, items, views, rays, sino, zero_row,
-                                       symtab, img, st);
+        launch_adjoint_sym_f<false, 4>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
+                                       sino, zero_row, symtab, img, st);
 }
 
 void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,

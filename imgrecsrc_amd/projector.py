@@ -159,13 +159,16 @@ class Projector:
         t = Tune()
         check(lib().ctprj_plan_get_tune(self.handle, ctypes.byref(t)))
         return {"fwd_block": t.fwd_block, "adj_tile": t.adj_tile, "flags": t.flags,
-                "adj_seg": t.adj_seg, "fwd_seg": t.fwd_seg}
+                "adj_seg": t.adj_seg, "fwd_seg": t.fwd_seg, "adj_split": t.adj_split,
+                "fwd_split": t.fwd_split}
 
-    def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None, fwd_seg=None):
+    def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None, fwd_seg=None,
+                 adj_split=None, fwd_split=None):
         cur = self.get_tune()
         t = Tune(fwd_block or cur["fwd_block"], adj_tile or cur["adj_tile"],
                  cur["flags"] if flags is None else flags, adj_seg or cur["adj_seg"],
-                 fwd_seg or cur["fwd_seg"])
+                 fwd_seg or cur["fwd_seg"], adj_split or cur["adj_split"],
+                 fwd_split or cur["fwd_split"])
         check(lib().ctprj_plan_set_tune(self.handle, ctypes.byref(t)))
 
     def autotune(self, reps: int = 3, cache_path: str | None = None, stream=None) -> dict:

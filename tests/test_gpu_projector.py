@@ -247,6 +247,45 @@ def test_adjoint_window_width_is_bitwise_neutral(torch_dev, geo, flags):
     assert torch.equal(P.fbp(y), fbp)
 
 
+@pytest.mark.parametrize("geo,flags", WINDOW_CASES)
+def test_split_work_items_match_one_work_group(torch_dev, geo, flags):
+    """Work-groups per item (tune adj_split / fwd_split) only regroup the partial sums of
+    the views (adjoint) or image rows (forward): forward, adjoint and FBP agree with the
+    one-work-group results to FP32 rounding, and the forward keeps its exact zeros."""
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom, gaussian
+    P = _proj(geo, flags)
+    x = torch.from_numpy(ellipse_phantom(geo[0], 5)).cuda().reshape(-1)
+    y = torch.from_numpy(gaussian((geo[2], geo[1]), 14)).cuda().reshape(-1)
+    f1, a1, b1 = P.forward(x).clone(), P.adjoint(y).clone(), P.fbp(y).clone()
+    for k in (2, 3, 4):
+        P.set_tune(adj_split=k, fwd_split=k)
+        f, a, b = P.forward(x), P.adjoint(y), P.fbp(y)
+        for got, ref in ((f, f1), (a, a1), (b, b1)):
+            assert torch.allclose(got, ref, rtol=0, atol=2e-6 * ref.abs().max().item()), k
+        assert torch.equal(f == 0, f1 == 0)
+    P.set_tune(adj_split=1, fwd_split=1)
+    assert torch.equal(P.adjoint(y), a1) and torch.equal(P.forward(x), f1)
+
+
+def test_split_work_items_in_a_mixed_batch(torch_dev):
+    """Slices of the three symmetry groups in one plan: each group's launches use the same
+    partial-sum space in stream order."""
+    import torch
+    from imgrecsrc_amd.phantom import gaussian
+    from imgrecsrc_amd.projector import Projector
+    geoms = [(64, 95, 36, 72, 1.0, 1.0, 0.0), (48, 70, 30, 30, 1.0, 1.0, 150.0),
+             (40, 61, 25, 25, 1.0, 1.0, 0.0), (96, 139, 64, 64, 1.0, 1.0, 400.0)]
+    P = Projector(geoms)
+    assert sorted(set(P.modes())) == [1, 2, 3]
+    x = torch.rand(P.img_elems, device="cuda")
+    y = torch.from_numpy(gaussian((1, P.sino_elems), 15)).cuda().reshape(-1)
+    f1, a1 = P.forward(x).clone(), P.adjoint(y).clone()
+    P.set_tune(adj_split=3, fwd_split=2)
+    assert torch.allclose(P.forward(x), f1, rtol=0, atol=2e-6 * f1.abs().max().item())
+    assert torch.allclose(P.adjoint(y), a1, rtol=0, atol=2e-6 * a1.abs().max().item())
+
+
 def test_nnz_matches_oracle(torch_dev, oracle_mod):
     """Device count of non-zero FP32 weights vs the FP64 count: overlaps of a pixel corner
     with a strip edge below ~1e-9 px^2 round to exactly 0 in FP32, so the FP32 count is
@@ -295,6 +334,7 @@ def test_autotune_roundtrip(torch_dev, tmp_path):
     t = P.autotune(reps=2, cache_path=cache)
     assert t["fwd_block"] in (64, 128, 256) and t["adj_tile"] in (8, 16)
     assert t["adj_seg"] in (1, 2, 4, 8) and t["fwd_seg"] in (1, 2, 4, 8, 16)
+    assert t["adj_split"] in (1, 2, 3, 4) and t["fwd_split"] in (1, 2, 3, 4)
     # shapes change the order of the partial sums, not the matrix
     assert torch.allclose(P.forward(x), base, rtol=1e-6, atol=1e-6 * base.abs().max().item())
     Q = _proj(geo)
