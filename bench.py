@@ -101,11 +101,13 @@ def _col(row, *names):
 
 def pmc_summary(fetch_dir, write_dir, kernel_substr, tag, command):
     """Reduce two rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE; -f csv) to
-    profiles/<tag>_pmc.json: per-dispatch KiB of the kernel whose name contains
-    kernel_substr (its most frequent grid size: the timed workload), and the same counter
-    for the largest plain device copy (the --calib-copy 256 MiB copy) to scale by."""
+    profiles/<tag>_pmc.json: per-call KiB of the kernels of one operator call, named
+    '+'-separated (each kernel's trailing dispatches at its last grid size: the timed
+    workload), and the same counter for the largest plain device copy (the --calib-copy
+    256 MiB copy) to scale by."""
     import glob
     import statistics
+    parts = [k for k in kernel_substr.split("+") if k]
     out = {"command": command, "kernel": kernel_substr}
     for counter, d in (("FETCH_SIZE", fetch_dir), ("WRITE_SIZE", write_dir)):
         files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
@@ -113,21 +115,34 @@ def pmc_summary(fetch_dir, write_dir, kernel_substr, tag, command):
             raise SystemExit(f"no counter_collection.csv under {d}")
         rows = [r for f in files for r in _csv_rows(f)
                 if _col(r, "Counter_Name") == counter]
-        kern = [r for r in rows if kernel_substr in _col(r, "Kernel_Name")]
-        if not kern:
+        total, per_kernel = 0.0, {}
+        for part in parts:
+            # truncated (-T) names are the bare function name; full ones carry <...>(...)
+            kern = [r for r in rows
+                    if _col(r, "Kernel_Name").split("<")[0].split("(")[0].split("::")[-1] == part]
+            if not kern:
+                continue
+            # the trailing dispatches at the last grid size: the timed shape, after the
+            # autotuner's trial shapes
+            kern.sort(key=lambda r: int(float(_col(r, "Dispatch_Id"))))
+            g = int(float(_col(kern[-1], "Grid_Size")))
+            vals = []
+            for r in reversed(kern):
+                if int(float(_col(r, "Grid_Size"))) != g:
+                    break
+                vals.append(float(_col(r, "Counter_Value")))
+            per_kernel[part] = {"KiB": statistics.mean(vals), "dispatches": len(vals), "grid": g}
+            total += statistics.mean(vals)
+        if not per_kernel:
             raise SystemExit(f"{counter}: no dispatch of {kernel_substr}")
-        grids = [int(float(_col(r, "Grid_Size"))) for r in kern]
-        g = max(set(grids), key=grids.count)
-        vals = [float(_col(r, "Counter_Value")) for r, gg in zip(kern, grids) if gg == g]
         copies = [r for r in rows if "copyBuffer" in _col(r, "Kernel_Name")]
         big = max(copies, key=lambda r: int(float(_col(r, "Grid_Size"))))
         bg = int(float(_col(big, "Grid_Size")))
         cv = [float(_col(r, "Counter_Value")) for r in copies
               if int(float(_col(r, "Grid_Size"))) == bg]
         copy_kib = 256.0 * 1024.0  # 64 Mi floats read and written
-        out[f"{counter}_KiB"] = statistics.mean(vals)
-        out[f"{counter}_dispatches"] = len(vals)
-        out[f"{counter}_grid"] = g
+        out[f"{counter}_KiB"] = total
+        out[f"{counter}_per_kernel"] = per_kernel
         out[f"copy_{'fetch' if counter == 'FETCH_SIZE' else 'write'}_factor"] = \
             statistics.mean(cv) / copy_kib
     out["traffic_bytes"] = (out["FETCH_SIZE_KiB"] / out["copy_fetch_factor"] +
@@ -238,6 +253,9 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "TAG"),
                     help="reduce two rocprofv3 --pmc csv runs to profiles/TAG_pmc.json")
     ap.add_argument("--pmc-command", default="", help="command recorded in the summary")
+    ap.add_argument("--tune-cache", default=None,
+                    help="autotuner cache file: a later run with the same file reuses the "
+                         "shapes the first one chose (profiling runs)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.pmc_command)
@@ -263,7 +281,7 @@ def main():
     n, W, npj = geo[0], geo[1], geo[2]
     log(f"workload {args.workload} {geo}")
     P = Projector([geo])
-    tune = P.autotune(reps=5)
+    tune = P.autotune(reps=5, cache_path=args.tune_cache)
     log(f"autotuned shapes {tune}")
     img_np = ellipse_phantom(n, 2)
     img = torch.from_numpy(img_np).to(dev).reshape(-1)
@@ -313,7 +331,12 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "kernel": dom_name, "kernel_ms": round(dom_ms, 4),
                 "algorithmic_bytes": alg_bytes,
-                "other_kernel_ms": round(t_fwd if dom_name.startswith("k_adjoint") else t_adj, 4)}
+                "other_kernel_ms": round(t_fwd if dom_name.startswith("k_adjoint") else t_adj, 4),
+                "launches": (["k_adjoint_sym"] + (["k_adjoint_sym_combine"] if tune["adj_split"] > 1
+                                                  else []) + (["k_zero_border"] if n % 2 == 0 else [])
+                             if dom_name.startswith("k_adjoint") else
+                             ["k_transpose", "k_forward_sym"] + (["k_forward_sym_combine"]
+                                                                 if tune["fwd_split"] > 1 else []))}
 
     # <Ax, y> = <x, A^T y> on seeded Gaussian x, y
     x = torch.from_numpy(gaussian((n, n), 21)).to(dev).reshape(-1)
