@@ -84,7 +84,9 @@ int prj_get_geom(ctprj_geom* out);
 /* Plan for `nslices` independent slices (ragged sizes and mixed parallel/fan allowed).
  * Images of all slices are packed back to back in one float array, likewise sinograms;
  * see ctprj_plan_layout.  Allocates the plan's device tables and workspace on the
- * current HIP device. */
+ * current HIP device.  The calls of one plan share its workspace (the transposed image,
+ * the filtered sinogram, partial sums): calls on one plan that overlap on different
+ * streams race; use one stream per plan, or one plan per stream. */
 int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_plan** plan);
 int ctprj_plan_destroy(ctprj_plan* plan);
 
