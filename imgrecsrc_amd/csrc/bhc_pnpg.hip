@@ -319,6 +319,10 @@ struct ctprj_pnpg {
     int maxInnerItr = 100, maxPossibleInnerItr = 1000, innerSearch = 0;
     double* d_wpart = nullptr;    // 5 partials per work-group
     float* d_mask = nullptr;      // support of alpha (NULL: every pixel)
+    // host-driven trials: the state as last queued to d_state (f_old / f_new excepted,
+    // which the trial's kernels write)
+    PnpgState h_state{};
+    bool h_valid = false;
     ~ctprj_pnpg()
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
@@ -408,11 +412,21 @@ static int state_get(ctprj_pnpg* s, PnpgState* h, hipStream_t st)
     return e == hipSuccess ? 0 : hip_fail(e, "pnpg state read");
 }
 
+__global__ void k_pnpg_state_put(PnpgState* __restrict__ S, PnpgState h)
+{
+    if (threadIdx.x == 0) *S = h;
+}
+
+// Queues the host's state to the device (a kernel argument: no copy, no wait) and keeps
+// it as the host-driven trials' copy.
 static int state_put(ctprj_pnpg* s, const PnpgState* h, hipStream_t st)
 {
-    hipError_t e = hipMemcpyAsync(s->d_state, h, sizeof(*h), hipMemcpyHostToDevice, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    return e == hipSuccess ? 0 : hip_fail(e, "pnpg state write");
+    hipLaunchKernelGGL(k_pnpg_state_put, dim3(1), dim3(64), 0, st, s->d_state, *h);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "pnpg state write");
+    s->h_state = *h;
+    s->h_valid = true;
+    return 0;
 }
 
 // The iterative prox of the solver: wavelet ADMM (npg/admm.m) or TV FGP
@@ -436,8 +450,9 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
 {
     const PnpgParams& P = s->pp;
     PnpgState h;
-    int rc = state_get(s, &h, st);
-    if (rc) return rc;
+    int rc = 0;
+    if (s->h_valid) h = s->h_state;  // the device state is what the last trial queued
+    else if ((rc = state_get(s, &h, st))) return rc;
     pnpg_begin(h, P);
     h.accept = 0;
     if ((rc = state_put(s, &h, st))) return rc;

@@ -7,8 +7,9 @@
 // ||C - D||^2 and ||C||^2) and the dual step (P = proj(R + Ltrans(D)/(8 lambda)), then
 // the momentum R = P + (tk - 1)(P - Pold)/tkp1).  The dual fields P1 (vertical
 // differences, rows 0..m-2) and P2 (horizontal, columns 0..n-2) are stored padded to
-// m x n.  The host reads the four sums once per iteration for the stopping test and the
-// momentum restart.  All arithmetic is FP64 in the reference's order; images are FP32.
+// m x n.  A one-work-group launch per iteration adds the four sums for the stopping test
+// and the momentum restart and keeps the loop state on the device; the host reads it once
+// per four iterations.  All arithmetic is FP64 in the reference's order; images are FP32.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -42,13 +43,27 @@ __device__ __forceinline__ void tv_block_sum4(const double v[4], double* part)
 }
 
 // primal step (utils/denoise_bound_mod.m:118-119 with utils/Lforward.m)
+// FGP loop state on the device (utils/denoise_bound_mod.m:95-150): the iteration's
+// kernels read tk from here and k_tv_ctrl updates it, so the host looks at the state once
+// per chunk of iterations; kernels queued after the stop return at once.
+struct TvCtl {
+    double t;     // tk of the next iteration (the last tkp1, or 1 after a restart)
+    double fval;  // objective of the last iteration
+    int i, count, done, pad;
+};
+
+// tkp1 from tk (utils/denoise_bound_mod.m:117), the same expression wherever it is formed
+__device__ __forceinline__ double tv_next_t(double tk) { return (1.0 + sqrt(1.0 + 4.0 * tk * tk)) / 2.0; }
+
 __global__ __launch_bounds__(kTvBlock) void k_tv_primal(const double* __restrict__ X,
                                                         double* __restrict__ D,
                                                         const double* __restrict__ R1,
                                                         const double* __restrict__ R2, int m,
                                                         int n, long long total, double lam,
-                                                        double* __restrict__ part)
+                                                        double* __restrict__ part,
+                                                        const TvCtl* __restrict__ c)
 {
+    if (c->done) return;  // uniform
     const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     if (t < total) {
@@ -75,8 +90,10 @@ __global__ __launch_bounds__(kTvBlock) void k_tv_dual(const double* __restrict__
                                                       double* __restrict__ R1,
                                                       double* __restrict__ R2, int m, int n,
                                                       long long total, double lam, int iso,
-                                                      double tk, double tkp1)
+                                                      const TvCtl* __restrict__ c)
 {
+    if (c->done) return;  // uniform
+    const double tk = c->t, tkp1 = tv_next_t(tk);
     const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
     if (t >= total) return;
     const int j = (int)(t % n);
@@ -104,6 +121,60 @@ __global__ __launch_bounds__(kTvBlock) void k_tv_dual(const double* __restrict__
         P2[t] = p2;
         R2[t] = p2 + (tk - 1.0) * (p2 - o) / tkp1;
     }
+}
+
+// D = P = R = 0 and the loop state: tk = 1, no iteration yet (utils/denoise_bound_mod.m:60-86)
+__global__ void k_tv_start(double* __restrict__ D, double* __restrict__ P1, double* __restrict__ P2,
+                           double* __restrict__ R1, double* __restrict__ R2, long long total,
+                           int maxIter, TvCtl* __restrict__ c)
+{
+    const long long t = (long long)blockIdx.x * kTvBlock + threadIdx.x;
+    if (t < total) D[t] = P1[t] = P2[t] = R1[t] = R2[t] = 0.0;
+    if (t == 0) {
+        c->t = 1.0;
+        c->fval = INFINITY;
+        c->i = 0;
+        c->count = 0;
+        c->done = maxIter <= 0;
+    }
+}
+
+// End of an iteration (utils/denoise_bound_mod.m:146-150): re = ||D - Dold|| / ||D||
+// counts towards the stop below epsilon, fval = -||C - D||^2 + ||C||^2, a rising fval
+// restarts the momentum (tkp1 = 1); stop after maxIter iterations or 5 in a row below
+// epsilon.  The per-group partials are summed in a fixed order.
+__global__ __launch_bounds__(kTvBlock) void k_tv_ctrl(const double* __restrict__ part,
+                                                      long long nblk, double epsilon,
+                                                      int maxIter, TvCtl* __restrict__ c)
+{
+    if (c->done) return;
+    __shared__ double sh[4][kTvBlock];
+    const int t = threadIdx.x;
+    double a[4] = {0.0, 0.0, 0.0, 0.0};
+    for (long long b = t; b < nblk; b += kTvBlock)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) a[k] += part[4 * b + k];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) sh[k][t] = a[k];
+    __syncthreads();
+    for (int off = kTvBlock / 2; off > 0; off >>= 1) {
+        if (t < off) {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) sh[k][t] += sh[k][t + off];
+        }
+        __syncthreads();
+    }
+    if (t != 0) return;
+    const double re = sqrt(sh[0][0]) / sqrt(sh[1][0]);
+    const int count = (re < epsilon) ? c->count + 1 : 0;
+    const double fval = -sh[2][0] + sh[3][0];
+    const double tkp1 = tv_next_t(c->t);
+    const int i = c->i + 1;
+    c->t = (fval > c->fval) ? 1.0 : tkp1;
+    c->fval = fval;
+    c->count = count;
+    c->i = i;
+    c->done = i >= maxIter || count >= 5;
 }
 
 // Support mask (utils/maskFunc.m): pixels where mask is 0 are zero; NULL keeps all.
@@ -154,12 +225,13 @@ struct ctprj_tv {
     long long n = 0, blocks = 0;
     double *d_X = nullptr, *d_D = nullptr, *d_P1 = nullptr, *d_P2 = nullptr, *d_R1 = nullptr,
            *d_R2 = nullptr, *d_part = nullptr;
+    TvCtl* d_ctl = nullptr;   // the FGP loop's state
     float* d_mask = nullptr;  // support (NULL: every pixel)
     std::vector<double> part;
     ~ctprj_tv()
     {
         for (void* p : {(void*)d_X, (void*)d_D, (void*)d_P1, (void*)d_P2, (void*)d_R1, (void*)d_R2,
-                        (void*)d_part, (void*)d_mask})
+                        (void*)d_part, (void*)d_ctl, (void*)d_mask})
             if (p) (void)hipFree(p);
     }
 };
@@ -203,6 +275,7 @@ extern "C" int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv*
     for (double** p : {&w->d_X, &w->d_D, &w->d_P1, &w->d_P2, &w->d_R1, &w->d_R2})
         if (e == hipSuccess) e = hipMalloc((void**)p, sizeof(double) * (size_t)n);
     if (e == hipSuccess) e = hipMalloc((void**)&w->d_part, sizeof(double) * 4 * (size_t)w->blocks);
+    if (e == hipSuccess) e = hipMalloc((void**)&w->d_ctl, sizeof(TvCtl));
     if (e != hipSuccess) {
         delete w;
         return fail(CTPRJ_ENOMEM, "tv workspace: %s", hipGetErrorString(e));
@@ -255,32 +328,30 @@ extern "C" int ctprj_tv_denoise(ctprj_tv* w, const float* d_x, double lambda, do
     const unsigned g = (unsigned)w->blocks;
     hipLaunchKernelGGL(k_tv_in, dim3(g), dim3(kTvBlock), 0, st, d_x, w->d_X, n, w->d_mask);
     // D = 0, P = proj(Ltrans(0)) = 0, R = P   (utils/denoise_bound_mod.m:60-86)
-    hipError_t e = hipSuccess;
-    for (double* p : {w->d_D, w->d_P1, w->d_P2, w->d_R1, w->d_R2})
-        if (e == hipSuccess) e = hipMemsetAsync(p, 0, sizeof(double) * (size_t)n, st);
-    if (e != hipSuccess) return hip_fail(e, "tv init");
-    double tk = 1.0, tkp1 = 1.0, fval = INFINITY;
-    int count = 0, i = 0, rc;
-    while (i < maxIter && count < 5) {
-        const double fold = fval;
-        ++i;
-        tk = tkp1;
-        hipLaunchKernelGGL(k_tv_primal, dim3(g), dim3(kTvBlock), 0, st, w->d_X, w->d_D, w->d_R1,
-                           w->d_R2, w->rows, w->cols, n, lambda, w->d_part);
-        tkp1 = (1.0 + std::sqrt(1.0 + 4.0 * tk * tk)) / 2.0;
-        hipLaunchKernelGGL(k_tv_dual, dim3(g), dim3(kTvBlock), 0, st, w->d_D, w->d_P1, w->d_P2,
-                           w->d_R1, w->d_R2, w->rows, w->cols, n, lambda, w->iso, tk, tkp1);
-        e = hipGetLastError();
+    TvCtl* C = w->d_ctl;
+    hipLaunchKernelGGL(k_tv_start, dim3(g), dim3(kTvBlock), 0, st, w->d_D, w->d_P1, w->d_P2,
+                       w->d_R1, w->d_R2, n, maxIter, C);
+    // iterations in chunks of kTvChunk, the host reading the state after each chunk
+    constexpr int kTvChunk = 4;
+    TvCtl h{};
+    for (;;) {
+        for (int k = 0; k < kTvChunk; ++k) {
+            hipLaunchKernelGGL(k_tv_primal, dim3(g), dim3(kTvBlock), 0, st, w->d_X, w->d_D,
+                               w->d_R1, w->d_R2, w->rows, w->cols, n, lambda, w->d_part, C);
+            hipLaunchKernelGGL(k_tv_dual, dim3(g), dim3(kTvBlock), 0, st, w->d_D, w->d_P1,
+                               w->d_P2, w->d_R1, w->d_R2, w->rows, w->cols, n, lambda, w->iso, C);
+            hipLaunchKernelGGL(k_tv_ctrl, dim3(1), dim3(kTvBlock), 0, st, w->d_part, w->blocks,
+                               epsilon, maxIter, C);
+        }
+        hipError_t e = hipGetLastError();
         if (e != hipSuccess) return hip_fail(e, "tv iteration launch");
-        double s[4];
-        if ((rc = tv_sums(w, st, s))) return rc;
-        const double re = std::sqrt(s[0]) / std::sqrt(s[1]);
-        count = (re < epsilon) ? count + 1 : 0;
-        fval = -s[2] + s[3];
-        if (fval > fold) tkp1 = 1.0;
+        e = hipMemcpyAsync(&h, C, sizeof(h), hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) return hip_fail(e, "tv state");
+        if (h.done) break;
     }
     hipLaunchKernelGGL(k_tv_out, dim3(g), dim3(kTvBlock), 0, st, w->d_D, d_out, n, w->d_mask);
-    if (iterations) *iterations = i;
-    e = hipGetLastError();
+    if (iterations) *iterations = h.i;
+    hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "tv output");
 }
