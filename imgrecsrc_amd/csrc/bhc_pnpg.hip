@@ -318,6 +318,8 @@ struct ctprj_pnpg {
     double u = 0.0, admmTol = 1e-2, difAlpha = 1.0;
     int maxInnerItr = 100, maxPossibleInnerItr = 1000, innerSearch = 0;
     double* d_wpart = nullptr;    // 5 partials per work-group
+    double* d_ppart = nullptr;    // partials of the two penalties of a trial
+    long long ppart_cap = 0;      // doubles per penalty held by d_ppart
     float* d_mask = nullptr;      // support of alpha (NULL: every pixel)
     // host-driven trials: the state as last queued to d_state (f_old / f_new excepted,
     // which the trial's kernels write)
@@ -327,7 +329,7 @@ struct ctprj_pnpg {
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
                         (void*)d_s, (void*)d_part, (void*)d_ray_part, (void*)d_wpart,
-                        (void*)d_mask})
+                        (void*)d_ppart, (void*)d_mask})
             if (p) (void)hipFree(p);
     }
 };
@@ -438,10 +440,21 @@ static int prox_solve(ctprj_pnpg* s, const float* a, double u, double tol, const
     return ctprj_tv_denoise(s->tv, a, u, tol, s->maxInnerItr, out, iters, st);
 }
 
-static int prox_penalty(ctprj_pnpg* s, const float* x, double* v, hipStream_t st)
+// The penalty ||W x||_1 or TV(x) queued as partial sums (penalty_parts doubles), added
+// on the host by penalty_total.
+static long long penalty_parts(const ctprj_pnpg* s)
 {
-    if (s->wv) return ctprj_wavelet_l1(s->wv, x, v, st);
-    return ctprj_tv_value(s->tv, x, v, st);
+    return s->wv ? wavelet_l1_parts(s->wv) : tv_value_parts(s->tv);
+}
+
+static int penalty_queue(ctprj_pnpg* s, const float* x, double* d_part, hipStream_t st)
+{
+    return s->wv ? wavelet_l1_queue(s->wv, x, d_part, st) : tv_value_queue(s->tv, x, d_part, st);
+}
+
+static double penalty_total(const ctprj_pnpg* s, const double* part)
+{
+    return s->wv ? wavelet_l1_total(s->wv, part) : tv_value_total(s->tv, part);
 }
 
 // One trial with an iterative prox: the same sequence as the device trial, with the
@@ -470,11 +483,20 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     if ((rc = pnpg_func(s, s->d_newx, &S->f_new, nullptr, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_wv_sums, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, s->d_xbar,
                        s->d_newx, s->d_grad, d_alpha, s->nimg, s->d_wpart);
+    // the penalties at newX and xbar (needed unless another trial follows) are queued
+    // with the line-search sums, so that one read-back serves both
+    const long long np = penalty_parts(s);
+    if ((rc = penalty_queue(s, s->d_newx, s->d_ppart, st)) ||
+        (rc = penalty_queue(s, s->d_xbar, s->d_ppart + np, st)))
+        return rc;
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "pnpg wavelet trial");
-    std::vector<double> part((size_t)5 * s->prox_blocks);
+    std::vector<double> part((size_t)5 * s->prox_blocks), pen((size_t)(2 * np));
     e = hipMemcpyAsync(part.data(), s->d_wpart, sizeof(double) * part.size(),
                        hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pen.data(), s->d_ppart, sizeof(double) * pen.size(),
+                           hipMemcpyDeviceToHost, st);
     if (e != hipSuccess) return hip_fail(e, "pnpg wavelet sums");
     PnpgState r;
     if ((rc = state_get(s, &r, st))) return rc;  // f_old, f_new (synchronises)
@@ -490,10 +512,7 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     const int ls = pnpg_linesearch(h, P, holds);
     if (ls == 0) return state_put(s, &h, st);
     const bool goodMM = ls == 1;
-    double pen_new, pen_bar;
-    if ((rc = prox_penalty(s, s->d_newx, &pen_new, st)) ||
-        (rc = prox_penalty(s, s->d_xbar, &pen_bar, st)))
-        return rc;
+    const double pen_new = penalty_total(s, pen.data()), pen_bar = penalty_total(s, pen.data() + np);
     double newObj = h.f_new + s->u * pen_new;
     const double objBar = h.f_old + s->u * pen_bar;
     bool keep = false;  // "force it to converge": newX = alpha (PNPG.m:141-144)
@@ -548,6 +567,15 @@ static int set_prox(ctprj_pnpg* s, ctprj_wavelet* w, ctprj_tv* tv, double u)
     if ((w || tv) && !s->d_wpart &&
         (rc = pnpg_alloc((void**)&s->d_wpart, sizeof(double) * 5 * s->prox_blocks, "pnpg sums")))
         return rc;
+    const long long np = w ? wavelet_l1_parts(w) : (tv ? tv_value_parts(tv) : 0);
+    if (np > s->ppart_cap) {
+        if (s->d_ppart) (void)hipFree(s->d_ppart);
+        s->d_ppart = nullptr;
+        s->ppart_cap = 0;
+        if ((rc = pnpg_alloc((void**)&s->d_ppart, sizeof(double) * 2 * np, "pnpg penalty sums")))
+            return rc;
+        s->ppart_cap = np;
+    }
     s->wv = w;
     s->tv = tv;
     s->u = (w || tv) ? u : 0.0;

@@ -112,5 +112,14 @@ void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* 
 long long wavelet_elems(const ctprj_wavelet* w);
 // Images held by a TV context (rows * cols * batch).
 long long tv_elems(const ctprj_tv* w);
+// Prior penalties in two steps: queue the per-group partial sums of ||W x||_1 or TV(x)
+// into d_part (*_parts doubles), then, once they are in host memory, add them in the
+// order ctprj_wavelet_l1 / ctprj_tv_value use.
+long long wavelet_l1_parts(const ctprj_wavelet* w);
+int wavelet_l1_queue(ctprj_wavelet* w, const float* d_x, double* d_part, hipStream_t st);
+double wavelet_l1_total(const ctprj_wavelet* w, const double* part);
+long long tv_value_parts(const ctprj_tv* w);
+int tv_value_queue(ctprj_tv* w, const float* d_x, double* d_part, hipStream_t st);
+double tv_value_total(const ctprj_tv* w, const double* part);
 
 }  // namespace ctprj

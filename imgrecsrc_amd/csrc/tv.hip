@@ -236,22 +236,6 @@ struct ctprj_tv {
     }
 };
 
-namespace {
-
-int tv_sums(ctprj_tv* w, hipStream_t st, double s[4])
-{
-    hipError_t e = hipMemcpyAsync(w->part.data(), w->d_part, sizeof(double) * 4 * w->blocks,
-                                  hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "tv reduction");
-    for (int k = 0; k < 4; ++k) s[k] = 0.0;
-    for (long long b = 0; b < w->blocks; ++b)
-        for (int k = 0; k < 4; ++k) s[k] += w->part[4 * b + k];
-    return 0;
-}
-
-}  // namespace
-
 long long ctprj::tv_elems(const ctprj_tv* w) { return w ? w->n : 0; }
 
 extern "C" int ctprj_tv_create(int rows, int cols, int batch, int iso, ctprj_tv** out)
@@ -302,18 +286,34 @@ extern "C" int ctprj_tv_destroy(ctprj_tv* w)
     return 0;
 }
 
+long long ctprj::tv_value_parts(const ctprj_tv* w) { return 4 * w->blocks; }
+
+int ctprj::tv_value_queue(ctprj_tv* w, const float* d_x, double* d_part, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tv_value, dim3((unsigned)w->blocks), dim3(kTvBlock), 0, st, d_x, w->rows,
+                       w->cols, w->n, w->iso, w->d_mask, d_part);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "tv value launch");
+}
+
+double ctprj::tv_value_total(const ctprj_tv* w, const double* part)
+{
+    double s = 0.0;
+    for (long long b = 0; b < w->blocks; ++b) s += part[4 * b];
+    return s;
+}
+
 extern "C" int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* stream)
 {
     if (!w || !d_x || !tv) return fail(CTPRJ_EINVAL, "tv_value: NULL argument");
     hipStream_t st = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_tv_value, dim3((unsigned)w->blocks), dim3(kTvBlock), 0, st, d_x, w->rows,
-                       w->cols, w->n, w->iso, w->d_mask, w->d_part);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_fail(e, "tv value launch");
-    double s[4];
-    int rc = tv_sums(w, st, s);
+    int rc = tv_value_queue(w, d_x, w->d_part, st);
     if (rc) return rc;
-    *tv = s[0];
+    hipError_t e = hipMemcpyAsync(w->part.data(), w->d_part, sizeof(double) * 4 * w->blocks,
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "tv value");
+    *tv = tv_value_total(w, w->part.data());
     return 0;
 }
 
@@ -334,7 +334,11 @@ extern "C" int ctprj_tv_denoise(ctprj_tv* w, const float* d_x, double lambda, do
     // iterations in chunks of kTvChunk, the host reading the state after each chunk
     constexpr int kTvChunk = 4;
     TvCtl h{};
-    for (;;) {
+    // the stop comes after at most maxIter iterations
+    const long long max_chunks = (long long)maxIter / kTvChunk + 2;
+    for (long long chunk = 0;; ++chunk) {
+        if (chunk == max_chunks)
+            return fail(CTPRJ_EHIP, "tv_denoise: the loop state did not stop");
         for (int k = 0; k < kTvChunk; ++k) {
             hipLaunchKernelGGL(k_tv_primal, dim3(g), dim3(kTvBlock), 0, st, w->d_X, w->d_D,
                                w->d_R1, w->d_R2, w->rows, w->cols, n, lambda, w->d_part, C);

@@ -812,23 +812,6 @@ int idwt(ctprj_wavelet* w, double* C, double* dst, double* t1, hipStream_t st,
     return last("idwt launch");
 }
 
-// the two partial sums of the last reduction launch, summed on the host in order
-int fetch2(ctprj_wavelet* w, hipStream_t st, double* a, double* b)
-{
-    hipError_t e = hipMemcpyAsync(w->part.data(), w->d_part, sizeof(double) * 2 * w->blocks,
-                                  hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
-    if (e != hipSuccess) return hip_fail(e, "wavelet reduction");
-    double sa = 0.0, sb = 0.0;
-    for (long long i = 0; i < w->blocks; ++i) {
-        sa += w->part[2 * i];
-        sb += w->part[2 * i + 1];
-    }
-    *a = sa;
-    *b = sb;
-    return 0;
-}
-
 }  // namespace
 
 extern "C" int ctprj_wavelet_create(int rows, int cols, int batch, const double* h, int lh, int L,
@@ -935,19 +918,38 @@ extern "C" int ctprj_idwt(ctprj_wavelet* w, const float* d_y, float* d_x, void* 
     return last("idwt output");
 }
 
-extern "C" int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, void* stream)
+long long ctprj::wavelet_l1_parts(const ctprj_wavelet* w) { return 2 * w->blocks; }
+
+int ctprj::wavelet_l1_queue(ctprj_wavelet* w, const float* d_x, double* d_part, hipStream_t st)
 {
-    if (!w || !d_x || !l1) return fail(CTPRJ_EINVAL, "wavelet_l1: NULL argument");
-    hipStream_t st = (hipStream_t)stream;
     hipLaunchKernelGGL(k_f32_to_f64, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, d_x, w->d_x, w->n, 1.0,
                        w->d_mask);
     int rc = dwt(w, w->d_x, w->d_y, w->d_pa, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_norms, dim3(grid1(w->n)), dim3(kWvBlock), 0, st, w->d_y, w->n, w->d_cmask,
-                       w->d_part);
-    if ((rc = last("wavelet l1 launch"))) return rc;
-    double sq;
-    return fetch2(w, st, l1, &sq);
+                       d_part);
+    return last("wavelet l1 launch");
+}
+
+double ctprj::wavelet_l1_total(const ctprj_wavelet* w, const double* part)
+{
+    double sa = 0.0;
+    for (long long i = 0; i < w->blocks; ++i) sa += part[2 * i];
+    return sa;
+}
+
+extern "C" int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, void* stream)
+{
+    if (!w || !d_x || !l1) return fail(CTPRJ_EINVAL, "wavelet_l1: NULL argument");
+    hipStream_t st = (hipStream_t)stream;
+    int rc = wavelet_l1_queue(w, d_x, w->d_part, st);
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(w->part.data(), w->d_part, sizeof(double) * 2 * w->blocks,
+                                  hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "wavelet l1");
+    *l1 = wavelet_l1_total(w, w->part.data());
+    return 0;
 }
 
 extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u,
@@ -982,7 +984,11 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
     constexpr int kAdmmChunk = 3;
     const AlphaOut aout{w->d_a, w->d_mask, C};
     AdmmCtl h{};
-    for (;;) {
+    // the stop comes after at most maxItr + 1 iterations
+    const long long max_chunks = ((long long)maxItr + 1) / kAdmmChunk + 2;
+    for (long long chunk = 0;; ++chunk) {
+        if (chunk == max_chunks)
+            return fail(CTPRJ_EHIP, "admm: the loop state did not stop");
         for (int k = 0; k < kAdmmChunk; ++k) {
             // alpha = max((a + rho Psi(s + nu)) / (1 + rho), 0) in d_pa
             hipLaunchKernelGGL(k_admm_sum, dim3(g), dim3(kWvBlock), 0, st, w->d_s, w->d_nu, w->d_x,
