@@ -318,12 +318,15 @@ __global__ __launch_bounds__(kWvBlock) void k_norms(const double* __restrict__ x
 // once per chunk of iterations.  Kernels queued after the loop has stopped return at
 // once.
 struct AdmmCtl {
-    double rho;    // penalty parameter
-    double f;      // factor nu takes at the next iteration's start (rho change), else 1
-    double scale;  // ||a||
-    double inv;    // 1 / scale
-    double u;      // the threshold weight divided by scale
+    double rho;     // penalty parameter
+    double f;       // factor nu takes at the next iteration's start (rho change), else 1
+    double scale;   // ||a||
+    double inv;     // 1 / scale
+    double u;       // the threshold weight divided by scale
+    double relTol;  // stopping tolerance of this call
     int itr, cnt, done, zero;
+    int maxItr;     // iteration bound of this call
+    int pad;
 };
 
 // sum over the blocks of one of the two per-block partials (k = 0, 1) by one work-group
@@ -348,11 +351,13 @@ __device__ double block_total(const double* __restrict__ part, long long nblk, i
 // scale = ||a|| from k_norms' partials; a zero input stops the loop before it starts
 // (the output is then zero)   (npg/admm.m:17-18)
 __global__ __launch_bounds__(kWvBlock) void k_admm_init(const double* __restrict__ part,
-                                                        long long nblk, double u,
-                                                        AdmmCtl* __restrict__ c)
+                                                        long long nblk, double u, double relTol,
+                                                        int maxItr, AdmmCtl* __restrict__ c)
 {
     const double sq = block_total(part, nblk, 1);
     if (threadIdx.x == 0) {
+        c->relTol = relTol;
+        c->maxItr = maxItr;
         const double scale = sqrt(sq);
         c->scale = scale;
         c->inv = 1.0 / scale;
@@ -463,10 +468,11 @@ __global__ __launch_bounds__(kWvBlock) void k_admm_shrink(const double* __restri
 // iterations without a change, rho halves (nu doubles) when difS > 10 residual and
 // doubles (nu halves) when difS < residual / 10.
 __global__ __launch_bounds__(kWvBlock) void k_admm_ctrl(const double* __restrict__ part,
-                                                        long long nblk, double relTol,
-                                                        int maxItr, AdmmCtl* __restrict__ c)
+                                                        long long nblk, AdmmCtl* __restrict__ c)
 {
     if (c->done) return;
+    const double relTol = c->relTol;
+    const int maxItr = c->maxItr;
     const double ds2 = block_total(part, nblk, 0);
     const double rs2 = block_total(part, nblk, 1);
     if (threadIdx.x != 0) return;
@@ -661,11 +667,15 @@ struct ctprj_wavelet {
     double* d_part = nullptr;
     AdmmCtl* d_ctl = nullptr;  // the prox loop's state
     float* d_mask = nullptr;   // image support (NULL: every pixel)
+    hipStream_t cap_stream = nullptr;        // where the ADMM chunk is captured
+    hipGraphExec_t chunk_exec = nullptr;     // the captured chunk (rebuilt with new masks)
     float* d_cmask = nullptr;  // coefficient support (NULL: every coefficient)
     long long blocks = 0;
     std::vector<double> part;
     ~ctprj_wavelet()
     {
+        if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+        if (cap_stream) (void)hipStreamDestroy(cap_stream);
         for (void* p : {(void*)d_x, (void*)d_y, (void*)d_a, (void*)d_s, (void*)d_nu,
                         (void*)d_pre, (void*)d_pa, (void*)d_part, (void*)d_ctl, (void*)d_mask,
                         (void*)d_cmask})
@@ -872,6 +882,10 @@ extern "C" int ctprj_wavelet_set_masks(ctprj_wavelet* w, const float* d_mask,
                                        const float* d_cmask)
 {
     if (!w) return fail(CTPRJ_EINVAL, "wavelet_set_masks: NULL context");
+    if (w->chunk_exec) {  // it holds the old masks' addresses
+        (void)hipGraphExecDestroy(w->chunk_exec);
+        w->chunk_exec = nullptr;
+    }
     const size_t bytes = sizeof(float) * (size_t)w->n;
     float** dst[2] = {&w->d_mask, &w->d_cmask};
     const float* src[2] = {d_mask, d_cmask};
@@ -952,6 +966,56 @@ extern "C" int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, 
     return 0;
 }
 
+namespace {
+
+constexpr int kAdmmChunk = 3;  // ADMM iterations between two looks at the loop state
+
+// kAdmmChunk iterations of npg/admm.m:25-50 on stream st.  Everything they read and
+// write is held by the context (the call's tolerance and bound are in the loop state),
+// so the sequence is captured once as a graph and replayed.
+int admm_chunk(ctprj_wavelet* w, hipStream_t st)
+{
+    const long long n = w->n;
+    const unsigned g = grid1(n);
+    AdmmCtl* C = w->d_ctl;
+    const int* stop = &C->done;
+    const AlphaOut aout{w->d_a, w->d_mask, C};
+    int rc;
+    for (int k = 0; k < kAdmmChunk; ++k) {
+        // alpha = max((a + rho Psi(s + nu)) / (1 + rho), 0) in d_pa
+        hipLaunchKernelGGL(k_admm_sum, dim3(g), dim3(kWvBlock), 0, st, w->d_s, w->d_nu, w->d_x, n,
+                           C, stop);
+        if ((rc = idwt(w, w->d_x, w->d_pa, w->d_y, st, stop, &aout))) return rc;
+        // Psit(alpha) in d_x, then the shrink and the stopping test
+        if ((rc = dwt(w, w->d_pa, w->d_x, w->d_y, st, stop))) return rc;
+        hipLaunchKernelGGL(k_admm_shrink, dim3(g), dim3(kWvBlock), 0, st, w->d_x, w->d_s, w->d_nu,
+                           w->d_pre, n, w->d_cmask, w->d_part, C, stop);
+        hipLaunchKernelGGL(k_admm_ctrl, dim3(1), dim3(kWvBlock), 0, st, w->d_part, w->blocks, C);
+        if ((rc = last("admm launch"))) return rc;
+    }
+    return 0;
+}
+
+// The chunk as an instantiated graph, captured on the context's own stream (default
+// priority) at first use.
+int admm_graph(ctprj_wavelet* w)
+{
+    if (w->chunk_exec) return 0;
+    hipError_t e = hipSuccess;
+    if (!w->cap_stream) e = hipStreamCreateWithFlags(&w->cap_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamBeginCapture(w->cap_stream, hipStreamCaptureModeRelaxed);
+    if (e != hipSuccess) return hip_fail(e, "admm capture");
+    const int rc = admm_chunk(w, w->cap_stream);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(w->cap_stream, &graph);
+    if (!rc && e == hipSuccess) e = hipGraphInstantiate(&w->chunk_exec, graph, nullptr, nullptr, 0);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (rc) return rc;
+    return e == hipSuccess ? 0 : hip_fail(e, "admm graph");
+}
+
+}  // namespace
+
 extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u,
                                        double relTol, int maxItr, const float* d_init,
                                        float* d_out, int* iterations, void* stream)
@@ -968,7 +1032,8 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
     // scale = ||a||, a /= scale, u /= scale  (npg/admm.m:17-18)
     hipLaunchKernelGGL(k_f32_to_f64, dim3(g), dim3(kWvBlock), 0, st, d_a, w->d_a, n, 1.0, w->d_mask);
     hipLaunchKernelGGL(k_norms, dim3(g), dim3(kWvBlock), 0, st, w->d_a, n, nullptr, w->d_part);
-    hipLaunchKernelGGL(k_admm_init, dim3(1), dim3(kWvBlock), 0, st, w->d_part, w->blocks, u, C);
+    hipLaunchKernelGGL(k_admm_init, dim3(1), dim3(kWvBlock), 0, st, w->d_part, w->blocks, u, relTol,
+                       maxItr, C);
     hipLaunchKernelGGL(k_admm_scale_in, dim3(g), dim3(kWvBlock), 0, st, w->d_a, n, C);
     // preS = s = Psit(init / scale), nu = 0
     hipLaunchKernelGGL(k_admm_load, dim3(g), dim3(kWvBlock), 0, st, d_init ? d_init : d_a,
@@ -980,29 +1045,18 @@ extern "C" int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, doubl
                            w->d_cmask);
     hipLaunchKernelGGL(k_admm_start, dim3(g), dim3(kWvBlock), 0, st, w->d_pre, w->d_s, w->d_nu, n);
     if ((rc = last("admm init launch"))) return rc;
-    // iterations in chunks of kAdmmChunk, the host reading the state after each chunk
-    constexpr int kAdmmChunk = 3;
-    const AlphaOut aout{w->d_a, w->d_mask, C};
+    // iterations in chunks of kAdmmChunk (one graph launch), the host reading the state
+    // after each chunk
+    if ((rc = admm_graph(w))) return rc;
     AdmmCtl h{};
     // the stop comes after at most maxItr + 1 iterations
     const long long max_chunks = ((long long)maxItr + 1) / kAdmmChunk + 2;
     for (long long chunk = 0;; ++chunk) {
         if (chunk == max_chunks)
             return fail(CTPRJ_EHIP, "admm: the loop state did not stop");
-        for (int k = 0; k < kAdmmChunk; ++k) {
-            // alpha = max((a + rho Psi(s + nu)) / (1 + rho), 0) in d_pa
-            hipLaunchKernelGGL(k_admm_sum, dim3(g), dim3(kWvBlock), 0, st, w->d_s, w->d_nu, w->d_x,
-                               n, C, stop);
-            if ((rc = idwt(w, w->d_x, w->d_pa, w->d_y, st, stop, &aout))) return rc;
-            // Psit(alpha) in d_x, then the shrink and the stopping test
-            if ((rc = dwt(w, w->d_pa, w->d_x, w->d_y, st, stop))) return rc;
-            hipLaunchKernelGGL(k_admm_shrink, dim3(g), dim3(kWvBlock), 0, st, w->d_x, w->d_s,
-                               w->d_nu, w->d_pre, n, w->d_cmask, w->d_part, C, stop);
-            hipLaunchKernelGGL(k_admm_ctrl, dim3(1), dim3(kWvBlock), 0, st, w->d_part, w->blocks,
-                               relTol, maxItr, C);
-            if ((rc = last("admm launch"))) return rc;
-        }
-        hipError_t e = hipMemcpyAsync(&h, C, sizeof(h), hipMemcpyDeviceToHost, st);
+        hipError_t e = hipGraphLaunch(w->chunk_exec, st);
+        if (e != hipSuccess) return hip_fail(e, "admm chunk");
+        e = hipMemcpyAsync(&h, C, sizeof(h), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "admm state");
         if (h.done) break;
