@@ -137,3 +137,30 @@ def test_masked_admm_matches_oracle_vectors():
     c = w.mdwt(x).cpu().numpy().reshape(n, n)
     assert not c[~mk].any()
     assert not w.midwt(torch.from_numpy(c).cuda().reshape(-1)).cpu().numpy().reshape(n, n)[~m].any()
+
+
+def test_admm_on_one_context_across_mask_changes():
+    """One context: an unmasked prox, then masks set, then other masks: every call
+    matches its own restatement (the replayed iteration sequence follows the masks)."""
+    import torch
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
+    h = daubcqf(4)[0]
+    L, n = 3, 128  # 128^2: one-launch levels and the LDS-resident coarse levels
+    a = np.random.default_rng(8).standard_normal((n, n)).astype(np.float32)
+    at = torch.from_numpy(a).cuda().reshape(-1)
+    w = Wavelet(n, n, h, L)
+    out, itr = w.admm(at, 1.2, 1e-7, 3000)
+    ref, itr_ref = W.admm(a.astype(np.float64), 1.2, h, L, relative_tol=1e-7, max_itr=3000)
+    assert np.abs(out.cpu().numpy().reshape(n, n) - ref).max() <= 1e-6 * np.abs(ref).max()
+    assert abs(itr - itr_ref) <= max(2, itr_ref // 50)
+    for r in (50, 37):
+        m = _disc(n, r)
+        mk = wvlt_mask(m, h, L)
+        w.set_masks(m, mk)
+        out, itr = w.admm(at, 1.2, 1e-7, 3000)
+        ref, itr_ref = W.admm(a.astype(np.float64)[m], 1.2, h, L, 1e-7, 3000,
+                              ops=W.masked_ops(h, L, m, mk))
+        got = out.cpu().numpy().reshape(n, n)
+        assert not got[~m].any()
+        assert np.abs(got[m] - ref).max() <= 1e-6 * np.abs(ref).max(), r
+        assert abs(itr - itr_ref) <= max(2, itr_ref // 50), r
