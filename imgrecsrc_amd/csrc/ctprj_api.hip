@@ -792,10 +792,12 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     const bool fwd = t->fwd_block != p->tune.fwd_block, adj = t->adj_tile != p->tune.adj_tile;
     p->tune = *t;
     int rc = 0;
-    if (tables) rc = p->build_tables();
-    else if (fwd && (rc = p->build_fwd_items())) return rc;
-    else if (adj && (rc = p->build_adj_items())) return rc;
-    if (rc) return rc;
+    if (tables) {
+        if ((rc = p->build_tables())) return rc;
+    } else {
+        if (fwd && (rc = p->build_fwd_items())) return rc;
+        if (adj && (rc = p->build_adj_items())) return rc;
+    }
     if ((rc = p->ensure_split())) {
         p->tune.adj_split = p->tune.fwd_split = 1;
         return rc;

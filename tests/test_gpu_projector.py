@@ -268,6 +268,21 @@ def test_split_work_items_match_one_work_group(torch_dev, geo, flags):
     assert torch.equal(P.adjoint(y), a1) and torch.equal(P.forward(x), f1)
 
 
+def test_set_tune_changes_several_shapes_at_once(torch_dev):
+    """One set_tune call that changes the general kernels' forward block and adjoint tile
+    and the symmetric shapes together rebuilds every work list it affects."""
+    import torch
+    from imgrecsrc_amd.phantom import ellipse_phantom, gaussian
+    for geo, flags in (((64, 95, 36, 72, 1.0, 1.0, 0.0), 0), ((72, 103, 48, 48, 1.0, 1.0, 300.0), 1)):
+        P = _proj(geo, flags)
+        x = torch.from_numpy(ellipse_phantom(geo[0], 4)).cuda().reshape(-1)
+        y = torch.from_numpy(gaussian((geo[2], geo[1]), 9)).cuda().reshape(-1)
+        f1, a1 = P.forward(x).clone(), P.adjoint(y).clone()
+        P.set_tune(fwd_block=64, adj_tile=8, adj_seg=2, fwd_seg=2, adj_split=2, fwd_split=3)
+        assert torch.allclose(P.forward(x), f1, rtol=0, atol=2e-6 * f1.abs().max().item())
+        assert torch.allclose(P.adjoint(y), a1, rtol=0, atol=2e-6 * a1.abs().max().item())
+
+
 def test_split_work_items_in_a_mixed_batch(torch_dev):
     """Slices of the three symmetry groups in one plan: each group's launches use the same
     partial-sum space in stream order."""
