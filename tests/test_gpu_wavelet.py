@@ -11,10 +11,11 @@ from oracle import wavelet_oracle as W
 pytestmark = pytest.mark.gpu
 
 # levels whose block holds at most 4096 coefficients run in one LDS-resident work-group;
-# 128^2, 256 x 64 and 256 x 128 start with one launch per level; 8 x 4096 is wider than
-# the one-launch level kernels and runs two passes per level
+# 128^2, 256 x 64 and 256 x 128 start with one launch per level (one level of 128^2: no
+# level fits in LDS); 8 x 4096 is wider than the one-launch level kernels and runs two
+# passes per level
 CASES = [(2, 3, 16, 16), (4, 3, 32, 16), (6, 4, 64, 64), (8, 2, 8, 12), (20, 2, 16, 16),
-         (6, 6, 128, 128), (4, 5, 256, 64), (2, 7, 256, 128), (4, 2, 8, 4096)]
+         (6, 6, 128, 128), (4, 5, 256, 64), (2, 7, 256, 128), (4, 2, 8, 4096), (4, 1, 128, 128)]
 
 
 def _wv(N, L, R, C, batch=1):
@@ -58,7 +59,7 @@ def test_batched_images_transform_independently(n, L):
 @pytest.mark.parametrize("N,L,shape,u,nonneg_active",
                          [(4, 3, (32, 32), 0.5, False), (6, 3, (64, 64), 2.0, True),
                           (2, 4, (32, 48), 1.0, True), (4, 4, (128, 128), 1.0, True),
-                          (2, 2, (8, 4096), 1.0, True)])
+                          (2, 2, (8, 4096), 1.0, True), (2, 1, (128, 128), 1.0, True)])
 def test_admm_matches_oracle(N, L, shape, u, nonneg_active):
     import torch
     w, h = _wv(N, L, *shape)
