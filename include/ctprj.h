@@ -289,8 +289,10 @@ int ctprj_wavelet_l1(ctprj_wavelet* w, const float* d_x, double* l1, void* strea
  * (scaled by ||a||, adaptive penalty rho, stopping when both the change of the split
  * variable and the residual are at most min(relTol, 1e-3), or after maxItr + 1
  * iterations); d_init (may be NULL: a) starts the split variable.  *iterations gets the
- * iteration count (innerSearch).  Host-driven: one small device-to-host copy per
- * iteration. */
+ * iteration count (innerSearch).  The loop state stays on the device; three iterations
+ * are one HIP graph (captured on the context's own stream at the first call, again
+ * after new masks) and the host reads the state once per three iterations.  Calls on one
+ * context share its buffers: they must not overlap on different streams. */
 int ctprj_prox_wavelet_admm(ctprj_wavelet* w, const float* d_a, double u, double relTol,
                             int maxItr, const float* d_init, float* d_out, int* iterations,
                             void* stream);
@@ -314,7 +316,7 @@ int ctprj_tv_value(ctprj_tv* w, const float* d_x, double* tv, void* stream);
  * maxIter iterations or when ||D - Dold|| / ||D|| < epsilon five times in a row; the
  * dual starts at zero (TV.denoise does not forward its init).  Norms and the stopping
  * test run over all images of the context together.  *iterations gets the count.
- * Host-driven: one small device-to-host copy per iteration. */
+ * The loop state stays on the device; the host reads it once per four iterations. */
 int ctprj_tv_denoise(ctprj_tv* w, const float* d_a, double lambda, double epsilon,
                      int maxIter, float* d_out, int* iterations, void* stream);
 
