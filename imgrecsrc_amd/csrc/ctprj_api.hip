@@ -786,8 +786,8 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
     if (t->flags & ~31) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
-    if (t->adj_split < 1 || t->adj_split > 4 || t->fwd_split < 1 || t->fwd_split > 4)
-        return fail(CTPRJ_EINVAL, "adj_split and fwd_split must be 1, 2, 3 or 4");
+    if (t->adj_split < 1 || t->adj_split > 8 || t->fwd_split < 1 || t->fwd_split > 8)
+        return fail(CTPRJ_EINVAL, "adj_split and fwd_split must be 1 to 8");
     const bool tables = (t->flags != p->tune.flags);
     const bool fwd = t->fwd_block != p->tune.fwd_block, adj = t->adj_tile != p->tune.adj_tile;
     p->tune = *t;
@@ -916,9 +916,11 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     auto search_split = [&](int ctprj_tune::*seg, int ctprj_tune::*split, const int* segs,
                             int ns, bool fwd, int nitems) -> int {
         float best_ms = 1e30f;
-        const int max_split = nitems < 8192 ? 4 : 1;
+        const int splits[7] = {1, 2, 3, 4, 5, 6, 8};
+        const int nsplits = nitems < 8192 ? 7 : 1;
         for (int i = 0; i < ns; ++i)
-            for (int k = 1; k <= max_split; ++k) {
+            for (int j = 0; j < nsplits; ++j) {
+                const int k = splits[j];
                 ctprj_tune t = best;
                 t.*seg = segs[i];
                 t.*split = k;

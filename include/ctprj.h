@@ -119,11 +119,11 @@ typedef struct ctprj_tune {
                         splitting the views: 1, 2, 4 or 8 */
     int fwd_seg;     /* symmetric forward: waves per 64 canonical rays splitting the
                         image rows: 1, 2, 4, 8 or 16 */
-    int adj_split;   /* symmetric adjoint: work-groups per tile splitting the views, 1-4;
+    int adj_split;   /* symmetric adjoint: work-groups per tile splitting the views, 1-8;
                         above 1 their partial sums are added in a fixed order by a second
                         launch (more work-groups for small slices) */
     int fwd_split;   /* symmetric forward: work-groups per 64 canonical rays splitting the
-                        image rows, 1-4, the same way */
+                        image rows, 1-8, the same way */
 } ctprj_tune;
 int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
 int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);
