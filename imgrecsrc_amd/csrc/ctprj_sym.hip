@@ -363,15 +363,16 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
                                        split, nsplit, mine);
 }
 
-// The nsplit partial sums of each canonical ray of an item, added in split order and
-// written like the one-work-group path (64 threads, one ray each).
+// The nsplit partial sums of each canonical ray and image of an item, added in split
+// order and written like the one-work-group path (512 threads: image e = thread / 64 of
+// ray lane = thread % 64).
 template <int GRP>
-__global__ __launch_bounds__(64) void k_forward_sym_combine(const SliceDesc* __restrict__ slices,
-                                                            const FwdItem* __restrict__ items,
-                                                            const float4* __restrict__ views,
-                                                            int nitems, int nsplit,
-                                                            const float* __restrict__ part,
-                                                            float* __restrict__ sino)
+__global__ __launch_bounds__(512) void k_forward_sym_combine(const SliceDesc* __restrict__ slices,
+                                                             const FwdItem* __restrict__ items,
+                                                             const float4* __restrict__ views,
+                                                             int nitems, int nsplit,
+                                                             const float* __restrict__ part,
+                                                             float* __restrict__ sino)
 {
     const FwdItem it = items[blockIdx.x];
     if (it.slice < 0) return;
@@ -379,16 +380,12 @@ __global__ __launch_bounds__(64) void k_forward_sym_combine(const SliceDesc* __r
     const int a0 = __float_as_int(views[S.view_off + it.view].z);
     constexpr GroupMasks GM = group_masks(GRP);
     const unsigned mask = (a0 == 0 || GM.q * a0 == S.P) ? GM.rot : GM.all;
-    const int lane = threadIdx.x;
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        if (!(mask >> e & 1u)) continue;
-        float t = 0.0f;
-        for (int k = 0; k < nsplit; ++k)
-            t = k ? t + part[((long long)k * nitems + blockIdx.x) * 512 + e * 64 + lane]
-                  : part[(long long)blockIdx.x * 512 + e * 64 + lane];
-        fwd_sym_store(S, it, a0, e, lane, t, sino);
-    }
+    const int e = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
+    if (!(mask >> e & 1u)) return;
+    const float* p = part + (long long)blockIdx.x * 512 + threadIdx.x;
+    float t = p[0];
+    for (int k = 1; k < nsplit; ++k) t += p[(long long)k * nitems * 512];
+    fwd_sym_store(S, it, a0, e, lane, t, sino);
 }
 
 // ------------------------------------------------------------------ adjoint ---
@@ -742,30 +739,26 @@ __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __res
 }
 
 // The nsplit partial sums of each orbit image of a tile, added in split order and
-// written like the one-work-group path (64 threads, one orbit base pixel each).
+// written like the one-work-group path (512 threads: orbit image h = thread / 64 of base
+// pixel lane = thread % 64).
 template <bool FBP, int GRP>
-__global__ __launch_bounds__(64) void k_adjoint_sym_combine(const SliceDesc* __restrict__ slices,
-                                                            const AdjItem* __restrict__ items,
-                                                            int nitems, int nsplit,
-                                                            const float* __restrict__ part,
-                                                            float* __restrict__ img)
+__global__ __launch_bounds__(512) void k_adjoint_sym_combine(const SliceDesc* __restrict__ slices,
+                                                             const AdjItem* __restrict__ items,
+                                                             int nitems, int nsplit,
+                                                             const float* __restrict__ part,
+                                                             float* __restrict__ img)
 {
     const AdjItem it = items[blockIdx.x];
     if (it.slice < 0) return;
     const SliceDesc S = slices[it.slice];
     constexpr GroupMasks G = group_masks(GRP);
-    const int lane = threadIdx.x;
+    const int h = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     int x0, y0;
-    if (!adj_sym_base<GRP>(S, it, lane, x0, y0)) return;
-#pragma unroll
-    for (int h = 0; h < 8; ++h) {
-        if (!(G.all >> h & 1u)) continue;
-        float t = 0.0f;
-        for (int k = 0; k < nsplit; ++k)
-            t = k ? t + part[((long long)k * nitems + blockIdx.x) * 512 + h * 64 + lane]
-                  : part[(long long)blockIdx.x * 512 + h * 64 + lane];
-        adj_sym_store<FBP, GRP>(S, x0, y0, h, t, img);
-    }
+    if (!(G.all >> h & 1u) || !adj_sym_base<GRP>(S, it, lane, x0, y0)) return;
+    const float* p = part + (long long)blockIdx.x * 512 + threadIdx.x;
+    float t = p[0];
+    for (int k = 1; k < nsplit; ++k) t += p[(long long)k * nitems * 512];
+    adj_sym_store<FBP, GRP>(S, x0, y0, h, t, img);
 }
 
 // Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the
@@ -803,7 +796,7 @@ static void launch_forward_sym_g(int seg, int nsplit, float* part, int nitems,
     }
 #undef CTPRJ_LAUNCH
     if (pp)
-        hipLaunchKernelGGL((k_forward_sym_combine<GRP>), dim3((unsigned)nitems), dim3(64), 0, st,
+        hipLaunchKernelGGL((k_forward_sym_combine<GRP>), dim3((unsigned)nitems), dim3(512), 0, st,
                            desc, items, views, nitems, nsplit, pp, sino);
 }
 
@@ -842,7 +835,7 @@ static void launch_adjoint_sym_t(int seg, int nsplit, float* part, int nitems,
     }
 #undef CTPRJ_LAUNCH
     if (pp)
-        hipLaunchKernelGGL((k_adjoint_sym_combine<FBP, GRP>), dim3((unsigned)nitems), dim3(64), 0,
+        hipLaunchKernelGGL((k_adjoint_sym_combine<FBP, GRP>), dim3((unsigned)nitems), dim3(512), 0,
                            st, desc, items, nitems, nsplit, pp, img);
 }
 
