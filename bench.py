@@ -333,7 +333,7 @@ def main():
                 "algorithmic_bytes": alg_bytes,
                 "other_kernel_ms": round(t_fwd if dom_name.startswith("k_adjoint") else t_adj, 4),
                 "launches": (["k_adjoint_sym"] + (["k_adjoint_sym_combine"] if tune["adj_split"] > 1
-                                                  else []) + (["k_zero_border"] if n % 2 == 0 else [])
+                                                  else [])
                              if dom_name.startswith("k_adjoint") else
                              ["k_transpose", "k_forward_sym"] + (["k_forward_sym_combine"]
                                                                  if tune["fwd_split"] > 1 else []))}

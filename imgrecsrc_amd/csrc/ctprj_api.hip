@@ -147,7 +147,6 @@ Plan::~Plan()
     free_ptr(d_syma_items);
     free_ptr(d_tr_items);
     free_ptr(d_fbp_items);
-    free_ptr(d_border_ids);
     free_ptr(d_zero_row);
     free_ptr(d_symtab);
     free_ptr(d_fbp_kern);
@@ -319,11 +318,10 @@ int Plan::build_tables()
     free_ptr(d_symtab);
     free_ptr(d_views);
     free_ptr(d_rays);
-    free_ptr(d_border_ids);
     std::vector<SliceDesc> desc(nslices);
     std::vector<float4> views, bins;
     std::vector<long long> bin_off(nslices), ray_prefix(nslices + 1, 0);
-    std::vector<int> border, symtab;
+    std::vector<int> symtab;
     long long nview = 0, nray = 0;
     slice_sym.assign(nslices, 0);
     n_sym = 0;
@@ -391,7 +389,6 @@ int Plan::build_tables()
             }
             slice_sym[s] = grp;
             ++n_sym;
-            if (g.n % 2 == 0) border.push_back(s);
         } else {
             for (int a = 0; a < g.np; ++a) {
                 float c, si;
@@ -414,7 +411,6 @@ int Plan::build_tables()
         }
         ray_prefix[s + 1] = nray;
     }
-    n_border = (int)border.size();
     // the transposed image serves the general forward's steep views, D4's odd rotations
     // and the column walks of V4 / Z2 canonical views beyond 45 degrees
     std::vector<AdjItem> tr;
@@ -430,7 +426,6 @@ int Plan::build_tables()
     if ((rc = upload(&d_desc, desc, "slice table")) || (rc = upload(&d_views, views, "view table")) ||
         (rc = upload(&d_bins, bins, "bin table")) || (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
         (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
-        (rc = upload(&d_border_ids, border, "border list")) ||
         (rc = upload(&d_symtab, symtab, "symmetric view-row table")) ||
         (rc = upload(&d_tr_items, tr, "transpose work list")) ||
         (rc = dev_alloc((void**)&d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
@@ -506,8 +501,10 @@ int Plan::build_adj_items()
         const int n = geoms[s].n;
         const int grp = slice_sym[s];
         if (grp) {
-            // 8x8 tiles of the group's fundamental domain of orbit base pixels
+            // 8x8 tiles of the group's fundamental domain of orbit base pixels; with even
+            // n the first tile's work-group also zeroes row 0 and column 0 (pad = 1)
             const int H = (n - 1) / 2, nt = (H + 1 + 7) / 8;
+            const size_t first = by[grp].size();
             if (grp == kGrpD4) {
                 for (int ty = 0; ty < nt; ++ty)
                     for (int tx = ty; tx < nt; ++tx) by[grp].push_back(AdjItem{s, ty * 8, tx * 8, 0});
@@ -518,6 +515,7 @@ int Plan::build_adj_items()
                 for (int y = -H; y <= H; y += 8)
                     for (int tx = 0; tx < nt; ++tx) by[grp].push_back(AdjItem{s, y, tx * 8, 0});
             }
+            if (n % 2 == 0 && by[grp].size() > first) by[grp][first].pad = 1;
         } else {
             for (int r = 0; r < n; r += T)
                 for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
@@ -688,7 +686,6 @@ int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool
                 launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4, tune.adj_split,
                                    d_split, syma_count[g], d_desc, d_syma_items + syma_start[g],
                                    d_views, d_rays, d_sino, d_zero_row, d_symtab, d_img, st);
-        launch_zero_border(n_border, d_desc, d_border_ids, d_img, st);
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "adjoint launch");

@@ -42,7 +42,6 @@ struct Plan {
     AdjItem* d_syma_items = nullptr;  // symmetric adjoint
     AdjItem* d_tr_items = nullptr;
     int2* d_fbp_items = nullptr;
-    int* d_border_ids = nullptr;      // symmetric slices of even n
     float* d_zero_row = nullptr;      // max_W zeros: reads of views beyond np
     int* d_symtab = nullptr;          // view-row table of the symmetric adjoint
     float* d_fbp_kern = nullptr;
@@ -54,7 +53,7 @@ struct Plan {
     // group g (1..3) in d_symf_items / d_syma_items
     int symf_start[4] = {0, 0, 0, 0}, symf_count[4] = {0, 0, 0, 0};
     int syma_start[4] = {0, 0, 0, 0}, syma_count[4] = {0, 0, 0, 0};
-    int n_tr_items = 0, n_fbp_items = 0, n_border = 0, n_sym = 0;
+    int n_tr_items = 0, n_fbp_items = 0, n_sym = 0;
 
     float* d_imgT = nullptr;     // transposed image (forward, steep views / symmetric images)
     float* d_sino_ws = nullptr;  // sinogram workspace (FBP filter output, BHC ray values)
@@ -105,8 +104,6 @@ void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* 
                         const SliceDesc* desc, const AdjItem* items, const float4* views,
                         const Ray* rays, const float* sino, const float* zero_row,
                         const int* symtab, float* img, hipStream_t st);
-void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
-                        hipStream_t st);
 
 // Images held by a wavelet context (rows * cols * batch).
 long long wavelet_elems(const ctprj_wavelet* w);

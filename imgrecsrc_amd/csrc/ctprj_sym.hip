@@ -629,6 +629,18 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
     }
 }
 
+// Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the adjoint
+// writes them as 0 (prj/cpuPrj.c:713-716).  The work-group that stores a slice's marked
+// tile (AdjItem.pad) does it; no orbit pixel lies there.
+__device__ __forceinline__ void sym_zero_border(const SliceDesc& S, float* __restrict__ img)
+{
+    float* out = img + S.img_off;
+    for (int i = threadIdx.x; i < S.n; i += blockDim.x) {
+        out[i] = 0.0f;
+        out[(long long)i * S.n] = 0.0f;
+    }
+}
+
 // Orbit base pixels of a work item: an 8x8 tile of the fundamental domain of the group,
 //   D4: 0 <= y0 <= x0 <= H,  V4: 0 <= x0, y0 <= H,  Z2: 0 <= x0 <= H, -H <= y0 <= H
 // (tiles start inside it; the upper ends are checked here).
@@ -696,6 +708,7 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
             if (G.all >> h & 1u) red[wave][h][lane] = acc[h].s;
         __syncthreads();
     }
+    if (!part && it.pad) sym_zero_border(S, img);
     if (!valid) return;
     // wave w finalises the elements h = w (mod SEG): the sum of the waves' partials in
     // wave order
@@ -752,6 +765,7 @@ __global__ __launch_bounds__(512) void k_adjoint_sym_combine(const SliceDesc* __
     if (it.slice < 0) return;
     const SliceDesc S = slices[it.slice];
     constexpr GroupMasks G = group_masks(GRP);
+    if (it.pad) sym_zero_border(S, img);
     const int h = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
     int x0, y0;
     if (!(G.all >> h & 1u) || !adj_sym_base<GRP>(S, it, lane, x0, y0)) return;
@@ -761,18 +775,6 @@ __global__ __launch_bounds__(512) void k_adjoint_sym_combine(const SliceDesc* __
     adj_sym_store<FBP, GRP>(S, x0, y0, h, t, img);
 }
 
-// Even n: row 0 and column 0 lie outside the pixel grid the projector reads; the
-// adjoint writes them as 0 (prj/cpuPrj.c:713-716).
-__global__ void k_zero_border(const SliceDesc* __restrict__ slices, const int* __restrict__ ids,
-                              float* __restrict__ img)
-{
-    const SliceDesc S = slices[ids[blockIdx.x]];
-    float* out = img + S.img_off;
-    for (int i = threadIdx.x; i < S.n; i += blockDim.x) {
-        out[i] = 0.0f;
-        out[(long long)i * S.n] = 0.0f;
-    }
-}
 
 // ----------------------------------------------------------------- launchers ---
 
@@ -875,10 +877,5 @@ void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* 
                                        sino, zero_row, symtab, img, st);
 }
 
-void launch_zero_border(int nids, const SliceDesc* desc, const int* ids, float* img,
-                        hipStream_t st)
-{
-    if (nids > 0) hipLaunchKernelGGL(k_zero_border, dim3(nids), dim3(256), 0, st, desc, ids, img);
-}
 
 }  // namespace ctprj
