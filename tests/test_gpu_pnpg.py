@@ -86,12 +86,17 @@ def test_pnpg_matches_fp64_restatement(problem, oracle_mod, method):
     assert np.linalg.norm(a - ref.alpha) <= 1e-4 * np.linalg.norm(ref.alpha)
 
 
-def test_pnpg_graph_replay_matches_eager(problem):
+@pytest.mark.parametrize("splits", [(1, 1), (3, 2)])
+def test_pnpg_graph_replay_matches_eager(problem, splits):
+    """Captured trials replayed R times, with eager work between the replays, equal the
+    eager trials bit for bit; (3, 2): the projector's split work items and their combine
+    launches inside the graph."""
     import torch
     from imgrecsrc_amd.pnpg import PNPG
     from imgrecsrc_amd.projector import Projector
     P = problem
     proj = Projector(GEOMS)
+    proj.set_tune(adj_split=splits[0], fwd_split=splits[1])
     a_eager = torch.from_numpy(P["alpha0"].astype(np.float32)).cuda()
     a_graph = a_eager.clone()
     s_eager = PNPG(proj, P["Imea_t"], P["knots"], P["Ie"], P["Ts"], a_eager, t0=50.0)
