@@ -165,10 +165,10 @@ class Projector:
     def set_tune(self, fwd_block=None, adj_tile=None, flags=None, adj_seg=None, fwd_seg=None,
                  adj_split=None, fwd_split=None):
         cur = self.get_tune()
-        t = Tune(fwd_block or cur["fwd_block"], adj_tile or cur["adj_tile"],
-                 cur["flags"] if flags is None else flags, adj_seg or cur["adj_seg"],
-                 fwd_seg or cur["fwd_seg"], adj_split or cur["adj_split"],
-                 fwd_split or cur["fwd_split"])
+        given = {"fwd_block": fwd_block, "adj_tile": adj_tile, "flags": flags,
+                 "adj_seg": adj_seg, "fwd_seg": fwd_seg, "adj_split": adj_split,
+                 "fwd_split": fwd_split}
+        t = Tune(*[int(cur[k] if v is None else v) for k, v in given.items()])
         check(lib().ctprj_plan_set_tune(self.handle, ctypes.byref(t)))
 
     def autotune(self, reps: int = 3, cache_path: str | None = None, stream=None) -> dict:

@@ -336,6 +336,12 @@ def test_plan_errors(torch_dev):
         P.forward(torch.zeros(15, device="cuda"))
     with pytest.raises(TypeError):
         P.forward(torch.zeros(256, device="cuda", dtype=torch.float64))
+    before = P.get_tune()
+    for bad in ({"adj_split": 9}, {"fwd_split": 0}, {"adj_seg": 16}, {"fwd_seg": 32},
+                {"fwd_block": 96}, {"adj_tile": 32}, {"flags": 64}):
+        with pytest.raises(CtprjError):
+            P.set_tune(**bad)
+    assert P.get_tune() == before  # a refused shape changes nothing
 
 
 def test_autotune_roundtrip(torch_dev, tmp_path):
