@@ -450,6 +450,31 @@ def main():
             "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
         del npg, wv
 
+        # ---- the same step with the isotropic TV prior ('tviso', utils/TV.m denoise)
+        log("npg + tv leg")
+        from imgrecsrc_amd.tv import TV
+        tvp = TV(nb, nb, tv="iso")
+        tvp.set_mask(disc)
+        npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
+        npg.set_method("NPG")
+        npg.set_mask(disc)
+        npg.set_tv(tvp, 1e-3 * npg.t0)
+        inner = 0
+        npg.step(a_w)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(ntr):
+            npg.step(a_w)
+            inner += npg.stats()["innerSearch"]
+        torch.cuda.synchronize()
+        t_tv = (time.perf_counter() - t0) * 1e3 / ntr
+        st = npg.stats()
+        out["npg_tv"] = {
+            "workload": "NPG + isotropic TV prior (FGP), circle support, fan 512^2, E=20",
+            "ms_per_trial": round(t_tv, 3), "fgp_iterations_per_trial": inner / ntr,
+            "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
+        del npg, tvp
+
         # ---- 64 ragged slices: one launch per operator vs one launch per slice
         log("batched leg")
         geoms = ragged_geoms()
