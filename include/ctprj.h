@@ -126,6 +126,9 @@ typedef struct ctprj_tune {
                         image rows, 1-8, the same way */
 } ctprj_tune;
 int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
+/* Out-of-range fields are refused (CTPRJ_EINVAL) and change nothing.  Splits whose
+ * partial sums would exceed the plan's 256 MiB are refused too; the other fields then
+ * stay applied and both splits fall back to 1. */
 int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);
 /* Times every candidate shape on this plan's geometry (HIP events, `reps` runs each),
  * keeps the fastest and, if cache_path is non-NULL, records it in that text file keyed
