@@ -126,6 +126,7 @@ SIGNATURES = [
     ("ctprj_last_error", ctypes.c_char_p, []),
     ("ctprj_device_arch", _i, [ctypes.c_char_p, _i]),
     ("ctprj_version", ctypes.c_char_p, []),
+    ("ctprj_debug_fail_alloc", _i, [_i]),
 ]
 
 _lib = None

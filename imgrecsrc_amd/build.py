@@ -47,19 +47,25 @@ def _run(cmd):
             sys.stderr.write(line + "\n")
 
 
-def build(verbose: bool = False) -> str:
+def build(verbose: bool = False, jobs: int | None = None) -> str:
+    """Compiles every source (in parallel, at most `jobs` at once) and links the library."""
+    from concurrent.futures import ThreadPoolExecutor
     os.makedirs(LIBDIR, exist_ok=True)
-    objs = []
+    cmds, objs = [], []
     for src in SOURCES:
         path = os.path.join(CSRC, src)
         if not os.path.exists(path):
             continue
         obj = os.path.join(LIBDIR, src.replace(".hip", ".o"))
-        cmd = [HIPCC, *FLAGS, "-c", path, "-o", obj]
-        if verbose:
-            print(" ".join(cmd))
-        _run(cmd)
+        cmds.append([HIPCC, *FLAGS, "-c", path, "-o", obj])
         objs.append(obj)
+    if jobs is None:
+        jobs = min(len(cmds), max(1, (os.cpu_count() or 1) // 2), 8)
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        for cmd in cmds:
+            if verbose:
+                print(" ".join(cmd))
+        list(ex.map(_run, cmds))
     tmp = LIB + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -136,19 +137,59 @@ static void free_ptr(T*& p)
     p = nullptr;
 }
 
-Plan::~Plan()
+void Tables::release()
 {
     free_ptr(d_desc);
     free_ptr(d_views);
     free_ptr(d_rays);
-    free_ptr(d_fwd_items);
-    free_ptr(d_adj_items);
-    free_ptr(d_symf_items);
-    free_ptr(d_syma_items);
+    free_ptr(d_symtab);
     free_ptr(d_tr_items);
+    free_ptr(d_fwd_items);
+    free_ptr(d_symf_items);
+    free_ptr(d_adj_items);
+    free_ptr(d_syma_items);
+}
+
+void Tables::swap_geometry(Tables& o)
+{
+    std::swap(d_desc, o.d_desc);
+    std::swap(d_views, o.d_views);
+    std::swap(d_rays, o.d_rays);
+    std::swap(d_symtab, o.d_symtab);
+    std::swap(d_tr_items, o.d_tr_items);
+    std::swap(n_tr_items, o.n_tr_items);
+    std::swap(n_sym, o.n_sym);
+    slice_sym.swap(o.slice_sym);
+}
+
+void Tables::swap_fwd(Tables& o)
+{
+    std::swap(d_fwd_items, o.d_fwd_items);
+    std::swap(d_symf_items, o.d_symf_items);
+    std::swap(n_fwd_items, o.n_fwd_items);
+    std::swap(n_symf_items, o.n_symf_items);
+    for (int g = 0; g < 4; ++g) {
+        std::swap(symf_start[g], o.symf_start[g]);
+        std::swap(symf_count[g], o.symf_count[g]);
+    }
+}
+
+void Tables::swap_adj(Tables& o)
+{
+    std::swap(d_adj_items, o.d_adj_items);
+    std::swap(d_syma_items, o.d_syma_items);
+    std::swap(n_adj_items, o.n_adj_items);
+    std::swap(n_syma_items, o.n_syma_items);
+    for (int g = 0; g < 4; ++g) {
+        std::swap(syma_start[g], o.syma_start[g]);
+        std::swap(syma_count[g], o.syma_count[g]);
+    }
+}
+
+Plan::~Plan()
+{
     free_ptr(d_fbp_items);
     free_ptr(d_zero_row);
-    free_ptr(d_symtab);
     free_ptr(d_fbp_kern);
     free_ptr(d_fbp_kern_off);
     free_ptr(d_fbp_prew);
@@ -161,14 +202,26 @@ Plan::~Plan()
     free_ptr(d_split);
 }
 
+// Failure injection for the error-path tests: when armed with k > 0, the k-th device
+// allocation from then on fails as if the device were out of memory (once).
+static std::atomic<int> g_fail_alloc{0};
+
+static bool injected_alloc_failure()
+{
+    int k = g_fail_alloc.load();
+    while (k > 0) {
+        if (g_fail_alloc.compare_exchange_weak(k, k - 1)) return k == 1;
+    }
+    return false;
+}
+
 template <class T>
 static int upload(T** dst, const std::vector<T>& src, const char* what)
 {
-    if (src.empty()) {
-        *dst = nullptr;
-        return 0;
-    }
-    hipError_t e = hipMalloc((void**)dst, sizeof(T) * src.size());
+    *dst = nullptr;
+    if (src.empty()) return 0;
+    hipError_t e = injected_alloc_failure() ? hipErrorOutOfMemory
+                                            : hipMalloc((void**)dst, sizeof(T) * src.size());
     if (e != hipSuccess) return fail(CTPRJ_ENOMEM, "hipMalloc(%s, %zu B): %s", what,
                                      sizeof(T) * src.size(), hipGetErrorString(e));
     e = hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice);
@@ -178,7 +231,7 @@ static int upload(T** dst, const std::vector<T>& src, const char* what)
 
 static int dev_alloc(void** p, size_t bytes, const char* what)
 {
-    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    hipError_t e = injected_alloc_failure() ? hipErrorOutOfMemory : hipMalloc(p, bytes ? bytes : 16);
     if (e != hipSuccess) return fail(CTPRJ_ENOMEM, "hipMalloc(%s, %zu B): %s", what, bytes,
                                      hipGetErrorString(e));
     return 0;
@@ -312,19 +365,15 @@ static int host_map_view(int P, bool fold, int k, int m, int a0, bool* flip)
     return v;
 }
 
-int Plan::build_tables()
+int Plan::build_geometry(int flags, Tables& G) const
 {
-    free_ptr(d_desc);
-    free_ptr(d_symtab);
-    free_ptr(d_views);
-    free_ptr(d_rays);
     std::vector<SliceDesc> desc(nslices);
     std::vector<float4> views, bins;
     std::vector<long long> bin_off(nslices), ray_prefix(nslices + 1, 0);
     std::vector<int> symtab;
     long long nview = 0, nray = 0;
-    slice_sym.assign(nslices, 0);
-    n_sym = 0;
+    G.slice_sym.assign(nslices, 0);
+    G.n_sym = 0;
     for (int s = 0; s < nslices; ++s) {
         const ctprj_geom& g = geoms[s];
         SliceDesc& D = desc[s];
@@ -351,7 +400,7 @@ int Plan::build_tables()
         D.ray_off = nray;
         std::vector<int> canon;
         bool fold = false;
-        const int grp = choose_group(g, tune.flags, &fold, &canon);
+        const int grp = choose_group(g, flags, &fold, &canon);
         const bool sym = grp != kGrpNone;
         if (sym) {
             D.sym = grp;
@@ -387,8 +436,8 @@ int Plan::build_tables()
                 std::memcpy(&tag, &a0, sizeof(tag));
                 views.push_back(make_float4(c, si, tag, 0.0f));
             }
-            slice_sym[s] = grp;
-            ++n_sym;
+            G.slice_sym[s] = grp;
+            ++G.n_sym;
         } else {
             for (int a = 0; a < g.np; ++a) {
                 float c, si;
@@ -418,31 +467,31 @@ int Plan::build_tables()
         for (int r = 0; r < geoms[s].n; r += 32)
             for (int c = 0; c < geoms[s].n; c += 32) tr.push_back(AdjItem{s, r, c, 0});
     }
-    free_ptr(d_tr_items);
-    n_tr_items = (int)tr.size();
+    G.n_tr_items = (int)tr.size();
     float4* d_bins = nullptr;
     long long *d_bin_off = nullptr, *d_ray_prefix = nullptr;
     int rc;
-    if ((rc = upload(&d_desc, desc, "slice table")) || (rc = upload(&d_views, views, "view table")) ||
+    if ((rc = upload(&G.d_desc, desc, "slice table")) ||
+        (rc = upload(&G.d_views, views, "view table")) ||
         (rc = upload(&d_bins, bins, "bin table")) || (rc = upload(&d_bin_off, bin_off, "bin offsets")) ||
         (rc = upload(&d_ray_prefix, ray_prefix, "ray offsets")) ||
-        (rc = upload(&d_symtab, symtab, "symmetric view-row table")) ||
-        (rc = upload(&d_tr_items, tr, "transpose work list")) ||
-        (rc = dev_alloc((void**)&d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
+        (rc = upload(&G.d_symtab, symtab, "symmetric view-row table")) ||
+        (rc = upload(&G.d_tr_items, tr, "transpose work list")) ||
+        (rc = dev_alloc((void**)&G.d_rays, sizeof(Ray) * (size_t)nray, "ray table"))) {
         free_dev(d_bins);
         free_dev(d_bin_off);
         free_dev(d_ray_prefix);
         return rc;
     }
-    launch_build_rays(d_desc, nslices, d_views, d_bins, d_bin_off, d_ray_prefix, d_rays, nray, 0);
+    launch_build_rays(G.d_desc, nslices, G.d_views, d_bins, d_bin_off, d_ray_prefix, G.d_rays,
+                      nray, 0);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipDeviceSynchronize();
     free_dev(d_bins);
     free_dev(d_bin_off);
     free_dev(d_ray_prefix);
     if (e != hipSuccess) return hip_fail(e, "building ray table");
-    if ((rc = build_fwd_items())) return rc;
-    return build_adj_items();
+    return 0;
 }
 
 // Concatenates per-group lists (each in batch order) and records the segments.
@@ -461,45 +510,43 @@ static std::vector<Item> group_segments(std::vector<Item> (&by)[4],
     return out;
 }
 
-int Plan::build_fwd_items()
+int Plan::build_fwd_items(const ctprj_tune& t, Tables& G) const
 {
     std::vector<FwdItem> items, by[4];
-    const int B = tune.fwd_block;
+    const int B = t.fwd_block;
     for (int s = 0; s < nslices; ++s) {
         const ctprj_geom& g = geoms[s];
-        if (slice_sym[s]) {
+        if (G.slice_sym[s]) {
             const int pC = g.prjWidth / 2, bhi = pC + (g.prjWidth - 1) / 2;
             std::vector<int> canon;
             bool fold = false;
-            choose_group(g, tune.flags, &fold, &canon);
+            choose_group(g, t.flags, &fold, &canon);
             const int bmax = fold ? pC : bhi;
             for (int i = 0; i < (int)canon.size(); ++i)
                 for (int b0 = 0; b0 <= bmax; b0 += 64)
-                    by[slice_sym[s]].push_back(FwdItem{s, i, b0, 0});
+                    by[G.slice_sym[s]].push_back(FwdItem{s, i, b0, 0});
         } else {
             for (int a = 0; a < g.np; ++a)
                 for (int b0 = 0; b0 < g.prjWidth; b0 += B) items.push_back(FwdItem{s, a, b0, 0});
         }
     }
-    const bool spread = !(tune.flags & 8);
+    const bool spread = !(t.flags & 8);
     items = batch_order(items, geoms, spread);
-    std::vector<FwdItem> sym = group_segments(by, geoms, spread, symf_start, symf_count);
-    free_ptr(d_fwd_items);
-    free_ptr(d_symf_items);
-    n_fwd_items = (int)items.size();
-    n_symf_items = (int)sym.size();
-    int rc = upload(&d_fwd_items, items, "forward work list");
+    std::vector<FwdItem> sym = group_segments(by, geoms, spread, G.symf_start, G.symf_count);
+    G.n_fwd_items = (int)items.size();
+    G.n_symf_items = (int)sym.size();
+    int rc = upload(&G.d_fwd_items, items, "forward work list");
     if (rc) return rc;
-    return upload(&d_symf_items, sym, "symmetric forward work list");
+    return upload(&G.d_symf_items, sym, "symmetric forward work list");
 }
 
-int Plan::build_adj_items()
+int Plan::build_adj_items(const ctprj_tune& t, Tables& G) const
 {
     std::vector<AdjItem> items, by[4];
-    const int T = tune.adj_tile;
+    const int T = t.adj_tile;
     for (int s = 0; s < nslices; ++s) {
         const int n = geoms[s].n;
-        const int grp = slice_sym[s];
+        const int grp = G.slice_sym[s];
         if (grp) {
             // 8x8 tiles of the group's fundamental domain of orbit base pixels; with even
             // n the first tile's work-group also zeroes row 0 and column 0 (pad = 1)
@@ -521,16 +568,14 @@ int Plan::build_adj_items()
                 for (int c = 0; c < n; c += T) items.push_back(AdjItem{s, r, c, 0});
         }
     }
-    const bool spread = !(tune.flags & 8);
+    const bool spread = !(t.flags & 8);
     items = batch_order(items, geoms, spread);
-    std::vector<AdjItem> sym = group_segments(by, geoms, spread, syma_start, syma_count);
-    free_ptr(d_adj_items);
-    free_ptr(d_syma_items);
-    n_adj_items = (int)items.size();
-    n_syma_items = (int)sym.size();
-    int rc = upload(&d_adj_items, items, "adjoint work list");
+    std::vector<AdjItem> sym = group_segments(by, geoms, spread, G.syma_start, G.syma_count);
+    G.n_adj_items = (int)items.size();
+    G.n_syma_items = (int)sym.size();
+    int rc = upload(&G.d_adj_items, items, "adjoint work list");
     if (rc) return rc;
-    return upload(&d_syma_items, sym, "symmetric adjoint work list");
+    return upload(&G.d_syma_items, sym, "symmetric adjoint work list");
 }
 
 static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out);
@@ -616,7 +661,8 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
         (rc = upload(&p->d_zero_row, zeros, "zero row")) ||
         (rc = dev_alloc((void**)&p->d_imgT, sizeof(float) * (size_t)img, "transposed image")) ||
         (rc = dev_alloc((void**)&p->d_sino_ws, sizeof(float) * (size_t)sino, "sinogram workspace")) ||
-        (rc = p->build_tables())) {
+        (rc = p->build_geometry(p->tune.flags, *p)) || (rc = p->build_fwd_items(p->tune, *p)) ||
+        (rc = p->build_adj_items(p->tune, *p))) {
         delete p;
         return rc;
     }
@@ -785,47 +831,74 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
     if (t->adj_split < 1 || t->adj_split > 8 || t->fwd_split < 1 || t->fwd_split > 8)
         return fail(CTPRJ_EINVAL, "adj_split and fwd_split must be 1 to 8");
-    const bool tables = (t->flags != p->tune.flags);
-    const bool fwd = t->fwd_block != p->tune.fwd_block, adj = t->adj_tile != p->tune.adj_tile;
-    p->tune = *t;
+    // the parts this change rebuilds go into N; the plan keeps its tables until all of
+    // them (and the split partial-sum space) exist
+    ctprj_tune nt = *t;
+    const bool geo = nt.flags != p->tune.flags;
+    const bool fwd = geo || nt.fwd_block != p->tune.fwd_block;
+    const bool adj = geo || nt.adj_tile != p->tune.adj_tile;
+    Tables N;
     int rc = 0;
-    if (tables) {
-        if ((rc = p->build_tables())) return rc;
-    } else {
-        if (fwd && (rc = p->build_fwd_items())) return rc;
-        if (adj && (rc = p->build_adj_items())) return rc;
+    if (geo) rc = p->build_geometry(nt.flags, N);
+    if (geo && !rc) {
+        // the work lists read the new slice groups
+        if (!(rc = p->build_fwd_items(nt, N))) rc = p->build_adj_items(nt, N);
+    } else if (!rc) {
+        // same geometry: build the lists from the plan's slice groups
+        N.slice_sym = p->slice_sym;
+        if (fwd) rc = p->build_fwd_items(nt, N);
+        if (adj && !rc) rc = p->build_adj_items(nt, N);
     }
-    if ((rc = p->ensure_split())) {
-        p->tune.adj_split = p->tune.fwd_split = 1;
+    if (rc) return rc;  // N frees what it holds; the plan is unchanged
+    const Tables& F = fwd ? N : *static_cast<const Tables*>(p);
+    const Tables& A = adj ? N : *static_cast<const Tables*>(p);
+    int split_rc = 0;
+    size_t need = p->split_need(nt, F, A);
+    if (need > kSplitMaxFloats) {
+        split_rc = fail(CTPRJ_EINVAL, "split work items need %zu MiB of partial sums (at most %zu)",
+                        need * sizeof(float) >> 20, kSplitMaxFloats * sizeof(float) >> 20);
+        nt.adj_split = nt.fwd_split = 1;
+        need = 0;
+    }
+    float* split = nullptr;
+    if (need > p->split_cap && (rc = dev_alloc((void**)&split, sizeof(float) * need,
+                                                "split partial sums")))
         return rc;
+    // commit: swap the rebuilt parts in (N then frees the old ones)
+    if (geo) p->swap_geometry(N);
+    if (fwd) p->swap_fwd(N);
+    if (adj) p->swap_adj(N);
+    if (split) {
+        free_ptr(p->d_split);
+        p->d_split = split;
+        p->split_cap = need;
     }
-    return 0;
+    p->tune = nt;
+    return split_rc;
 }
 
 namespace ctprj {
 
 // Partial-sum space of the split symmetric launches: per group launch nitems * nsplit
 // work-groups of 512 floats; the launches of one call run in stream order and share it.
-int Plan::ensure_split()
+size_t Plan::split_need(const ctprj_tune& t, const Tables& F, const Tables& A) const
 {
     size_t need = 0;
     for (int g = 1; g < 4; ++g) {
-        if (tune.fwd_split > 1) need = std::max(need, (size_t)symf_count[g] * tune.fwd_split);
-        if (tune.adj_split > 1) need = std::max(need, (size_t)syma_count[g] * tune.adj_split);
+        if (t.fwd_split > 1) need = std::max(need, (size_t)F.symf_count[g] * t.fwd_split);
+        if (t.adj_split > 1) need = std::max(need, (size_t)A.syma_count[g] * t.adj_split);
     }
-    need *= 512;
-    if (need > kSplitMaxFloats)
-        return fail(CTPRJ_EINVAL, "split work items need %zu MiB of partial sums (at most %zu)",
-                    need * sizeof(float) >> 20, kSplitMaxFloats * sizeof(float) >> 20);
-    if (need <= split_cap) return 0;
-    free_ptr(d_split);
-    split_cap = 0;
-    int rc = dev_alloc((void**)&d_split, sizeof(float) * need, "split partial sums");
-    if (!rc) split_cap = need;
-    return rc;
+    return need * 512;
 }
 
 }  // namespace ctprj
+
+extern "C" int ctprj_debug_fail_alloc(int k)
+{
+    if (k < 0) return fail(CTPRJ_EINVAL, "ctprj_debug_fail_alloc: k must be >= 0");
+    g_fail_alloc.store(k);
+    return 0;
+}
 
 static std::string tune_key(const ctprj_plan* p, const char* arch)
 {

@@ -23,37 +23,55 @@ struct AdjItem {
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
 
-struct Plan {
-    int device = 0;
-    int nslices = 0;
-    std::vector<ctprj_geom> geoms;
-    std::vector<long long> img_offsets, sino_offsets;
-    std::vector<int> slice_sym;  // per slice: symmetry group (kGrp*), 0 = general kernels
-    long long img_elems = 0, sino_elems = 0;
-    int max_W = 0;
-    ctprj_tune tune{64, 16, 1, 8, 8, 1, 1};
-
+// Device tables of a plan.  The geometry tables depend on tune.flags, the work lists on
+// tune.flags and the general kernels' shapes (fwd_block, adj_tile).  A tune change builds
+// the parts it changes into a second Tables and swaps them in only when every step
+// succeeded, so a failed change leaves the plan's tables as they were.
+struct Tables {
     SliceDesc* d_desc = nullptr;
     float4* d_views = nullptr;
     Ray* d_rays = nullptr;
-    FwdItem* d_fwd_items = nullptr;   // general forward
-    AdjItem* d_adj_items = nullptr;   // general adjoint
-    FwdItem* d_symf_items = nullptr;  // symmetric forward
-    AdjItem* d_syma_items = nullptr;  // symmetric adjoint
-    AdjItem* d_tr_items = nullptr;
-    int2* d_fbp_items = nullptr;
-    float* d_zero_row = nullptr;      // max_W zeros: reads of views beyond np
     int* d_symtab = nullptr;          // view-row table of the symmetric adjoint
-    float* d_fbp_kern = nullptr;
-    long long* d_fbp_kern_off = nullptr;
-    float* d_fbp_prew = nullptr;
-    long long* d_fbp_prew_off = nullptr;
-    int n_fwd_items = 0, n_adj_items = 0, n_symf_items = 0, n_syma_items = 0;
+    AdjItem* d_tr_items = nullptr;    // transpose tiles
+    int n_tr_items = 0, n_sym = 0;
+    std::vector<int> slice_sym;       // per slice: symmetry group (kGrp*), 0 = general kernels
+    FwdItem* d_fwd_items = nullptr;   // general forward
+    FwdItem* d_symf_items = nullptr;  // symmetric forward
+    int n_fwd_items = 0, n_symf_items = 0;
+    AdjItem* d_adj_items = nullptr;   // general adjoint
+    AdjItem* d_syma_items = nullptr;  // symmetric adjoint
+    int n_adj_items = 0, n_syma_items = 0;
     // symmetric work lists are grouped by symmetry group: [start, start + count) of
     // group g (1..3) in d_symf_items / d_syma_items
     int symf_start[4] = {0, 0, 0, 0}, symf_count[4] = {0, 0, 0, 0};
     int syma_start[4] = {0, 0, 0, 0}, syma_count[4] = {0, 0, 0, 0};
-    int n_tr_items = 0, n_fbp_items = 0, n_sym = 0;
+
+    Tables() = default;
+    Tables(const Tables&) = delete;
+    Tables& operator=(const Tables&) = delete;
+    ~Tables() { release(); }
+    void release();
+    void swap_geometry(Tables& o);
+    void swap_fwd(Tables& o);
+    void swap_adj(Tables& o);
+};
+
+struct Plan : Tables {
+    int device = 0;
+    int nslices = 0;
+    std::vector<ctprj_geom> geoms;
+    std::vector<long long> img_offsets, sino_offsets;
+    long long img_elems = 0, sino_elems = 0;
+    int max_W = 0;
+    ctprj_tune tune{64, 16, 1, 8, 8, 1, 1};
+
+    int2* d_fbp_items = nullptr;
+    float* d_zero_row = nullptr;      // max_W zeros: reads of views beyond np
+    float* d_fbp_kern = nullptr;
+    long long* d_fbp_kern_off = nullptr;
+    float* d_fbp_prew = nullptr;
+    long long* d_fbp_prew_off = nullptr;
+    int n_fbp_items = 0;
 
     float* d_imgT = nullptr;     // transposed image (forward, steep views / symmetric images)
     float* d_sino_ws = nullptr;  // sinogram workspace (FBP filter output, BHC ray values)
@@ -67,10 +85,13 @@ struct Plan {
     size_t split_cap = 0;        // floats held by d_split
 
     ~Plan();
-    int build_tables();          // geometry tables for the current tune.flags
-    int build_fwd_items();
-    int build_adj_items();
-    int ensure_split();          // d_split sized for tune.fwd_split / adj_split
+    // geometry tables for tune flags `flags` into G
+    int build_geometry(int flags, Tables& G) const;
+    // work lists for tune t on the geometry of G, into G
+    int build_fwd_items(const ctprj_tune& t, Tables& G) const;
+    int build_adj_items(const ctprj_tune& t, Tables& G) const;
+    // partial-sum floats the split symmetric launches of tune t need on tables G
+    size_t split_need(const ctprj_tune& t, const Tables& fwd, const Tables& adj) const;
     int launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count);
     int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp);
     int launch_fbp(const float* d_sino, float* d_img, hipStream_t st);

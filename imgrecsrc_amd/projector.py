@@ -53,6 +53,8 @@ class Projector:
         if device is None:
             device = torch.device("cuda", torch.cuda.current_device())
         self.device = torch.device(device)
+        if self.device.index is None:
+            self.device = torch.device(self.device.type, torch.cuda.current_device())
         self.geoms = geoms
         arr = (Geom * len(geoms))(*geoms)
         h = ctypes.c_void_p()
@@ -114,31 +116,41 @@ class Projector:
     def _check(self, t: torch.Tensor, elems: int, what: str) -> torch.Tensor:
         if not isinstance(t, torch.Tensor) or t.device.type != "cuda":
             raise TypeError(f"{what} must be a CUDA (HIP) tensor")
+        if t.device != self.device:
+            raise ValueError(f"{what} is on {t.device}, the plan on {self.device}")
         if t.dtype != torch.float32:
             raise TypeError(f"{what} must be float32, got {t.dtype}")
         if t.numel() != elems:
             raise ValueError(f"{what} has {t.numel()} elements, plan needs {elems}")
         return t.contiguous()
 
+    def _out(self, out: torch.Tensor | None, elems: int, like: torch.Tensor) -> torch.Tensor:
+        """The output buffer: a new one, or the caller's after the same checks as an input
+        plus contiguity (the kernels write `elems` floats from its first element; a
+        contiguous copy would silently drop the result)."""
+        if out is None:
+            return torch.empty(elems, dtype=torch.float32, device=like.device)
+        self._check(out, elems, "out")
+        if not out.is_contiguous():
+            raise ValueError("out must be contiguous")
+        return out
+
     # ------------------------------------------------------------------ operators
     def forward(self, img: torch.Tensor, out: torch.Tensor | None = None, stream=None):
         img = self._check(img, self.img_elems, "image")
-        if out is None:
-            out = torch.empty(self.sino_elems, dtype=torch.float32, device=img.device)
+        out = self._out(out, self.sino_elems, img)
         check(lib().ctprj_forward(self.handle, _ptr(img), _ptr(out), _stream_ptr(stream)))
         return out
 
     def adjoint(self, sino: torch.Tensor, out: torch.Tensor | None = None, stream=None):
         sino = self._check(sino, self.sino_elems, "sinogram")
-        if out is None:
-            out = torch.empty(self.img_elems, dtype=torch.float32, device=sino.device)
+        out = self._out(out, self.img_elems, sino)
         check(lib().ctprj_adjoint(self.handle, _ptr(sino), _ptr(out), _stream_ptr(stream)))
         return out
 
     def fbp(self, sino: torch.Tensor, out: torch.Tensor | None = None, stream=None):
         sino = self._check(sino, self.sino_elems, "sinogram")
-        if out is None:
-            out = torch.empty(self.img_elems, dtype=torch.float32, device=sino.device)
+        out = self._out(out, self.img_elems, sino)
         check(lib().ctprj_fbp(self.handle, _ptr(sino), _ptr(out), _stream_ptr(stream)))
         return out
 

@@ -128,7 +128,8 @@ typedef struct ctprj_tune {
 int ctprj_plan_get_tune(const ctprj_plan* plan, ctprj_tune* tune);
 /* Out-of-range fields are refused (CTPRJ_EINVAL) and change nothing.  Splits whose
  * partial sums would exceed the plan's 256 MiB are refused too; the other fields then
- * stay applied and both splits fall back to 1. */
+ * stay applied and both splits fall back to 1.  A change that fails part way (a device
+ * allocation) changes nothing: the plan keeps its previous tables and shapes. */
 int ctprj_plan_set_tune(ctprj_plan* plan, const ctprj_tune* tune);
 /* Times every candidate shape on this plan's geometry (HIP events, `reps` runs each),
  * keeps the fastest and, if cache_path is non-NULL, records it in that text file keyed
@@ -393,6 +394,10 @@ const char* ctprj_last_error(void);
 int ctprj_device_arch(char* buf, int buflen);
 /* Library version string. */
 const char* ctprj_version(void);
+/* Failure injection for tests of the error paths: with k > 0, the k-th device
+ * allocation the library makes from now on fails as out of memory (once); 0 disarms.
+ * No reference counterpart. */
+int ctprj_debug_fail_alloc(int k);
 
 #ifdef __cplusplus
 }

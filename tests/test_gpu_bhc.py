@@ -160,6 +160,41 @@ def test_autograd_binding(problem):
     s = torch.randn(GEO[1] * GEO[2], device="cuda", requires_grad=True)
     (backproject(s, proj) * x.detach()).sum().backward()
     assert torch.equal(s.grad, proj.forward(x.detach()))
+    # 2-D inputs: an n x n image and an np x prjWidth sinogram get gradients of their shape
+    x2 = x.detach().view(GEO[0], GEO[0]).clone().requires_grad_(True)
+    (project(x2, proj) * y).sum().backward()
+    assert x2.grad.shape == x2.shape and torch.equal(x2.grad.reshape(-1), proj.adjoint(y))
+    s2 = s.detach().view(GEO[2], GEO[1]).clone().requires_grad_(True)
+    (backproject(s2, proj) * x.detach()).sum().backward()
+    assert s2.grad.shape == s2.shape and torch.equal(s2.grad.reshape(-1), proj.forward(x.detach()))
+    a2 = P["alpha_t"].view(GEO[0], GEO[0]).clone().requires_grad_(True)
+    poisson_nll(a2, ie.detach(), proj, P["Imea_t"], P["knots"], P["Ts"]).backward()
+    assert a2.grad.shape == a2.shape and torch.equal(a2.grad.reshape(-1), g_ref)
+
+
+def test_out_buffer_checked(problem):
+    """A caller's output buffer goes through the input checks plus contiguity: a short,
+    wrongly typed or strided `out` raises instead of being written past its end."""
+    import torch
+    from imgrecsrc_amd.projector import Projector
+    proj = Projector([GEO])
+    x = problem["alpha_t"]
+    y = proj.forward(x)
+    bad = [torch.empty(proj.sino_elems - 1, device="cuda"),
+           torch.empty(proj.sino_elems, device="cuda", dtype=torch.float64),
+           torch.empty(2 * proj.sino_elems, device="cuda")[::2],
+           torch.empty(proj.sino_elems)]
+    for out in bad:
+        with pytest.raises((ValueError, TypeError)):
+            proj.forward(x, out=out)
+    for out in [torch.empty(proj.img_elems + 7, device="cuda"),
+                torch.empty(2 * proj.img_elems, device="cuda")[::2]]:
+        with pytest.raises((ValueError, TypeError)):
+            proj.adjoint(y, out=out)
+        with pytest.raises((ValueError, TypeError)):
+            proj.fbp(y, out=out)
+    good = torch.full((proj.sino_elems,), 7.0, device="cuda")
+    assert proj.forward(x, out=good) is good and torch.equal(good, y)
 
 
 def test_zero_counts(problem):

@@ -74,6 +74,12 @@ def test_argument_errors_without_device(L):
     assert L.ctprj_plan_create(ctypes.byref(g), 1, ctypes.byref(h)) == _lib.CTPRJ_EINVAL
 
 
+def test_failure_injection_hook(L):
+    from imgrecsrc_amd import _lib
+    assert L.ctprj_debug_fail_alloc(-1) == _lib.CTPRJ_EINVAL
+    assert L.ctprj_debug_fail_alloc(0) == 0
+
+
 def test_no_device_is_reported(L):
     import torch
     if torch.cuda.is_available():
