@@ -81,6 +81,13 @@ def lib():
         L.orc_fbp.argtypes = [P, dp, dp, ctypes.c_int]
         L.orc_ramp_filter.restype = None
         L.orc_ramp_filter.argtypes = [dp, ctypes.c_int]
+        L.orc_csr_counts.restype = ctypes.c_int
+        L.orc_csr_counts.argtypes = [P, dp, ctypes.c_int]
+        L.orc_csr_fill.restype = ctypes.c_int
+        L.orc_csr_fill.argtypes = [P, dp, dp, dp, ctypes.c_int]
+        L.orc_csr_apply.restype = ctypes.c_int
+        L.orc_csr_apply.argtypes = [ctypes.c_longlong, ctypes.c_longlong, dp, dp, dp, dp, dp,
+                                    ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -92,7 +99,9 @@ def conf(n, prjWidth, np_, prjFull, dSize=1.0, effectiveRate=1.0, d=0.0,
 
 
 def _nth():
-    return min(8, os.cpu_count() or 1)
+    """Worker threads: the cores this process may use, at most 16."""
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return max(1, min(16, cores or 1))
 
 
 def get_weight(dist, bw, c, s) -> float:
@@ -116,6 +125,52 @@ def adjoint(cf: OrcConf, sino: np.ndarray):
     img = np.zeros((cf.n, cf.n), np.float64)
     nnz = lib().orc_adjoint(ctypes.byref(cf), sino.ctypes.data, img.ctypes.data, _nth())
     return img, nnz
+
+
+class Matrix:
+    """The FP64 system matrix A of one geometry, stored in compressed rows (12 bytes per
+    nonzero), for checking many images of the same geometry: one visit of every ray
+    builds it, after which forward / adjoint of K stacked images cost a sparse product.
+    The stored forward adds each ray's terms in the order orc_forward does, so it equals
+    `forward` bit for bit; the stored adjoint equals `adjoint` up to FP64 rounding."""
+
+    def __init__(self, cf: OrcConf):
+        self.cf = cf
+        self.rows = cf.np * cf.prjWidth
+        self.ncols = cf.n * cf.n
+        cnt = np.zeros(self.rows, np.int64)
+        L = lib()
+        L.orc_csr_counts(ctypes.byref(cf), cnt.ctypes.data, _nth())
+        self.rowptr = np.zeros(self.rows + 1, np.int64)
+        np.cumsum(cnt, out=self.rowptr[1:])
+        self.nnz = int(self.rowptr[-1])
+        self.col = np.empty(self.nnz, np.int32)
+        self.val = np.empty(self.nnz, np.float64)
+        L.orc_csr_fill(ctypes.byref(cf), self.rowptr.ctypes.data, self.col.ctypes.data,
+                       self.val.ctypes.data, _nth())
+
+    def _apply(self, x, trans):
+        n_in, n_out = (self.rows, self.ncols) if trans else (self.ncols, self.rows)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        K = x.size // n_in
+        assert x.size == K * n_in, "input size is not a whole number of vectors"
+        out = np.empty(K * n_out, np.float64)
+        rc = lib().orc_csr_apply(self.rows, self.ncols, self.rowptr.ctypes.data,
+                                 self.col.ctypes.data, self.val.ctypes.data, x.ctypes.data,
+                                 out.ctypes.data, K, int(trans), _nth())
+        if rc != 0:
+            raise MemoryError("orc_csr_apply: no memory for the per-thread images")
+        return out
+
+    def forward(self, img):
+        """Sinograms (K, np, prjWidth) of K stacked images (K, n, n) or one (n, n)."""
+        out = self._apply(img, False)
+        return out.reshape(-1, self.cf.np, self.cf.prjWidth)
+
+    def adjoint(self, sino):
+        """Images (K, n, n) of K stacked sinograms."""
+        out = self._apply(sino, True)
+        return out.reshape(-1, self.cf.n, self.cf.n)
 
 
 def nnz(cf: OrcConf) -> int:

@@ -222,8 +222,10 @@ static double ray_pixel_weight(const orc_conf* c, view_t v, ray_t r, int x, int 
     return orc_get_weight(dist, bw, r.cosR, r.sinR) / c->effectiveRate;
 }
 
-/* Visit every (pixel, weight) of ray (a, b).  op: 0 fwd, 1 adj, 2 count, 3 FBP adj. */
-static long long visit_ray(const orc_conf* c, int a, int b, const double* in, double* out, int op)
+/* Visit every (pixel, weight) of ray (a, b).  op: 0 fwd, 1 adj, 2 count, 3 FBP adj,
+ * 4 record the row of A (pixel indices to col, weights to val, in visiting order). */
+static long long visit_ray_rec(const orc_conf* c, int a, int b, const double* in, double* out,
+                               int op, int* col, double* val)
 {
     const int N = c->n, H = (N - 1) / 2;
     view_t v = view_of(c, a);
@@ -249,6 +251,10 @@ static long long visit_ray(const orc_conf* c, int a, int b, const double* in, do
             if (w == 0.0) continue;
             nnz++;
             long pidx = (long)(y + N / 2) * N + x + N / 2;
+            if (op == 4) {
+                col[nnz - 1] = (int)pidx;
+                val[nnz - 1] = w;
+            }
             if (op == 0) acc += w * in[pidx];
             else if (op == 1) out[pidx] += w * in[sidx];
             else if (op == 3) {
@@ -263,6 +269,11 @@ static long long visit_ray(const orc_conf* c, int a, int b, const double* in, do
     }
     if (op == 0) out[sidx] = acc;
     return nnz;
+}
+
+static long long visit_ray(const orc_conf* c, int a, int b, const double* in, double* out, int op)
+{
+    return visit_ray_rec(c, a, b, in, out, op, NULL, NULL);
 }
 
 /* ------------------------------------------------------------ threading --- */
@@ -402,5 +413,127 @@ int orc_fbp(const orc_conf* c, const double* sino, double* img, int nthreads)
     const double scale = ORC_PI / c->np / c->dSize / c->dSize; /* prj/cpuPrj.c:161,304 */
     for (long k = 0; k < isz; k++) img[k] *= scale;
     free(ws);
+    return 0;
+}
+
+/* ------------------------------------------------------------ stored A --- */
+
+/* A in compressed rows: row r = a * prjWidth + b (the sinogram index), columns the image
+ * index, each row in visit_ray's order, so a stored forward adds the same terms in the
+ * same order as orc_forward.  Rows of bin 0 of an even detector are empty. */
+typedef struct {
+    const orc_conf* c;
+    long long* rowcnt;
+    const long long* rowptr;
+    int* col;
+    double* val;
+    int tid, nth;
+} csr_job_t;
+
+static void* csr_worker(void* p)
+{
+    csr_job_t* j = (csr_job_t*)p;
+    const orc_conf* c = j->c;
+    const int W = c->prjWidth;
+    for (int a = j->tid; a < c->np; a += j->nth)
+        for (int b = bin_lo(c); b < W; b++) {
+            const long r = (long)a * W + b;
+            if (j->rowptr)
+                visit_ray_rec(c, a, b, NULL, NULL, 4, j->col + j->rowptr[r], j->val + j->rowptr[r]);
+            else
+                j->rowcnt[r] = visit_ray(c, a, b, NULL, NULL, 2);
+        }
+    return NULL;
+}
+
+static void csr_run(const orc_conf* c, long long* rowcnt, const long long* rowptr, int* col,
+                    double* val, int nth)
+{
+    if (nth < 1) nth = 1;
+    if (nth > 64) nth = 64;
+    pthread_t th[64];
+    csr_job_t jobs[64];
+    for (int i = 0; i < nth; i++) {
+        jobs[i] = (csr_job_t){c, rowcnt, rowptr, col, val, i, nth};
+        pthread_create(&th[i], NULL, csr_worker, &jobs[i]);
+    }
+    for (int i = 0; i < nth; i++) pthread_join(th[i], NULL);
+}
+
+int orc_csr_counts(const orc_conf* c, long long* rowcnt, int nthreads)
+{
+    memset(rowcnt, 0, sizeof(long long) * (size_t)c->np * c->prjWidth);
+    csr_run(c, rowcnt, NULL, NULL, NULL, nthreads);
+    return 0;
+}
+
+int orc_csr_fill(const orc_conf* c, const long long* rowptr, int* col, double* val, int nthreads)
+{
+    csr_run(c, NULL, rowptr, col, val, nthreads);
+    return 0;
+}
+
+typedef struct {
+    long long rows, ncols;
+    const long long* rowptr;
+    const int* col;
+    const double* val;
+    const double* in;
+    double* out;
+    int K, tid, nth, trans;
+} spmv_job_t;
+
+static void* spmv_worker(void* p)
+{
+    spmv_job_t* j = (spmv_job_t*)p;
+    /* contiguous row blocks: forward writes disjoint outputs; the transpose adds into
+     * the thread's own image buffers */
+    const long long r0 = j->rows * j->tid / j->nth, r1 = j->rows * (j->tid + 1) / j->nth;
+    for (long long r = r0; r < r1; r++) {
+        const long long e0 = j->rowptr[r], e1 = j->rowptr[r + 1];
+        for (int k = 0; k < j->K; k++) {
+            if (!j->trans) {
+                const double* x = j->in + (size_t)k * j->ncols;
+                double acc = 0;
+                for (long long e = e0; e < e1; e++) acc += j->val[e] * x[j->col[e]];
+                j->out[(size_t)k * j->rows + r] = acc;
+            } else {
+                const double y = j->in[(size_t)k * j->rows + r];
+                double* x = j->out + (size_t)k * j->ncols;
+                if (y != 0.0)
+                    for (long long e = e0; e < e1; e++) x[j->col[e]] += j->val[e] * y;
+            }
+        }
+    }
+    return NULL;
+}
+
+int orc_csr_apply(long long rows, long long ncols, const long long* rowptr, const int* col,
+                  const double* val, const double* in, double* out, int K, int trans,
+                  int nthreads)
+{
+    int nth = nthreads < 1 ? 1 : (nthreads > 64 ? 64 : nthreads);
+    pthread_t th[64];
+    spmv_job_t jobs[64];
+    double* priv[64];
+    const size_t osz = (size_t)K * (trans ? ncols : rows);
+    for (int i = 0; i < nth; i++) {
+        priv[i] = trans ? (double*)calloc(osz, sizeof(double)) : NULL;
+        if (trans && !priv[i]) {
+            for (int k = 0; k < i; k++) free(priv[k]);
+            return -1;
+        }
+        jobs[i] = (spmv_job_t){rows, ncols, rowptr, col, val, in, trans ? priv[i] : out,
+                               K, i, nth, trans};
+        pthread_create(&th[i], NULL, spmv_worker, &jobs[i]);
+    }
+    for (int i = 0; i < nth; i++) pthread_join(th[i], NULL);
+    if (trans) {
+        memset(out, 0, osz * sizeof(double));
+        for (int i = 0; i < nth; i++) {
+            for (size_t k = 0; k < osz; k++) out[k] += priv[i][k];
+            free(priv[i]);
+        }
+    }
     return 0;
 }

@@ -74,6 +74,18 @@ int orc_fbp(const orc_conf* c, const double* sino, double* img, int nthreads);
  * by direct O(size^2) circular convolution in double */
 void orc_ramp_filter(double* signal, int size);
 
+/* A stored in compressed rows (row = sinogram index a * prjWidth + b, column = image
+ * index), built from the same visit as orc_forward: orc_csr_counts gives the entries per
+ * row, the caller forms rowptr (exclusive prefix sum, np * prjWidth + 1 entries) and
+ * orc_csr_fill writes each row in visiting order.  orc_csr_apply multiplies K stacked
+ * vectors: trans = 0 gives out[k] = A in[k] (in: K images, out: K sinograms; each row
+ * adds its terms in orc_forward's order), trans = 1 gives out[k] = A^T in[k]. */
+int orc_csr_counts(const orc_conf* c, long long* rowcnt, int nthreads);
+int orc_csr_fill(const orc_conf* c, const long long* rowptr, int* col, double* val, int nthreads);
+int orc_csr_apply(long long rows, long long ncols, const long long* rowptr, const int* col,
+                  const double* val, const double* in, double* out, int K, int trans,
+                  int nthreads);
+
 #ifdef __cplusplus
 }
 #endif
