@@ -92,6 +92,7 @@ SIGNATURES = [
     ("ctprj_plan_set_bhc_model", _i, [_vp, _i, _i]),
     ("ctprj_plan_get_bhc_model", _i, [_vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
     ("ctprj_bhc_grad_alpha", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
+    ("ctprj_bhc_grad_alpha_noalloc", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_alpha_rays", _i, [_vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_bhc_grad_ie", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _vp, _vp, _vp, _vp]),
     ("ctprj_ie_solve", _i, [_vp, _vp, _vp, _vp, _dp, _i, _dp, _f, _i, ctypes.c_double,

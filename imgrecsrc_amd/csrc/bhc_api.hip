@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "bhc_internal.h"
@@ -157,6 +158,72 @@ extern "C" int ctprj_bhc_alpha_rays(ctprj_plan* p, const float* d_sraw, const fl
     if ((rc = last_launch("bhc alpha ray launch"))) return rc;
     launch_reduce_partials(p->d_red, blocks, 1, d_f, st);
     return last_launch("bhc reduce launch");
+}
+
+// ------------------------------------------------ the entry point that allocates nothing ---
+
+namespace ctprj {
+
+// Reduction partials of ctprj_bhc_grad_alpha_noalloc in a file-scope device array: one
+// double per group of kBhcBlock rays, then the reduced value.  Each device holds its own
+// copy until the process ends; every call on a device shares that device's copy.
+constexpr long long kNoallocParts = 1LL << 16;  // groups per call: 16.7 M rays at most
+static __device__ __align__(256) double scratch[kNoallocParts + 1];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxDevices];  // device address of `scratch`, per device
+
+// The current device's address of `scratch`, looked up once per device with that device
+// current.  Host code never passes `scratch` itself: its name is a host-side stand-in.
+static int noalloc_scratch(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) return fail(CTPRJ_EINVAL, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[dev] = static_cast<double*>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+}  // namespace ctprj
+
+extern "C" int ctprj_bhc_grad_alpha_noalloc(ctprj_plan* p, const float* d_alpha,
+                                            const float* d_Imea, const double* knots, int E,
+                                            const double* Ie, float Ts, double* d_f,
+                                            float* d_grad, float* d_hw, void* stream)
+{
+    if (!p || !d_alpha || !d_Imea || !d_f)
+        return fail(CTPRJ_EINVAL, "bhc_grad_alpha_noalloc: NULL argument");
+    if (!Ie) return fail(CTPRJ_EINVAL, "bhc_grad_alpha_noalloc: Ie is NULL");
+    BhcParams P;
+    int rc = make_params(p, knots, E, Ie, &P);
+    if (rc) return rc;
+    const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
+    if (blocks > kNoallocParts)
+        return fail(CTPRJ_EINVAL,
+                    "bhc_grad_alpha_noalloc: %lld rays exceed the static workspace's %lld "
+                    "(use ctprj_bhc_grad_alpha)", rays, kNoallocParts * kBhcBlock);
+    double* part = nullptr;
+    if ((rc = noalloc_scratch(&part))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    // the same launches as ctprj_bhc_grad_alpha; every partial is written before the
+    // reduction reads it, so nothing needs clearing
+    if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
+    float* w = d_grad ? p->d_sino_ws : nullptr;
+    launch_bhc_alpha_ray(P, p->d_sino_ws, Ts, d_Imea, rays, Ts, w, d_hw, part, st);
+    if ((rc = last_launch("bhc alpha ray launch"))) return rc;
+    launch_reduce_partials(part, blocks, 1, d_f, st);
+    if ((rc = last_launch("bhc reduce launch"))) return rc;
+    if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false);
+    return 0;
 }
 
 extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const float* d_s,

@@ -188,6 +188,20 @@ int ctprj_bhc_grad_alpha(ctprj_plan* plan, const float* d_alpha, const float* d_
                          const double* knots, int E, const double* Ie, float Ts,
                          double* d_f, float* d_grad, float* d_hw, void* stream);
 
+/* ctprj_bhc_grad_alpha for callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart in the
+ * reference.  Same arguments, results and launches as ctprj_bhc_grad_alpha, but the FP64
+ * reduction partials live in a static device array of the library (512 KiB per device,
+ * held until the process ends) instead of the plan's lazily allocated workspace, so the
+ * call allocates nothing, also on its first call; the plan's own buffers (created with
+ * the plan) serve as before.  At most 16,777,216 rays (np*prjWidth summed over the
+ * plan's slices) per call: larger plans get CTPRJ_EINVAL.  Every call on one device
+ * shares that array, so calls that overlap on one device (other streams or threads)
+ * race and must be serialised by the caller. */
+int ctprj_bhc_grad_alpha_noalloc(ctprj_plan* plan, const float* d_alpha, const float* d_Imea,
+                                 const double* knots, int E, const double* Ie, float Ts,
+                                 double* d_f, float* d_grad, float* d_hw, void* stream);
+
 /* The per-ray part of poissLAlpha on a given projection: with s = Ts * d_sraw
  * (d_sraw = A*alpha, np*prjWidth per slice, packed), writes d_f (one device double) and,
  * when non-NULL, d_w = Ts*(1 - Imea/BL)*sBL (may alias d_sraw) and d_hw (Hessian ray
