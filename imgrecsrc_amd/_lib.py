@@ -52,7 +52,8 @@ class PnpgInfo(ctypes.Structure):
                 ("iterations", ctypes.c_int), ("trials", ctypes.c_int), ("rejects", ctypes.c_int),
                 ("restarts", ctypes.c_int), ("last_accepted", ctypes.c_int),
                 ("difAlpha", ctypes.c_double), ("admmTol", ctypes.c_double),
-                ("innerSearch", ctypes.c_int), ("maxInnerItr", ctypes.c_int)]
+                ("innerSearch", ctypes.c_int), ("maxInnerItr", ctypes.c_int),
+                ("t_reset", ctypes.c_double), ("resets", ctypes.c_int)]
 
 
 class IeInfo(ctypes.Structure):
@@ -122,12 +123,14 @@ SIGNATURES = [
     ("ctprj_pnpg_set_mask", _i, [_vp, _vp]),
     ("ctprj_pnpg_set_wavelet", _i, [_vp, _vp, ctypes.c_double]),
     ("ctprj_pnpg_set_tv", _i, [_vp, _vp, ctypes.c_double]),
+    ("ctprj_pnpg_set_linesearch", _i, [_vp, _i]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),
     ("ctprj_pnpg_destroy", _i, [_vp]),
     ("ctprj_last_error", ctypes.c_char_p, []),
     ("ctprj_device_arch", _i, [ctypes.c_char_p, _i]),
     ("ctprj_version", ctypes.c_char_p, []),
     ("ctprj_debug_fail_alloc", _i, [_i]),
+    ("ctprj_debug_pnpg_decide", _i, [_vp, _i, _i, _vp, ctypes.POINTER(ctypes.c_int)]),
 ]
 
 _lib = None

@@ -43,6 +43,9 @@ struct PnpgState {
     int restarts, rejects;
     int outer;        // NPG: outer iterations begun (NPG.m obj.p, retries included)
     int goodStep;     // NPG: no failed test in this iteration yet
+    int moved;        // this trial's xbar differs from alpha (||xbar - alpha||_1 != 0)
+    int keep;         // this accept keeps alpha ("force it to converge")
+    double admmTol;   // non-negativity prox: obj.admmTol, whose retries PNPG.m:125-131 counts
 };
 
 // PNPG.m / NPG.m properties (defaults of the reference classes)
@@ -55,6 +58,7 @@ struct PnpgParams {
     int incCumuTol;
     int npg;           // 1: the NPG variant (npg/NPG.m:45-148)
     int preSteps;      // NPG: iterations whose line search also grows the step
+    int maxLs;         // failed tests that shrink the step before giving up (20)
 };
 
 // Start of a trial.  PNPG (PNPG.m:56-77): theta and xbar are recomputed each trial
@@ -83,6 +87,7 @@ __host__ __device__ inline void pnpg_begin(PnpgState& S, const PnpgParams& P)
     }
     S.ppp += 1;
     S.trials += 1;
+    S.moved = 0;
     if (!P.npg) {
         const double B = S.t / S.preT;
         S.newTheta = (S.p == 0) ? 1.0 : 1.0 / P.gamma + sqrt(P.a + B * S.theta * S.theta);
@@ -102,7 +107,7 @@ __host__ __device__ inline int pnpg_linesearch(PnpgState& S, const PnpgParams& P
         }
         return 1;
     }
-    if (S.ppp <= 20 && S.t > 0) {
+    if (S.ppp <= P.maxLs && S.t > 0) {
         S.t = S.t / P.stepShrnk;
         S.goodStep = 0;
         if (S.incStep) {
@@ -143,14 +148,19 @@ __device__ __forceinline__ float pm(const float* mask, long long i, float v)
     return (!mask || mask[i] != 0.0f) ? v : 0.0f;
 }
 
-__global__ void k_pnpg_xbar(const PnpgState* __restrict__ S, const float* __restrict__ alpha,
+// S->moved is set when any pixel of xbar differs from alpha (the restart test's
+// ||xbar - alpha||_1 != 0, PNPG.m:104); k_pnpg_begin clears it.
+__global__ void k_pnpg_xbar(PnpgState* __restrict__ S, const float* __restrict__ alpha,
                             const float* __restrict__ pre, float* __restrict__ xbar, long long n,
                             const float* __restrict__ mask)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const float c = (float)S->coef;
-    xbar[i] = pm(mask, i, fmaxf(fmaf(c, alpha[i] - pre[i], alpha[i]), 0.0f));
+    const float a = alpha[i];
+    const float x = pm(mask, i, fmaxf(fmaf(c, a - pre[i], a), 0.0f));
+    xbar[i] = x;
+    if (x != a) atomicOr(&S->moved, 1);
 }
 
 // newX = max(xbar - grad/t, 0) and the FP64 partials of <newX-xbar, grad>, ||newX-xbar||^2
@@ -216,23 +226,54 @@ __global__ void k_pnpg_dot_reduce(const double* __restrict__ partials, long long
     }
 }
 
+// The non-negativity prox (customMax, bhc/beamhardenSpline.m:200-203,586-589) reports
+// one inner iteration against the reference's maxInnerItr of 100.
+constexpr int kNonnegInnerSearch = 1, kNonnegMaxInnerItr = 100;
+
+// The decision after a finished line search with penalty u = 0 (PNPG.m:99-160 /
+// NPG.m:99-146; goodMM = (ls == 1)), on the host and the device alike.  Returns 1 when
+// the iterate is taken (keep: newX is alpha itself), 0 when the iteration is retried.
+__host__ __device__ inline int pnpg_nonneg_decide(PnpgState& s, const PnpgParams& P, int ls)
+{
+    const bool goodMM = ls == 1;
+    double newObj = s.f_new;
+    s.keep = 0;
+    if (newObj - s.cost > 0) {
+        if (goodMM && s.moved && P.restart >= 0) {  // restart: drop the momentum, retry
+            s.theta = 1.0;
+            s.restarts += 1;
+            s.ppp = 0;
+            return 0;
+        }
+        if (!goodMM || s.f_old < newObj) {
+            // reset() sets theta = 1 (its step re-estimate from the Hessian is not taken on
+            // this graph-capturable path); then the admmTol schedule retries the
+            // iteration until admmTol reaches 1e-6, and finally alpha is kept
+            if (!goodMM) s.theta = 1.0;
+            if (kNonnegInnerSearch < kNonnegMaxInnerItr && s.admmTol > 1e-6) {
+                s.admmTol /= 10.0;
+                s.ppp = 0;
+                return 0;
+            }
+            newObj = s.cost;
+            s.keep = 1;
+        }
+    }
+    pnpg_accept(s, P, newObj);
+    return 1;
+}
+
 // Majorization test f(newX) <= f(xbar) + <newX - xbar, g> + t/2 ||newX - xbar||^2
-// (utils/Utils.m:470-480), then accept / restart / shrink (PNPG.m:79-146).
+// (utils/Utils.m:470-480), then accept / restart / retry / keep (PNPG.m:79-160).
 __global__ void k_pnpg_decide(PnpgState* S, PnpgParams P)
 {
     if (threadIdx.x | blockIdx.x) return;
     PnpgState s = *S;
     const bool holds = (s.f_new - s.f_old) <= s.dg + 0.5 * s.t * s.dd;
     s.accept = 0;
+    s.keep = 0;
     const int ls = pnpg_linesearch(s, P, holds);
-    if (ls == 1 && s.p > 0 && s.f_new > s.cost && s.coef != 0.0 && P.restart >= 0) {
-        s.theta = 1.0;  // restart: drop the momentum and retry (PNPG.m:103-112)
-        s.restarts += 1;
-        s.ppp = 0;
-    } else if (ls != 0) {
-        // a given-up line search (ls == 2) is accepted as it is (goodMM = false)
-        pnpg_accept(s, P, s.f_new);
-    }
+    if (ls != 0) pnpg_nonneg_decide(s, P, ls);
     *S = s;
 }
 
@@ -285,13 +326,45 @@ __global__ __launch_bounds__(256) void k_pnpg_wv_sums(const float* __restrict__ 
 }
 
 // on acceptance: preAlpha <- alpha, alpha <- newX   (PNPG.m:148-151)
+// FP64 partial sums of sum_i w_i a_i^2 (w == NULL: sum a_i^2; mask zeroes a outside the
+// support) per work-group of 256.
+__global__ __launch_bounds__(256) void k_pnpg_sq_sum(const float* __restrict__ a,
+                                                     const float* __restrict__ w, long long n,
+                                                     const float* __restrict__ mask,
+                                                     double* __restrict__ partials)
+{
+    __shared__ double sh[256];
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const double x = (double)pm(mask, i, a[i]);
+        v = w ? (double)w[i] * x * x : x * x;
+    }
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (t < off) sh[t] += sh[t + off];
+        __syncthreads();
+    }
+    if (t == 0) partials[blockIdx.x] = sh[0];
+}
+
+// a = a * mask (pixels outside the support set to zero)
+__global__ void k_pnpg_mask(float* __restrict__ a, long long n, const float* __restrict__ mask)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = pm(mask, i, a[i]);
+}
+
 __global__ void k_pnpg_commit(const PnpgState* __restrict__ S, float* __restrict__ alpha,
                               float* __restrict__ pre, const float* __restrict__ newx, long long n)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !S->accept) return;
-    pre[i] = alpha[i];
-    alpha[i] = newx[i];
+    const float a = alpha[i];
+    pre[i] = a;
+    if (!S->keep) alpha[i] = newx[i];
 }
 
 }  // namespace ctprj
@@ -325,6 +398,8 @@ struct ctprj_pnpg {
     // which the trial's kernels write)
     PnpgState h_state{};
     bool h_valid = false;
+    double t_reset = 0.0;         // last Hessian step estimate of reset()
+    int resets = 0;
     ~ctprj_pnpg()
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
@@ -358,7 +433,7 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
     s->d_Imea = d_Imea;
     s->Ts = Ts;
     s->bp = bp;
-    s->pp = PnpgParams{0.5, 0.9, 2.0, 0.25, 1, 0, 1, 0, 2};
+    s->pp = PnpgParams{0.5, 0.9, 2.0, 0.25, 1, 0, 1, 0, 2, 20};
     s->nimg = plan->img_elems;
     s->nsino = plan->sino_elems;
     s->prox_blocks = (s->nimg + 255) / 256;
@@ -381,6 +456,7 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
     st.theta = 0.0;
     st.cumuTol = 4;
     st.cost = INFINITY;
+    st.admmTol = 1e-2;  // PNPG.m / NPG.m default
     hipError_t e = hipMemcpy(s->d_state, &st, sizeof(st), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(s->d_pre, d_alpha0, sizeof(float) * s->nimg, hipMemcpyDeviceToDevice);
@@ -405,6 +481,45 @@ static int pnpg_func(ctprj_pnpg* s, const float* x, double* dst, float* grad, hi
     if (grad) return p->launch_adjoint(s->d_s, grad, st, false);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "pnpg objective");
+}
+
+// reset()'s step estimate stepSizeInit('hessian') (npg/PNPG.m:168-172,
+// npg/Methods.m:108-115) at alpha: with g the (masked) gradient and Phi = Ts*A,
+// t = hessian(g, 2)/||g||^2 = Ts^2 <A g, hw .* A g>/||g||^2, hw the Hessian ray weight of
+// bhc/poissLAlpha.m:15 (or bhc/gaussLAlpha.m:18); NaN gives 1 and 0 gives eps as there.
+// Uses d_xbar and d_s (free after the trial's decision) and the plan's sinogram
+// workspace; synchronises.
+static int pnpg_hessian_step(ctprj_pnpg* s, const float* d_alpha, double* t_out, hipStream_t st)
+{
+    Plan* p = s->plan;
+    float* hw = p->d_sino_ws;
+    int rc = p->launch_forward(d_alpha, s->d_s, st, false);
+    if (rc) return rc;
+    launch_bhc_alpha_ray(s->bp, s->d_s, s->Ts, s->d_Imea, s->nsino, s->Ts, s->d_s, hw,
+                         s->d_ray_part, st);
+    if ((rc = p->launch_adjoint(s->d_s, s->d_xbar, st, false))) return rc;
+    const unsigned gi = (unsigned)((s->nimg + 255) / 256);
+    if (s->d_mask)
+        hipLaunchKernelGGL(k_pnpg_mask, dim3(gi), dim3(256), 0, st, s->d_xbar, s->nimg, s->d_mask);
+    if ((rc = p->launch_forward(s->d_xbar, s->d_s, st, false))) return rc;
+    // <A g, hw .* A g> over the rays, then ||g||^2 over the pixels, each to a device double
+    double* sums = s->d_part;  // two doubles: the prox partials are free here
+    hipLaunchKernelGGL(k_pnpg_sq_sum, dim3((unsigned)s->ray_blocks), dim3(256), 0, st, s->d_s, hw,
+                       s->nsino, nullptr, s->d_ray_part);
+    launch_reduce_partials(s->d_ray_part, s->ray_blocks, 1, sums, st);
+    hipLaunchKernelGGL(k_pnpg_sq_sum, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, s->d_xbar,
+                       nullptr, s->nimg, nullptr, s->d_wpart);
+    launch_reduce_partials(s->d_wpart, s->prox_blocks, 1, sums + 1, st);
+    double h[2] = {0.0, 0.0};
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(h, sums, sizeof(h), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "pnpg hessian step");
+    double t = (double)s->Ts * (double)s->Ts * h[0] / h[1];
+    if (std::isnan(t)) t = 1.0;
+    if (t == 0.0) t = 2.220446049250313e-16;
+    *t_out = t;
+    return 0;
 }
 
 static int state_get(ctprj_pnpg* s, PnpgState* h, hipStream_t st)
@@ -468,6 +583,7 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     else if ((rc = state_get(s, &h, st))) return rc;
     pnpg_begin(h, P);
     h.accept = 0;
+    h.keep = 0;
     if ((rc = state_put(s, &h, st))) return rc;
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
@@ -523,7 +639,14 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
             h.ppp = 0;
             return state_put(s, &h, st);
         } else if (!goodMM || objBar < newObj) {
-            if (!goodMM) h.theta = 1.0;
+            if (!goodMM) {  // reset(): theta = 1, t = min(t, Hessian step at alpha)
+                h.theta = 1.0;
+                double th = 0.0;
+                if ((rc = pnpg_hessian_step(s, d_alpha, &th, st))) return rc;
+                h.t = std::min(h.t, th);
+                s->t_reset = th;
+                s->resets += 1;
+            }
             if (s->innerSearch < s->maxInnerItr && s->admmTol > 1e-6) {
                 s->admmTol /= 10;
                 h.ppp = 0;
@@ -541,11 +664,7 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     // accept (PNPG.m:147-160)
     pnpg_accept(h, P, newObj);
     s->difAlpha = keep ? 0.0 : std::sqrt(sum[3]) / std::max(std::sqrt(sum[4]), 1e-10);
-    if (keep) {
-        e = hipMemcpyAsync(s->d_newx, d_alpha, sizeof(float) * (size_t)s->nimg,
-                           hipMemcpyDeviceToDevice, st);
-        if (e != hipSuccess) return hip_fail(e, "pnpg keep");
-    }
+    h.keep = keep ? 1 : 0;  // the commit then leaves alpha as it is
     if ((rc = state_put(s, &h, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_commit, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_newx, s->nimg);
     e = hipGetLastError();
@@ -607,6 +726,14 @@ extern "C" int ctprj_pnpg_set_method(ctprj_pnpg* s, int npg, int preSteps)
     return 0;
 }
 
+extern "C" int ctprj_pnpg_set_linesearch(ctprj_pnpg* s, int max_shrinks)
+{
+    if (!s) return fail(CTPRJ_EINVAL, "ctprj_pnpg_set_linesearch: NULL solver");
+    if (max_shrinks < 0 || max_shrinks > 1000) return fail(CTPRJ_EINVAL, "max_shrinks must be 0 to 1000");
+    s->pp.maxLs = max_shrinks;
+    return 0;
+}
+
 extern "C" int ctprj_pnpg_set_wavelet(ctprj_pnpg* s, ctprj_wavelet* w, double u)
 {
     return set_prox(s, w, nullptr, u);
@@ -656,9 +783,35 @@ extern "C" int ctprj_pnpg_stats(ctprj_pnpg* s, ctprj_pnpg_info* info, void* stre
     info->restarts = st.restarts;
     info->last_accepted = st.accept;
     info->difAlpha = s->difAlpha;
-    info->admmTol = s->admmTol;
+    info->admmTol = (s->wv || s->tv) ? s->admmTol : st.admmTol;
     info->innerSearch = s->innerSearch;
     info->maxInnerItr = s->maxInnerItr;
+    info->t_reset = s->t_reset;
+    info->resets = s->resets;
+    return 0;
+}
+
+extern "C" int ctprj_debug_pnpg_decide(const double* in, int ls, int moved, double* out,
+                                       int* code)
+{
+    if (!in || !out || !code) return fail(CTPRJ_EINVAL, "ctprj_debug_pnpg_decide: NULL argument");
+    if (ls != 1 && ls != 2) return fail(CTPRJ_EINVAL, "ls must be 1 or 2");
+    PnpgState s{};
+    s.cost = in[0];
+    s.f_old = in[1];
+    s.f_new = in[2];
+    s.theta = in[3];
+    s.admmTol = in[4];
+    s.t = s.preT = 1.0;
+    s.newTheta = in[3];
+    s.ppp = 1;
+    s.moved = moved;
+    PnpgParams P{0.5, 0.9, 2.0, 0.25, 1, 0, 1, 0, 2, 20};
+    const int took = pnpg_nonneg_decide(s, P, ls);
+    *code = took ? (s.keep ? 3 : 2) : (s.ppp == 0 && s.restarts ? 0 : 1);
+    out[0] = s.cost;
+    out[1] = s.theta;
+    out[2] = s.admmTol;
     return 0;
 }
 

@@ -92,6 +92,11 @@ class PNPG:
         check(lib().ctprj_pnpg_set_tv(self._h, tv.handle if tv is not None else None,
                                       ctypes.c_double(float(u))))
 
+    def set_linesearch(self, max_shrinks: int = 20):
+        """Failed tests per iteration that shrink the step before the line search gives up
+        (the reference's 20)."""
+        check(lib().ctprj_pnpg_set_linesearch(self._h, int(max_shrinks)))
+
     def step(self, alpha: torch.Tensor, stream=None):
         check(lib().ctprj_pnpg_step(self._h, _ptr(alpha), _stream_ptr(stream)))
 

@@ -358,6 +358,9 @@ typedef struct ctprj_pnpg_info {
     double admmTol;    /* wavelet prox: ADMM tolerance factor (npg/PNPG.m admmTol) */
     int innerSearch;   /* wavelet prox: ADMM iterations of the last trial */
     int maxInnerItr;   /* wavelet prox: ADMM iteration cap */
+    double t_reset;    /* iterative priors: the Hessian step estimate of the last reset()
+                          (npg/PNPG.m:168-172), 0 before any */
+    int resets;        /* iterative priors: reset() calls (given-up line searches) */
 } ctprj_pnpg_info;
 
 /* Solver state over `plan` (all slices share one objective): Imea (device, packed like
@@ -370,6 +373,10 @@ int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const double* knots
  * Enqueues a fixed kernel sequence on `stream` with no host synchronisation and no
  * allocation: calls may be captured in a HIP graph and replayed. */
 int ctprj_pnpg_step(ctprj_pnpg* solver, float* d_alpha, void* stream);
+/* Failed majorization tests per iteration that shrink the step before the line search
+ * gives up (goodMM = false, which resets theta and, with an iterative prior, the step
+ * from the Hessian): the reference's 20 (npg/PNPG.m:85, npg/NPG.m:85) unless changed. */
+int ctprj_pnpg_set_linesearch(ctprj_pnpg* solver, int max_shrinks);
 /* Copies the device scalars to `info` (synchronises `stream`). */
 int ctprj_pnpg_stats(ctprj_pnpg* solver, ctprj_pnpg_info* info, void* stream);
 /* Selects the method before the first trial: npg = 0 PNPG (npg/PNPG.m, the default
@@ -412,6 +419,13 @@ const char* ctprj_version(void);
  * allocation the library makes from now on fails as out of memory (once); 0 disarms.
  * No reference counterpart. */
 int ctprj_debug_fail_alloc(int k);
+/* The PNPG decision after a finished line search on the non-negativity path
+ * (npg/PNPG.m:99-160 with u = 0), evaluated on the host by the same function the device
+ * runs: in = {cost, f_old, f_new, theta (= this trial's newTheta), admmTol}, ls 1 (the
+ * majorization test held) or 2 (given up), moved = ||xbar - alpha||_1 != 0.  code: 0
+ * restart, 1 retry with admmTol/10, 2 take newX, 3 keep alpha; out = {cost, theta,
+ * admmTol} after it.  For tests of the rule; no device work. */
+int ctprj_debug_pnpg_decide(const double* in, int ls, int moved, double* out, int* code);
 
 #ifdef __cplusplus
 }

@@ -37,6 +37,8 @@ class PnpgOracle:
         self.goodStep = True
         self.newTheta = 1.0
         self.coef = 0.0
+        self.admmTol = 1e-2
+        self.max_ls = 20
         self.log = []
 
     def _masked(self, func):
@@ -79,7 +81,7 @@ class PnpgOracle:
                 self.log.append(("grow", f_new))
                 return 0
             return 1
-        if self.ppp <= 20 and self.t > 0:
+        if self.ppp <= self.max_ls and self.t > 0:
             self.t /= 0.5
             self.goodStep = False
             if self.incStep:
@@ -110,24 +112,49 @@ class PnpgOracle:
         ls = self._linesearch(holds, f_new)
         if ls == 0:
             return
-        if ls == 1 and self.p > 0 and f_new > self.cost and self.coef != 0.0:
-            self.theta = 1.0
-            self.ppp = 0
-            self.log.append(("restart", f_new))
-            return
-        self._accept(newx, f_new)
+        code = self._decide(ls, f_old, f_new, bool(np.any(xbar != self.alpha)))
+        if code >= 2:
+            self._accept(self.alpha if code == 3 else newx, self.cost if code == 3 else f_new)
+
+    def _decide(self, ls, f_old, f_new, moved):
+        """PNPG.m:99-160 with u = 0 after a finished line search (goodMM = ls == 1).  The
+        non-negativity prox (customMax) reports one inner iteration, below maxInnerItr =
+        100, so the admmTol schedule applies.  Returns 0 restart, 1 retry with admmTol / 10,
+        2 take newX, 3 keep alpha ("force it to converge")."""
+        good = ls == 1
+        if f_new - self.cost > 0:
+            if good and moved:
+                self.theta = 1.0
+                self.ppp = 0
+                self.log.append(("restart", f_new))
+                return 0
+            if not good or f_old < f_new:
+                if not good:
+                    self.theta = 1.0
+                if self.admmTol > 1e-6:
+                    self.admmTol /= 10
+                    self.ppp = 0
+                    self.log.append(("retry", f_new))
+                    return 1
+                return 3
+        return 2
 
 
 class PnpgWaveletOracle(PnpgOracle):
     """npg/PNPG.m:50-167 with the 'wvltADMM' prox (bhc/beamhardenSpline.m:204-209):
     newX = admm(xbar - grad/t, u/t, admmTol*difAlpha, maxInnerItr, alpha) on the images
     of ``shape``, objective f + u ||mdwt(x)||_1, and the admmTol / maxInnerItr schedule of
-    PNPG.m:113-146.  A failed line search sets theta = 1 (reset() without re-estimating
-    t).  ``h``, ``L``: the wavelet filter and levels."""
+    PNPG.m:113-146.  A given-up line search calls reset() (PNPG.m:168-172): theta = 1
+    and t = min(t, hess_step(alpha)), hess_step giving stepSizeInit('hessian')
+    (npg/Methods.m:108-115) at the current iterate when supplied.  ``h``, ``L``: the
+    wavelet filter and levels."""
 
     def __init__(self, func, alpha0, t0, u, h, L, shape, npg=False, pre_steps=2, mask=None,
-                 coef_mask=None):
+                 coef_mask=None, hess_step=None):
         super().__init__(func, alpha0, t0, npg, pre_steps, mask)
+        self.hess_step = hess_step
+        self.t_reset = 0.0
+        self.resets = 0
         from oracle import wavelet_oracle as W
         self.W = W
         self.ops = None
@@ -180,6 +207,12 @@ class PnpgWaveletOracle(PnpgOracle):
             if (not goodMM) or objBar < newObj:
                 if not goodMM:
                     self.theta = 1.0
+                    if self.hess_step is not None:
+                        th = self.hess_step(self.alpha)
+                        th = 1.0 if np.isnan(th) else (np.finfo(float).eps if th == 0 else th)
+                        self.t = min(self.t, th)
+                        self.t_reset = th
+                        self.resets += 1
                 if self.innerSearch < self.maxInnerItr and self.admmTol > 1e-6:
                     self.admmTol /= 10
                     self.ppp = 0
@@ -202,8 +235,9 @@ class PnpgTVOracle(PnpgWaveletOracle):
     """The same trials with the TV prox ('tvl1' / 'tviso': utils/TV.m denoise with
     innerThresh = admmTol*difAlpha, penalty utils/tlv.m)."""
 
-    def __init__(self, func, alpha0, t0, u, tv, shape, npg=False, pre_steps=2):
-        super().__init__(func, alpha0, t0, u, None, 0, shape, npg, pre_steps)
+    def __init__(self, func, alpha0, t0, u, tv, shape, npg=False, pre_steps=2, hess_step=None):
+        super().__init__(func, alpha0, t0, u, None, 0, shape, npg, pre_steps,
+                         hess_step=hess_step)
         from oracle import tv_oracle as T
         self.T = T
         self.tv = tv

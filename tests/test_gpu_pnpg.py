@@ -250,3 +250,78 @@ def test_pnpg_masked_wavelet_prior_matches_restatement(oracle_mod):
     assert not got[~mask.ravel()].any()
     assert st["iterations"] == ref.p and st["iterations"] >= 5
     assert np.abs(got - ref.alpha).max() <= 1e-4 * np.abs(ref.alpha).max()
+
+
+def test_pnpg_reset_step_estimate_matches_restatement(oracle_mod):
+    """Given-up line searches (forced: a failed majorization test gives up at once)
+    call reset() (npg/PNPG.m:168-172): theta = 1 and t = min(t, stepSizeInit('hessian'))
+    with the Hessian step Ts^2 <A g, hw A g>/||g||^2 at the current iterate
+    (npg/Methods.m:108-115, bhc/poissLAlpha.m:15-26).  The device's estimates match the
+    FP64 restatement's (1e-5), and so do the decisions and the objective."""
+    import torch
+    from oracle import bhc_oracle as B
+    from oracle.pnpg_oracle import PnpgWaveletOracle
+    from imgrecsrc_amd.phantom import ellipse_params, rasterise
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf
+    geo = (64, 91, 72, 72, 1.0, 1.0, 250.0)
+    n = geo[0]
+    knots = B.spline_knots(B.kappa_logspace(E, 0.05, 20.0))
+    Ie = np.exp(-0.5 * ((np.arange(E) - 3.0) / 1.5) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    cf = oracle_mod.conf(*geo)
+    truth = rasterise(ellipse_params(81, n), n)
+    raw = oracle_mod.forward(cf, truth)[0].ravel()
+    Ts = 2.0 / raw.max()
+    BL, _, _ = B.b1_iout_stable(knots, Ts * raw, Ie)
+    counts = np.random.default_rng(82).poisson(1e5 * BL)
+    Imea = counts / counts.max()
+    A = lambda x: oracle_mod.forward(cf, x.reshape(n, n))[0].ravel()  # noqa: E731
+    At = lambda y: oracle_mod.adjoint(cf, y.reshape(geo[2], geo[1]))[0].ravel()  # noqa: E731
+
+    def func(x, grad=True):
+        BLx, sBL, _ = B.b1_iout_stable(knots, Ts * A(x), Ie)
+        f = B._nll(BLx, Imea)
+        if not grad:
+            return f, None
+        return f, At(Ts * (1 - Imea / BLx) * sBL)
+
+    def hess_step(x):
+        BLx, sBL, ssBL = B.b1_iout_stable(knots, Ts * A(x), Ie)
+        g = At(Ts * (1 - Imea / BLx) * sBL)
+        hw = (sBL / BLx) ** 2 * Imea + ssBL * (1 - Imea / BLx)
+        Ag = Ts * A(g)
+        return float(hw @ (Ag * Ag) / (g @ g))
+
+    h = daubcqf(4)[0]
+    L = 3
+    proj = Projector([geo])
+    alpha0 = np.full(n * n, 0.05)
+    alpha = torch.from_numpy(alpha0.astype(np.float32)).cuda()
+    solver = PNPG(proj, torch.from_numpy(Imea.astype(np.float32)).cuda(), knots, Ie, Ts, alpha)
+    t0 = 0.3 * solver.t0  # steps too long: the test fails early and often
+    solver = PNPG(proj, torch.from_numpy(Imea.astype(np.float32)).cuda(), knots, Ie, Ts, alpha,
+                  t0=t0)
+    u = 2e-3 * solver.t0
+    wv = Wavelet(n, n, h, L)
+    solver.set_wavelet(wv, u)
+    solver.set_linesearch(0)
+    ref = PnpgWaveletOracle(func, alpha0, t0, u, h, L, (n, n), hess_step=hess_step)
+    ref.max_ls = 0
+    log = []
+    for k in range(14):
+        solver.step(alpha)
+        st = solver.stats()
+        ref.trial()
+        log.append((ref.log[-1][0], st["last_accepted"], st["cost"], ref.cost, st["t"], ref.t,
+                    st["resets"], ref.resets, st["t_reset"], ref.t_reset))
+        assert st["resets"] == ref.resets, log
+        if ref.resets:
+            assert st["t_reset"] == pytest.approx(ref.t_reset, rel=1e-5), log
+        assert st["t"] == pytest.approx(ref.t, rel=1e-5), log
+        assert (st["last_accepted"] == 1) == (ref.log[-1][0] == "accept"), log
+        if np.isfinite(ref.cost):
+            assert abs(st["cost"] - ref.cost) <= 1e-5 * abs(ref.cost), log
+    print("\n".join(map(str, log)))
+    assert ref.resets >= 1

@@ -97,3 +97,56 @@ def test_wavelet_prior_reaches_the_regularised_minimum():
     assert F(o.alpha) <= F(x) * (1 + 1e-9)
     assert np.linalg.norm(o.alpha - x) <= 1e-6 * np.linalg.norm(x)
 
+
+
+DECIDE_CASES = [
+    # (cost, f_old, f_new, ls, moved, admmTol): expected code
+    ((10.0, 9.0, 9.5, 1, 1, 1e-2), 2),    # lower objective: take newX
+    ((10.0, 9.0, 10.5, 1, 1, 1e-2), 0),   # higher, momentum moved xbar: restart
+    ((10.0, 11.0, 10.5, 1, 0, 1e-2), 2),  # higher, no momentum, objBar >= newObj: take
+    ((10.0, 9.0, 10.5, 1, 0, 1e-2), 1),   # higher, no momentum, objBar < newObj: retry
+    ((10.0, 9.0, 10.5, 1, 0, 1e-6), 3),   # ... with admmTol used up: keep alpha
+    ((10.0, 12.0, 10.5, 2, 1, 1e-3), 1),  # given up, higher: reset, retry
+    ((10.0, 12.0, 10.5, 2, 1, 1e-7), 3),  # given up, admmTol used up: keep alpha
+    ((10.0, 12.0, 9.0, 2, 1, 1e-2), 2),   # given up but lower: take newX
+]
+
+
+def test_decision_rule_restatement():
+    """PNPG.m:99-160 (u = 0): the restart needs momentum (xbar != alpha), a higher
+    objective without it retries through the admmTol schedule and then keeps alpha, and a
+    given-up line search resets theta."""
+    from oracle.pnpg_oracle import PnpgOracle
+    for (cost, fo, fn, ls, moved, tol), want in DECIDE_CASES:
+        o = PnpgOracle(lambda x, grad=True: (0.0, None), np.zeros(3), 1.0)
+        o.cost, o.admmTol, o.theta = cost, tol, 0.7
+        code = o._decide(ls, fo, fn, bool(moved))
+        assert code == want, ((cost, fo, fn, ls, moved, tol), code)
+        if ls == 2 and code in (1, 3):
+            assert o.theta == 1.0
+        if code == 1:
+            assert o.admmTol == pytest.approx(tol / 10)
+
+
+def test_device_decision_rule_matches_restatement():
+    """The library's decision function (the one k_pnpg_decide runs, called on the host)
+    gives the restatement's outcome, theta and admmTol on every case."""
+    import ctypes
+    import os
+    from imgrecsrc_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libctprj.so not built")
+    from oracle.pnpg_oracle import PnpgOracle
+    L = _lib.lib()
+    for (cost, fo, fn, ls, moved, tol), want in DECIDE_CASES:
+        inp = (ctypes.c_double * 5)(cost, fo, fn, 0.7, tol)
+        out = (ctypes.c_double * 3)()
+        code = ctypes.c_int(-1)
+        _lib.check(L.ctprj_debug_pnpg_decide(inp, ls, moved, out, ctypes.byref(code)))
+        o = PnpgOracle(lambda x, grad=True: (0.0, None), np.zeros(3), 1.0)
+        o.cost, o.admmTol, o.theta = cost, tol, 0.7
+        ref = o._decide(ls, fo, fn, bool(moved))
+        assert code.value == ref == want
+        assert out[2] == pytest.approx(o.admmTol)
+        if ref in (0, 1):
+            assert out[1] == o.theta
