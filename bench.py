@@ -30,31 +30,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-FAN1024 = (1024, 1024, 360, 360, 1.0, 1.0, 4096.0)
-PAR256 = (256, 367, 180, 360, 1.0, 1.0, 0.0)
-BHC512 = (512, 729, 360, 360, 1.0, 1.0, 2000.0)
-PAR512 = (512, 729, 720, 1440, 1.0, 1.0, 0.0)
-WORKLOADS = {"fan1024": FAN1024, "par256": PAR256, "fan512": BHC512, "par512": PAR512}
+
+from imgrecsrc_amd.workloads import FAN1024, FAN512, PAR256, STACK512, ragged_geoms  # noqa: E402
+WORKLOADS = {"fan1024": FAN1024, "par256": PAR256, "fan512": FAN512, "par512": STACK512}
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
-
-
-def ragged_geoms(seed=4, count=64):
-    """64 slices: n from {128..1024}, views U{90..720}, parallel or fan with equal odds,
-    bins = ceil(n*sqrt(2)) + 4; fan d = 4n over 360 degrees (prjFull = views), parallel
-    over 180 degrees (prjFull = 2 views)."""
-    import numpy as np
-    rng = np.random.Generator(np.random.PCG64(seed))
-    out = []
-    for _ in range(count):
-        n = int(rng.choice([128, 256, 384, 512, 768, 1024]))
-        na = int(rng.integers(90, 721))
-        fan = bool(rng.integers(0, 2))
-        W = int(np.ceil(n * np.sqrt(2))) + 4
-        out.append((n, W, na, na, 1.0, 1.0, 4.0 * n) if fan else (n, W, na, 2 * na, 1.0, 1.0, 0.0))
-    return out
 
 
 def timed(fn, reps, stream):
@@ -153,53 +135,144 @@ def pmc_summary(fetch_dir, write_dir, kernel_substr, tag, command):
     print(json.dumps(out))
 
 
-def cpu_baseline(geo, img, sino, reps=3):
-    """Reference cpuPrj (64 pthreads, prj/cpuPrj.c:40) fwd + adj on the same slice."""
+VALU_PEAK_CLOCK_GHZ = 2.4   # MI355X peak engine clock
+VALU_LANES_PER_CU = 64      # 4 SIMDs x 16 lanes: one non-packed FP32 lane-op each per clock
+VALU_CUS = 256
+
+
+def valu_peak_lane_ops():
+    """Non-packed VALU issue slots per second of the whole GPU (lane-ops/s)."""
+    return VALU_CUS * VALU_LANES_PER_CU * VALU_PEAK_CLOCK_GHZ * 1e9
+
+
+def valu_summary(sq_dir, kernel_substr, tag, command):
+    """Reduce a rocprofv3 --pmc run of SQ counters (-f csv) to profiles/<tag>_valu.json:
+    per kernel of one operator call ('+'-separated names, each at its trailing dispatches
+    of the last grid size: the timed shape) the mean per dispatch of every counter
+    collected, and the call's sums.  lane_ops = SQ_INSTS_VALU * 64 (wave64 issue slots)."""
+    import glob
+    import statistics
+    files = glob.glob(os.path.join(sq_dir, "**", "*counter_collection.csv"), recursive=True)
+    if not files:
+        raise SystemExit(f"no counter_collection.csv under {sq_dir}")
+    rows = [r for f in files for r in _csv_rows(f)]
+    parts = [k for k in kernel_substr.split("+") if k]
+    out = {"command": command, "kernel": kernel_substr, "per_kernel": {}}
+    totals = {}
+    for part in parts:
+        kern = [r for r in rows
+                if _col(r, "Kernel_Name").split("<")[0].split("(")[0].split("::")[-1] == part]
+        if not kern:
+            continue
+        kern.sort(key=lambda r: int(float(_col(r, "Dispatch_Id"))))
+        g = int(float(_col(kern[-1], "Grid_Size")))
+        by = {}
+        for r in reversed(kern):
+            if int(float(_col(r, "Grid_Size"))) != g:
+                break
+            by.setdefault(_col(r, "Counter_Name"), []).append(float(_col(r, "Counter_Value")))
+        means = {c: statistics.mean(v) for c, v in by.items()}
+        means["dispatches"] = max(len(v) for v in by.values())
+        means["grid"] = g
+        out["per_kernel"][part] = means
+        for c, v in means.items():
+            if c.startswith(("SQ_", "GRBM_", "TA_", "TD_", "TCP_")):
+                totals[c] = totals.get(c, 0.0) + v
+    if not out["per_kernel"]:
+        raise SystemExit(f"no dispatch of {kernel_substr}")
+    out["call"] = totals
+    if "SQ_INSTS_VALU" in totals:
+        out["lane_ops_per_call"] = totals["SQ_INSTS_VALU"] * 64.0
+    path = os.path.join(ROOT, "profiles", f"{tag}_valu.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=1)
+    print(json.dumps(out))
+
+
+def valu_from_profiles(tag):
+    """Lane-ops per call from profiles/<tag>_valu.json, or None."""
+    path = os.path.join(ROOT, "profiles", f"{tag}_valu.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def cpu_model():
+    """The host CPU's model name and logical CPU count (/proc/cpuinfo)."""
+    name, count = "unknown", 0
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    count += 1
+                    name = line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return f"{name} ({count} logical CPUs on the host)"
+
+
+def cpu_baseline(geo, img, sino, mode, gpu_sino, gpu_back, reps=3):
+    """Reference cpuPrj (64 pthreads, prj/cpuPrj.c:40) forward and adjoint of the same
+    slice, timed separately (wall clock, median of `reps` after a warm-up).  With the GPU
+    outputs given, also their relative L2 errors against the reference's outputs and the
+    FP64 oracle's (at the kernels' angles; the adjoint of the GPU's own sinogram)."""
     import numpy as np
     import oracle
     n, W, npj, pf, ds, er, d = geo
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     if oracle.ref_available():
-        kind = "reference"
-        run = lambda: (oracle.ref_run(n, W, npj, pf, ds, er, d, img, oracle.FWD_BIT),  # noqa: E731
-                       oracle.ref_run(n, W, npj, pf, ds, er, d, sino, oracle.BWD_BIT))
-        threads = 64
+        kind, threads = "reference", 64
+        fwd = lambda: oracle.ref_run(n, W, npj, pf, ds, er, d, img, oracle.FWD_BIT)  # noqa: E731
+        adj = lambda: oracle.ref_run(n, W, npj, pf, ds, er, d, sino, oracle.BWD_BIT)  # noqa: E731
     else:
-        kind = "port"
+        kind, threads = "port", oracle._nth()
         cf = oracle.conf(*geo)
-        run = lambda: (oracle.forward(cf, img), oracle.adjoint(cf, sino))  # noqa: E731
-        threads = min(8, cores or 1)
-    run()  # warm-up
-    ts = []
-    for _ in range(reps):
-        t0 = time.perf_counter()
-        run()
-        ts.append(time.perf_counter() - t0)
-    return float(np.median(ts)), kind, threads, cores
+        fwd = lambda: oracle.forward(cf, img)[0]  # noqa: E731
+        adj = lambda: oracle.adjoint(cf, sino)[0]  # noqa: E731
+
+    def med(fn):
+        out = fn()  # warm-up
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), out
+
+    t_f, ref_f = med(fwd)
+    t_a, ref_a = med(adj)
+    res = {"fwd_s": round(t_f, 4), "adj_s": round(t_a, 4), "kind": kind, "threads": threads,
+           "cores": cores, "cpu_model": cpu_model(), "vs_reference": None, "vs_fp64": None}
+    if gpu_sino is not None:
+        def rel(a, b):
+            a = np.asarray(a, np.float64).ravel()
+            b = np.asarray(b, np.float64).ravel()
+            return float(f"{np.linalg.norm(a - b) / np.linalg.norm(b):.3e}")
+        if kind == "reference":
+            res["vs_reference"] = {"fwd": rel(gpu_sino, ref_f), "adj": rel(gpu_back, ref_a)}
+        cf = oracle.conf(*geo, angle_mode={0: 1, 1: 2, 2: 3, 3: 4}[mode])
+        res["vs_fp64"] = {"fwd": rel(gpu_sino, oracle.forward(cf, img)[0]),
+                          "adj": rel(gpu_back, oracle.adjoint(cf, sino)[0])}
+    return res
 
 
-def pnpg_leg(nslices, trials, dev, stream):
-    """PNPG alpha step on a stack of 512^2 parallel slices (720 views over 180 degrees,
-    729 bins), polychromatic Poisson data with I0 = 1e5, E = 20: `trials` line-search
-    trials captured once in a HIP graph and replayed; each trial is one gradient
-    evaluation (forward + FP64 NLL + adjoint) and one cost evaluation (forward)."""
+STACK_CHECK = (0, 137, 300, 511)  # slices of the stack whose iterates are checked in FP64
+
+
+def stack_problem(nslices, dev, tune_reps=2):
+    """The slice stack's polychromatic Poisson problem: 512^2 parallel slices (720 views
+    over 180 degrees, 729 bins), E = 20 spline bases, counts at I0 = 1e5."""
     import numpy as np
     import torch
 
     from imgrecsrc_amd.bhc import Spline, kappa_logspace, normalize_ie, spline_knots
-    from imgrecsrc_amd.phantom import ellipse_params, rasterise
-    from imgrecsrc_amd.pnpg import PNPG
     from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.workloads import STACK512, stack_truth
 
-    log(f"pnpg leg: {nslices} slices x {trials} trials")
-    geo = PAR512
-    P = Projector([geo] * nslices)
-    P.autotune(reps=2)
-    n = geo[0]
-    base = [torch.from_numpy(rasterise(ellipse_params(500 + k, n), n).astype(np.float32)).to(dev)
-            for k in range(8)]
-    truth = torch.cat([torch.roll(base[i % 8], shifts=(i // 8) % 32, dims=0).reshape(-1)
-                       for i in range(nslices)])
+    P = Projector([STACK512] * nslices)
+    P.autotune(reps=tune_reps)
+    truth = torch.from_numpy(stack_truth(nslices)).to(dev).reshape(-1)
     knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
     Ie = normalize_ie(np.exp(-0.5 * ((np.arange(20) - 7.5) / 3.0) ** 2) + 0.05, knots)
     raw = P.forward(truth)
@@ -208,32 +281,181 @@ def pnpg_leg(nslices, trials, dev, stream):
     g = torch.Generator(device=dev).manual_seed(6)
     counts = torch.poisson(1e5 * BL, generator=g)
     Imea = (counts / counts.max()).float()
-    del raw, BL, counts
-    alpha = torch.full_like(truth, 0.02)
+    del raw, BL, counts, truth
+    return P, Imea, knots, Ie, Ts
+
+
+def stack_batched_vs_plain(P, Imea, knots, Ie, Ts, dev, stream):
+    """The stack's forward + adjoint pair and its beam-hardening gradient (poissLAlpha),
+    batched (one call over all slices: one launch per kernel) against the plain path (a
+    one-slice plan called once per slice, on the slice's part of the same buffers)."""
+    import torch
+
+    from imgrecsrc_amd.bhc import poissLAlpha
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.workloads import STACK512, stack_truth
+
+    ns = P.nslices
+    one = Projector([STACK512])
+    one.autotune(reps=2)
+    ni, nr = one.img_elems, one.sino_elems
+    x = torch.from_numpy(stack_truth(ns)).to(dev).reshape(-1)
+    y = torch.empty(P.sino_elems, dtype=torch.float32, device=dev)
+    b = torch.empty(P.img_elems, dtype=torch.float32, device=dev)
+    g = torch.empty(P.img_elems, dtype=torch.float32, device=dev)
+
+    def batched():
+        P.forward(x, out=y)
+        P.adjoint(y, out=b)
+
+    def plain():
+        for k in range(ns):
+            one.forward(x[k * ni:(k + 1) * ni], out=y[k * nr:(k + 1) * nr])
+            one.adjoint(y[k * nr:(k + 1) * nr], out=b[k * ni:(k + 1) * ni])
+
+    def grad_batched():
+        poissLAlpha(Imea, Ie, x, P, Ts, knots)
+
+    def grad_plain():
+        for k in range(ns):
+            poissLAlpha(Imea[k * nr:(k + 1) * nr], Ie, x[k * ni:(k + 1) * ni], one, Ts, knots)
+
+    for fn in (batched, plain, grad_batched, grad_plain):
+        fn()
+    t_b, t_p = timed(batched, 3, stream), timed(plain, 3, stream)
+    t_gb, t_gp = timed(grad_batched, 3, stream), timed(grad_plain, 3, stream)
+    nnz = one.nnz() * ns
+    del g
+    return {"workload": f"{ns} x 512^2 parallel, 720 views x 729 bins",
+            "fwd_adj_batched_ms": round(t_b, 3), "fwd_adj_plain_ms": round(t_p, 3),
+            "fwd_adj_batched_GUPS": round(2 * nnz / t_b / 1e6, 1),
+            "fwd_adj_plain_GUPS": round(2 * nnz / t_p / 1e6, 1),
+            "fwd_adj_speedup": round(t_p / t_b, 3),
+            "grad_batched_ms": round(t_gb, 3), "grad_plain_ms": round(t_gp, 3),
+            "grad_batched_GUPS": round(2 * nnz / t_gb / 1e6, 1),
+            "grad_speedup": round(t_gp / t_gb, 3),
+            "tune_batched": P.get_tune(), "tune_plain": one.get_tune()}
+
+
+def pnpg_leg(nslices, iters, dev, stream, compare=True):
+    """PNPG alpha step (npg/PNPG.m:50-167, non-negativity prox) on the slice stack: one
+    line-search trial (gradient evaluation: forward + FP64 NLL + adjoint; cost
+    evaluation: forward + FP64 NLL; decision) captured once in a HIP graph and replayed
+    until `iters` iterations are accepted; the alpha of STACK_CHECK's slices is kept after
+    every accepted iteration, and the device's f and grad_alpha of those slices at each
+    kept iterate are evaluated afterwards (for the FP64 comparison of the CPU leg)."""
+    import numpy as np
+    import torch
+
+    from imgrecsrc_amd.bhc import poissLAlpha
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.workloads import STACK512
+
+    log(f"pnpg leg: {nslices} slices, {iters} iterations")
+    P, Imea, knots, Ie, Ts = stack_problem(nslices, dev)
+    stack_cmp = None
+    if compare:
+        stack_cmp = stack_batched_vs_plain(P, Imea, knots, Ie, Ts, dev, stream)
+        log(f"stack batched vs plain: {stack_cmp}")
+    check = [k for k in STACK_CHECK if k < nslices]
+    img = STACK512[0] ** 2
+    alpha = torch.full((P.img_elems,), 0.02, dtype=torch.float32, device=dev)
     solver = PNPG(P, Imea, knots, Ie, Ts, alpha)
-    solver.step(alpha)  # warm-up trial outside the graph
     torch.cuda.synchronize()
     graph = torch.cuda.CUDAGraph()
     cap = torch.cuda.Stream()
     with torch.cuda.graph(graph, stream=cap):
-        for _ in range(trials):
-            solver.step(alpha)
+        solver.step(alpha)
     torch.cuda.synchronize()
-    c0 = solver.stats()["cost"]
+    st0 = solver.stats()
+    c0 = st0["cost"]
+    snaps = [alpha.view(-1, img)[check].clone()]
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
-    e0.record(stream)
-    graph.replay()
-    e1.record(stream)
-    e1.synchronize()
-    ms = e0.elapsed_time(e1)
+    ms, trials, last, costs = 0.0, 0, 0, [c0]
+    t_wall = time.perf_counter()
+    while last < iters and trials < 4 * iters:
+        e0.record(stream)
+        graph.replay()
+        e1.record(stream)
+        e1.synchronize()
+        ms += e0.elapsed_time(e1)
+        trials += 1
+        st = solver.stats()
+        if st["iterations"] > last:
+            last = st["iterations"]
+            costs.append(st["cost"])
+            snaps.append(alpha.view(-1, img)[check].clone())
+    wall = time.perf_counter() - t_wall
     st = solver.stats()
-    nnz = Projector([geo]).nnz() * nslices
-    return {"workload": f"{nslices} x 512^2 parallel, 720 views x 729 bins, E=20, I0=1e5",
-            "trials": trials, "ms_per_trial": round(ms / trials, 3),
-            "GUPS": round(3 * nnz * trials / (ms * 1e-3) / 1e9, 1),
-            "cost_before": c0, "cost_after": st["cost"], "iterations": st["iterations"],
-            "rejects": st["rejects"], "restarts": st["restarts"], "t": st["t"]}
+    del solver, graph
+    nnz = Projector([STACK512]).nnz() * nslices
+    # the device's f and grad_alpha of the checked slices at every kept iterate
+    P4 = Projector([STACK512] * len(check))
+    Im4 = Imea.view(-1, STACK512[1] * STACK512[2])[check].reshape(-1).contiguous()
+    dev_f, dev_g, alphas = [], [], []
+    for a in snaps:
+        f, g = poissLAlpha(Im4, Ie, a.reshape(-1), P4, Ts, knots)
+        dev_f.append(f.item())
+        dev_g.append(g.cpu().numpy())
+        alphas.append(a.cpu().numpy())
+    out = {"workload": f"{nslices} x 512^2 parallel, 720 views x 729 bins, E=20, I0=1e5",
+           "iterations": st["iterations"], "trials": trials,
+           "ms_per_trial": round(ms / trials, 3),
+           "ms_per_iteration": round(ms / max(1, st["iterations"]), 3),
+           "wall_s": round(wall, 3),
+           "GUPS": round(3 * nnz * trials / (ms * 1e-3) / 1e9, 1),
+           "cost_first": c0, "cost_last": st["cost"],
+           "cost_monotone": bool(all(b <= a for a, b in zip(costs, costs[1:]))),
+           "rejects": st["rejects"], "restarts": st["restarts"], "t": st["t"],
+           "checked_slices": check, "batched_vs_plain": stack_cmp}
+    check_data = {"alphas": alphas, "f": dev_f, "g": dev_g,
+                  "Imea": Im4.cpu().numpy().astype(np.float64), "Ie": Ie, "knots": knots,
+                  "Ts": Ts, "mode": P4.modes()[0]}
+    return out, check_data
+
+
+def stack_fp64_check(cd):
+    """FP64 f and grad_alpha (bhc/poissLAlpha.m restated, over the oracle's stored A) of
+    the checked slices at every kept iterate; relative errors of the device's values."""
+    import numpy as np
+
+    import oracle
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.workloads import STACK512
+
+    ANGLE_MODE = {0: 1, 1: 2, 2: 3, 3: 4}
+    t0 = time.perf_counter()
+    M = oracle.Matrix(oracle.conf(*STACK512, angle_mode=ANGLE_MODE[cd["mode"]]))
+    t_build = time.perf_counter() - t0
+    Ts = cd["Ts"]
+    Phi = lambda a: Ts * M.forward(a).ravel()  # noqa: E731
+    Phit = lambda y: Ts * M.adjoint(y).ravel()  # noqa: E731
+    pol = lambda k, s, I: B.b1_iout_stable(k, s, I)  # noqa: E731
+    from concurrent.futures import ThreadPoolExecutor
+
+    def check(k):
+        a, f, g = cd["alphas"][k], cd["f"][k], cd["g"][k]
+        f64, g64, _ = B.poiss_l_alpha(cd["Imea"], cd["Ie"], a.astype(np.float64).ravel(), Phi,
+                                      Phit, pol, cd["knots"])
+        return abs(f - f64) / abs(f64), float(np.linalg.norm(g.ravel() - g64) / np.linalg.norm(g64))
+
+    # numpy's array operations and the oracle's sparse products release the GIL: the
+    # iterates are checked by a pool of threads
+    n_it = len(cd["alphas"])
+    ef, eg = [0.0] * n_it, [0.0] * n_it
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=8) as ex:
+        for k, (e1, e2) in enumerate(ex.map(check, range(n_it))):
+            ef[k], eg[k] = e1, e2
+            log(f"FP64 check iterate {k}: f {e1:.2e}, grad {e2:.2e}")
+    t_eval = time.perf_counter() - t0
+    return {"f_rel_err": [float(f"{e:.3e}") for e in ef],
+            "grad_rel_l2": [float(f"{e:.3e}") for e in eg],
+            "max_f_rel_err": max(ef), "max_grad_rel_l2": max(eg),
+            "fp64_s_per_eval": round(t_eval / len(ef), 3), "matrix_build_s": round(t_build, 2),
+            "matrix_nnz": M.nnz}
 
 
 def main():
@@ -248,17 +470,23 @@ def main():
                     help="also run a plain 256 MiB device copy (PMC scaling reference)")
     ap.add_argument("--pnpg-slices", type=int, default=512,
                     help="slices of the 512^2 parallel stack in the PNPG leg (0: skip)")
-    ap.add_argument("--pnpg-trials", type=int, default=8,
-                    help="graph-captured PNPG line-search trials to time")
+    ap.add_argument("--pnpg-iters", type=int, default=50,
+                    help="PNPG iterations of the stack leg (graph-replayed trials)")
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH_DIR", "WRITE_DIR", "KERNEL", "TAG"),
                     help="reduce two rocprofv3 --pmc csv runs to profiles/TAG_pmc.json")
     ap.add_argument("--pmc-command", default="", help="command recorded in the summary")
+    ap.add_argument("--valu-summary", nargs=3, metavar=("SQ_DIR", "KERNEL", "TAG"),
+                    help="reduce a rocprofv3 --pmc csv run of SQ counters to "
+                         "profiles/TAG_valu.json")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: a later run with the same file reuses the "
                          "shapes the first one chose (profiling runs)")
     args = ap.parse_args()
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.pmc_command)
+        return
+    if args.valu_summary:
+        valu_summary(*args.valu_summary, args.pmc_command)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -362,7 +590,7 @@ def main():
     if not args.no_extras:
         # ---- beam-hardening gradient evaluations (512^2 fan, E = 20)
         log("bhc leg")
-        bg = BHC512
+        bg = FAN512
         PB = Projector([bg])
         PB.autotune(reps=3)
         knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
@@ -553,17 +781,46 @@ def main():
                                  "path": "prj_setup + prj_run(FWD), prj_run(BWD) on pageable "
                                          "host arrays (the MEX boundary), default shapes"}
 
+    check_data = None
     if not args.no_extras and args.pnpg_slices > 0:
-        out["pnpg"] = pnpg_leg(args.pnpg_slices, args.pnpg_trials, dev, stream)
+        out["pnpg"], check_data = pnpg_leg(args.pnpg_slices, args.pnpg_iters, dev, stream)
+
+    if not args.no_extras:
+        # ---- the 256^2 parallel slice (180 views over 360, 367 bins): plain GPU times
+        PC1 = Projector([PAR256])
+        PC1.autotune(reps=3)
+        c1_img = torch.from_numpy(ellipse_phantom(256, 1)).to(dev).reshape(-1)
+        c1_s = PC1.forward(c1_img)
+        c1_b = PC1.adjoint(c1_s)
+        out["par256"] = {"workload": "par 256^2, 180 views (prjFull 360), 367 bins",
+                         "nnz": PC1.nnz(),
+                         "gpu_fwd_ms": round(timed(lambda: PC1.forward(c1_img, out=c1_s), 20,
+                                                   stream), 4),
+                         "gpu_adj_ms": round(timed(lambda: PC1.adjoint(c1_s, out=c1_b), 20,
+                                                   stream), 4)}
 
     if not args.no_cpu_baseline:
         log("cpu baseline (reference cpuPrj)")
         sino_np = sino.cpu().numpy()
-        t_cpu, kind, threads, cores = cpu_baseline(geo, img_np, sino_np)
-        out["cpu_baseline"] = {"value": round(2.0 * nnz / t_cpu / 1e9, 3), "unit": "GUPS",
-                               "cores": cores, "threads": threads, "kind": kind,
+        back_np = back.cpu().numpy()
+        cb = cpu_baseline(geo, img_np, sino_np, P.modes()[0], sino_np, back_np)
+        out["cpu_baseline"] = {"value": round(2.0 * nnz / (cb["fwd_s"] + cb["adj_s"]) / 1e9, 3),
+                               "unit": "GUPS", "cores": cb["cores"], "threads": cb["threads"],
+                               "kind": cb["kind"], "cpu_model": cb["cpu_model"],
+                               "fwd_s": cb["fwd_s"], "adj_s": cb["adj_s"],
                                "sample": f"one fwd + one adj of the {args.workload} slice, "
-                                         f"median of 3 after 1 warm-up ({t_cpu:.2f} s/pair)"}
+                                         f"median of 3 after 1 warm-up each",
+                               "gpu_vs_reference": cb["vs_reference"],
+                               "gpu_vs_fp64": cb["vs_fp64"]}
+        if "par256" in out:
+            c1 = cpu_baseline(PAR256, ellipse_phantom(256, 1), c1_s.cpu().numpy(), None,
+                              None, None)
+            out["par256"].update({"cpu_fwd_ms": round(1e3 * c1["fwd_s"], 3),
+                                  "cpu_adj_ms": round(1e3 * c1["adj_s"], 3),
+                                  "cpu_kind": c1["kind"], "cpu_threads": c1["threads"]})
+        if check_data is not None:
+            log("FP64 check of the stack's iterates")
+            out["pnpg"]["fp64_check"] = stack_fp64_check(check_data)
     print(json.dumps(out), flush=True)
 
 

@@ -10,40 +10,85 @@
 namespace ctprj {
 
 // Moments M_n(z) = int_0^1 u^n exp(-z u) du, n = 0..3, of one spline segment.
-// |z| <= 2: the series sum_k (-z)^k / (k! (n+k+1)) (18 terms, truncation < 3e-11);
+// |z| <= 2: the series sum_k c_nk z^k, c_nk = (-1)^k / (k! (n+k+1)), 14 terms in Horner
+// form (truncation below 1.2e-8 of M_n; FP32 evaluation error below 6e-7 relative);
 // otherwise the recurrence M_0 = (1-e^-z)/z, M_n = (n M_{n-1} - e^-z)/z, whose error
 // growth is at most n/|z| <= 1.5 per step there.
+constexpr int kMomentTerms = 14;
+
+constexpr double moment_coef(int n, int k)
+{
+    double f = 1.0;
+    for (int i = 2; i <= k; ++i) f *= i;
+    return ((k & 1) ? -1.0 : 1.0) / (f * (n + k + 1));
+}
+
+struct MomentTable {
+    float c[4][kMomentTerms];
+};
+
+constexpr MomentTable make_moment_table()
+{
+    MomentTable t{};
+    for (int n = 0; n < 4; ++n)
+        for (int k = 0; k < kMomentTerms; ++k) t.c[n][k] = (float)moment_coef(n, k);
+    return t;
+}
+
+__constant__ constexpr MomentTable kMoment = make_moment_table();
+
 struct Moments {
     float m0, m1, m2, m3;
 };
 
+// The first NM (1..4) moments by the series (|z| <= 2 only).
+template <int NM>
+__device__ __forceinline__ Moments moments_series(float z)
+{
+    float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int n = 0; n < NM; ++n) {
+        float a = kMoment.c[n][kMomentTerms - 1];
+#pragma unroll
+        for (int k = kMomentTerms - 2; k >= 0; --k) a = fmaf(a, z, kMoment.c[n][k]);
+        m[n] = a;
+    }
+    return Moments{m[0], m[1], m[2], m[3]};
+}
+
 __device__ __forceinline__ Moments moments(float z)
 {
+    if (fabsf(z) <= 2.0f) return moments_series<4>(z);
     Moments M;
-    if (fabsf(z) <= 2.0f) {
-        float t = 1.0f;  // (-z)^k / k!
-        float a0 = 1.0f, a1 = 0.5f, a2 = 1.0f / 3.0f, a3 = 0.25f;
-#pragma unroll
-        for (int k = 1; k <= 18; ++k) {
-            t *= -z * (1.0f / (float)k);
-            a0 = fmaf(t, 1.0f / (float)(k + 1), a0);
-            a1 = fmaf(t, 1.0f / (float)(k + 2), a1);
-            a2 = fmaf(t, 1.0f / (float)(k + 3), a2);
-            a3 = fmaf(t, 1.0f / (float)(k + 4), a3);
-        }
-        M.m0 = a0;
-        M.m1 = a1;
-        M.m2 = a2;
-        M.m3 = a3;
-    } else {
-        const float e = expf(-z);
-        const float iz = 1.0f / z;
-        M.m0 = (1.0f - e) * iz;
-        M.m1 = (M.m0 - e) * iz;
-        M.m2 = (2.0f * M.m1 - e) * iz;
-        M.m3 = (3.0f * M.m2 - e) * iz;
-    }
+    const float e = expf(-z);
+    const float iz = 1.0f / z;
+    M.m0 = (1.0f - e) * iz;
+    M.m1 = (M.m0 - e) * iz;
+    M.m2 = (2.0f * M.m1 - e) * iz;
+    M.m3 = (3.0f * M.m2 - e) * iz;
     return M;
+}
+
+// Weighted moments W_n = w M_n(z) of b1 segment j, w = Delta_j e_j, z = Delta_j s, from
+// the exponentials e_j = exp(-k_j s) and e_{j+1} = exp(-k_{j+1} s) of its two knots
+// (e^-z = e_{j+1}/e_j): the series for |z| <= 2, else the recurrence
+// W_0 = Delta_j (e_j - e_{j+1})/z, W_n = (n W_{n-1} - Delta_j e_{j+1})/z, which needs no
+// exponential of its own.  NM = moments needed (2..4).
+template <int NM>
+__device__ __forceinline__ Moments weighted_moments(float z, float dj, float ej, float ej1)
+{
+    if (fabsf(z) <= 2.0f) {
+        const Moments M = moments_series<NM>(z);
+        const float w = dj * ej;
+        return Moments{w * M.m0, w * M.m1, w * M.m2, w * M.m3};
+    }
+    Moments W{0.0f, 0.0f, 0.0f, 0.0f};
+    const float iz = 1.0f / z, de = dj * ej1;
+    W.m0 = dj * (ej - ej1) * iz;
+    W.m1 = (W.m0 - de) * iz;
+    if (NM >= 3) W.m2 = (2.0f * W.m1 - de) * iz;
+    if (NM >= 4) W.m3 = (3.0f * W.m2 - de) * iz;
+    return W;
 }
 
 // Spectrum basis values for one ray value s, by basis type; calls f(j, phi, dphi, ddphi)
@@ -58,7 +103,8 @@ __device__ __forceinline__ Moments moments(float z)
 //   right half of basis j-1: w_j (M0-M1)           ,  -w_j (k_j (M0-M1) + Delta_j (M1-M2))
 //                            w_j (k_j^2 (M0-M1) + 2 k_j Delta_j (M1-M2) + Delta_j^2 (M2-M3))
 // for phi, phi' and phi'' (all moments at z_j).  These equal the closed forms of
-// bhc/Spline.m:154-201 without their small-s cancellation.
+// bhc/Spline.m:154-201 without their small-s cancellation.  The products w_j M_n are
+// formed by weighted_moments from the knots' exponentials, one exp per knot.
 // b0: basis j is the box [k_j, k_{j+1}] (bhc/Spline.m:205-262):
 //   w_j M0,  -w_j (k_j M0 + Delta_j M1),  w_j (k_j^2 M0 + 2 k_j Delta_j M1 + Delta_j^2 M2).
 // dis: basis j is the node k_{j+1} with weight (k_{j+2} - k_j)/2 (bhc/Spline.m:264-302):
@@ -87,23 +133,24 @@ __device__ __forceinline__ void spline_basis(const BhcParams& P, float s, F&& f)
         return;
     }
     float cp = 0.0f, cd = 0.0f, cdd = 0.0f;  // left half of the pending basis j-1
+    float ej = expf(-P.knot[0] * s);
     for (int j = 0; j <= P.E; ++j) {
         const float kj = P.knot[j], dj = P.delta[j];
-        const float w = dj * expf(-kj * s);
-        const Moments M = moments(dj * s);
-        const float c0 = M.m0 - M.m1;
+        const float ej1 = expf(-P.knot[j + 1] * s);
+        const Moments W = weighted_moments<WANT + 1>(dj * s, dj, ej, ej1);
+        ej = ej1;
+        const float c0 = W.m0 - W.m1;
         if (j >= 1) {  // right half of basis j-1
-            float rp = w * c0, rd = 0.0f, rdd = 0.0f;
-            if (WANT >= 2) rd = -w * fmaf(kj, c0, dj * (M.m1 - M.m2));
+            float rp = c0, rd = 0.0f, rdd = 0.0f;
+            if (WANT >= 2) rd = -fmaf(kj, c0, dj * (W.m1 - W.m2));
             if (WANT >= 3)
-                rdd = w * fmaf(kj * kj, c0,
-                               fmaf(2.0f * kj * dj, M.m1 - M.m2, dj * dj * (M.m2 - M.m3)));
+                rdd = fmaf(kj * kj, c0, fmaf(2.0f * kj * dj, W.m1 - W.m2, dj * dj * (W.m2 - W.m3)));
             f(j - 1, cp + rp, cd + rd, cdd + rdd);
         }
         if (j < P.E) {  // left half of basis j
-            cp = w * M.m1;
-            if (WANT >= 2) cd = -w * fmaf(kj, M.m1, dj * M.m2);
-            if (WANT >= 3) cdd = w * fmaf(kj * kj, M.m1, fmaf(2.0f * kj * dj, M.m2, dj * dj * M.m3));
+            cp = W.m1;
+            if (WANT >= 2) cd = -fmaf(kj, W.m1, dj * W.m2);
+            if (WANT >= 3) cdd = fmaf(kj * kj, W.m1, fmaf(2.0f * kj * dj, W.m2, dj * dj * W.m3));
         }
     }
 }
@@ -137,16 +184,19 @@ __global__ __launch_bounds__(kBhcBlock) void k_poly_iout(BhcParams P, const floa
 
 // --------------------------------------------------------- FP64 block reduce ---
 
+// Sum of v over a work-group of kBhcBlock threads: a butterfly over each wave's 64 lanes
+// (register exchange, no LDS tree), then the waves' sums in wave order.  The summation
+// order is fixed, so the result is deterministic.  Every thread returns the total.
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
-    const int t = threadIdx.x;
-    sh[t] = v;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = (int)threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
     __syncthreads();
-    for (int off = kBhcBlock / 2; off > 0; off >>= 1) {
-        if (t < off) sh[t] += sh[t + off];
-        __syncthreads();
-    }
-    const double r = sh[0];
+    double r = sh[0];
+#pragma unroll
+    for (int w = 1; w < kBhcBlock / 64; ++w) r += sh[w];
     __syncthreads();
     return r;
 }
@@ -206,22 +256,43 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 
 // ------------------------------------------------------------ I-epsilon pass ---
 
-// Each work-group takes kBhcBlock rays: the basis values phi_ij go to LDS, then thread v
-// forms value v of [H upper triangle (row-major j<=k), g, f] over the group's rays:
+// Each work-group takes kBhcBlock rays: the basis values phi_ij (FP32) and the ray
+// weights q_i, r_i, f_i (FP64) go to LDS, then the group forms [H upper triangle
+// (row-major j<=k), g, f] over its rays:
 //   H_jk = sum_i phi_ij phi_ik q_i,  g_j = sum_i phi_ij r_i,  f = sum_i f_i,
 //   Ir_i = sum_j Ie_j phi_ij (FP64), with
 //   Poisson  (bhc/poissLI.m):  q = Imea/Ir^2,       r = 1 - Imea/Ir,  f_i = nll(Ir, Imea)
 //   Gaussian (bhc/gaussLI.m):  q = (1 - Err)/Ir^2,  r = Err/Ir,       f_i = Err^2/2.
+// Register blocking: the outputs are cut into 4x4 blocks of H (upper block triangle)
+// and 4-entry blocks of g (the last one also carries f); thread t takes block
+// slot = t % S over the rays lane, lane + L, ... with lane = t / S (S = slots rounded up
+// to a power of two, L = kBhcBlock / S), so each LDS read of 4 basis values feeds 4 or
+// 16 FP64 FMAs.  The L lane sums of every output are then added in lane order.
+constexpr int kIeStride = ((CTPRJ_MAX_E + 3) / 4) * 4;  // phi row: a multiple of 4 floats
+
+__host__ __device__ inline int ie_blocks(int E) { return (E + 3) / 4; }
+__host__ __device__ inline int ie_slots(int E)
+{
+    const int nb = ie_blocks(E);
+    return nb * (nb + 1) / 2 + nb;
+}
+
 __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s,
                                                           const float* __restrict__ Imea,
                                                           long long count, float sscale,
                                                           double* __restrict__ partials)
 {
-    __shared__ float phi[kBhcBlock][CTPRJ_MAX_E + 1];
+    // the basis values, then (after a barrier) the lane sums share one LDS area
+    constexpr size_t kPhiBytes = sizeof(float) * kBhcBlock * kIeStride;
+    constexpr size_t kSumBytes = sizeof(double) * kBhcBlock * 17;
+    __shared__ __attribute__((aligned(16))) char area[kPhiBytes > kSumBytes ? kPhiBytes : kSumBytes];
     __shared__ double q[kBhcBlock], r[kBhcBlock], ft[kBhcBlock];
+    auto phi = reinterpret_cast<float(*)[kIeStride]>(area);
+    auto lanesum = reinterpret_cast<double(*)[17]>(area);  // [thread][16 block values + f]
     const int t = threadIdx.x;
     const long long i = (long long)blockIdx.x * kBhcBlock + t;
     const int E = P.E;
+    for (int j = 0; j < kIeStride; ++j) phi[t][j] = 0.0f;
     if (i < count) {
         const float si = sscale * s[i];
         const double im = (double)Imea[i];
@@ -241,29 +312,73 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
             ft[t] = nll_term(ir, im);
         }
     } else {
-        for (int j = 0; j < E; ++j) phi[t][j] = 0.0f;
         q[t] = 0.0;
         r[t] = 0.0;
         ft[t] = 0.0;
     }
     __syncthreads();
+    const int nb = ie_blocks(E), nhb = nb * (nb + 1) / 2, nslot = nhb + nb;
+    int S = 1;
+    while (S < nslot) S <<= 1;  // nslot <= 44 for E <= 32, so S <= 64
+    const int L = kBhcBlock / S, slot = t % S, lane = t / S;
+    double acc[17];
+#pragma unroll
+    for (int v = 0; v < 17; ++v) acc[v] = 0.0;
+    if (slot < nhb) {
+        int bi = 0, rem = slot;  // block row bi of the upper triangle holds nb-bi blocks
+        while (rem >= nb - bi) { rem -= nb - bi; ++bi; }
+        const int bj = bi + rem;
+        for (int m = lane; m < kBhcBlock; m += L) {
+            const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bi]);
+            const float4 b = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+            const double w = q[m];
+            const double ad[4] = {(double)a.x * w, (double)a.y * w, (double)a.z * w,
+                                  (double)a.w * w};
+            const double bd[4] = {(double)b.x, (double)b.y, (double)b.z, (double)b.w};
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) acc[4 * x + y] = fma(ad[x], bd[y], acc[4 * x + y]);
+        }
+    } else if (slot < nslot) {
+        const int bj = slot - nhb;
+        const bool last = bj == nb - 1;
+        for (int m = lane; m < kBhcBlock; m += L) {
+            const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+            const double w = r[m];
+            acc[0] = fma((double)a.x, w, acc[0]);
+            acc[1] = fma((double)a.y, w, acc[1]);
+            acc[2] = fma((double)a.z, w, acc[2]);
+            acc[3] = fma((double)a.w, w, acc[3]);
+            if (last) acc[16] += ft[m];
+        }
+    }
+    __syncthreads();  // every basis value read: the area now takes the lane sums
+#pragma unroll
+    for (int v = 0; v < 17; ++v) lanesum[t][v] = acc[v];
+    __syncthreads();
     const int nh = E * (E + 1) / 2, nv = nh + E + 1;
     double* out = partials + (long long)blockIdx.x * nv;
     for (int v = t; v < nv; v += kBhcBlock) {
-        double acc = 0.0;
+        int sl, idx;
         if (v < nh) {
             int j = 0, rem = v;  // row j of the upper triangle holds E-j entries
             while (rem >= E - j) { rem -= E - j; ++j; }
             const int k = j + rem;
-            for (int m = 0; m < kBhcBlock; ++m)
-                acc = fma((double)phi[m][j] * (double)phi[m][k], q[m], acc);
+            const int bi = j >> 2, bj = k >> 2;
+            sl = bi * nb - bi * (bi - 1) / 2 + (bj - bi);
+            idx = 4 * (j & 3) + (k & 3);
         } else if (v < nh + E) {
             const int j = v - nh;
-            for (int m = 0; m < kBhcBlock; ++m) acc = fma((double)phi[m][j], r[m], acc);
+            sl = nhb + (j >> 2);
+            idx = j & 3;
         } else {
-            for (int m = 0; m < kBhcBlock; ++m) acc += ft[m];
+            sl = nslot - 1;
+            idx = 16;
         }
-        out[v] = acc;
+        double tot = 0.0;
+        for (int l = 0; l < L; ++l) tot += lanesum[l * S + sl][idx];
+        out[v] = tot;
     }
 }
 

@@ -120,54 +120,73 @@ def b1_iout_quad(k, s, j):
     return tuple(out)
 
 
+_M3_COEF = None
+
+
 def _moments(z):
     """M_n(z) = int_0^1 u^n exp(-z u) du, n = 0..3, vectorised over z (FP64).
-    |z| <= 2: 18-term series; else the upward recurrence."""
+    |z| <= 2: M_3 by its series sum_k (-z)^k / (k! (k+4)) (22 terms, truncation below
+    2e-16 of M_3), then the downward recurrence M_{n-1} = (z M_n + e^-z)/n, which damps
+    errors by |z|/n <= 1 per step; else the upward recurrence M_0 = (1-e^-z)/z,
+    M_n = (n M_{n-1} - e^-z)/z.  Each form is evaluated only where it applies."""
+    global _M3_COEF
+    if _M3_COEF is None:
+        from math import factorial
+        _M3_COEF = [(-1.0) ** k / (factorial(k) * (k + 4)) for k in range(22)]
     z = np.asarray(z, np.float64)
     small = np.abs(z) <= 2.0
-    M = np.zeros((4,) + z.shape)
-    zs = np.where(small, z, 0.0)
-    t = np.ones_like(z)
-    a = [np.ones_like(z), np.full_like(z, 0.5), np.full_like(z, 1 / 3), np.full_like(z, 0.25)]
-    for k in range(1, 19):
-        t = t * (-zs / k)
-        for n in range(4):
-            a[n] = a[n] + t / (k + n + 1)
-    zl = np.where(small, 1.0, z)
+    M = np.empty((4,) + z.shape)
+    zs = z[small]
+    es = np.exp(-zs)
+    m3 = np.full_like(zs, _M3_COEF[-1])
+    for c in reversed(_M3_COEF[:-1]):
+        m3 = m3 * zs + c
+    m2 = (zs * m3 + es) / 3
+    m1 = (zs * m2 + es) / 2
+    m0 = zs * m1 + es
+    zl = z[~small]
     e = np.exp(-zl)
-    m0 = (1 - e) / zl
-    m1 = (m0 - e) / zl
-    m2 = (2 * m1 - e) / zl
-    m3 = (3 * m2 - e) / zl
-    for n, big in enumerate((m0, m1, m2, m3)):
-        M[n] = np.where(small, a[n], big)
+    b0 = (1 - e) / zl
+    b1 = (b0 - e) / zl
+    b2 = (2 * b1 - e) / zl
+    b3 = (3 * b2 - e) / zl
+    for n, (lo, big) in enumerate(((m0, b0), (m1, b1), (m2, b2), (m3, b3))):
+        M[n][small] = lo
+        M[n][~small] = big
     return M
 
 
 def b1_iout_stable(k, s, I=None, nout=3):
     """The B1 spectrum transform by segment moments, smooth in s (no branch switch):
     substituting kappa = k_j + Delta_j u in phi_j(s) = int hat_j(kappa) e^{-kappa s} dkappa.
-    Same contract as b1_iout; pinned against quadrature (b1_iout_quad)."""
+    Same contract as b1_iout; pinned against quadrature (b1_iout_quad).  With I given,
+    BL, sBL, ssBL are accumulated basis by basis (the ray x basis matrix is not formed)."""
     k = np.asarray(k, np.float64).ravel()
     s = np.asarray(s, np.float64).ravel()
     E = len(k) - 2
-    P = np.zeros((3, len(s), E))
+    if I is None:
+        P = np.zeros((3, E, len(s)))
+        add = lambda o, j, v: P[o, j].__iadd__(v)  # noqa: E731
+    else:
+        I = np.asarray(I, np.float64)
+        out = np.zeros((3, len(s)))
+        add = lambda o, j, v: out[o].__iadd__(I[j] * v)  # noqa: E731
     for j in range(E + 1):
         kj, dj = k[j], k[j + 1] - k[j]
         w = dj * np.exp(-kj * s)
         m0, m1, m2, m3 = _moments(dj * s)
         if j >= 1:
-            P[0, :, j - 1] += w * (m0 - m1)
-            P[1, :, j - 1] += -w * (kj * (m0 - m1) + dj * (m1 - m2))
-            P[2, :, j - 1] += w * (kj * kj * (m0 - m1) + 2 * kj * dj * (m1 - m2) + dj * dj * (m2 - m3))
+            c0, c1, c2 = m0 - m1, m1 - m2, m2 - m3
+            add(0, j - 1, w * c0)
+            add(1, j - 1, -w * (kj * c0 + dj * c1))
+            add(2, j - 1, w * (kj * kj * c0 + 2 * kj * dj * c1 + dj * dj * c2))
         if j < E:
-            P[0, :, j] += w * m1
-            P[1, :, j] += -w * (kj * m1 + dj * m2)
-            P[2, :, j] += w * (kj * kj * m1 + 2 * kj * dj * m2 + dj * dj * m3)
+            add(0, j, w * m1)
+            add(1, j, -w * (kj * m1 + dj * m2))
+            add(2, j, w * (kj * kj * m1 + 2 * kj * dj * m2 + dj * dj * m3))
     if I is None:
-        return P[0], P[1], P[2]
-    I = np.asarray(I, np.float64)
-    return P[0] @ I, P[1] @ I, P[2] @ I
+        return P[0].T, P[1].T, P[2].T
+    return out[0], out[1], out[2]
 
 
 def b0_iout(k, s, I=None, nout=3):
