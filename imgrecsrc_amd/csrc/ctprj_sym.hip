@@ -66,18 +66,22 @@ __device__ __forceinline__ void pixel_bins(const SliceDesc& S, const Ray* rays_v
                                            float fy, float cT, float sT, int& blo, int& bhi)
 {
     if (FAN) {
+        // the pixel's shadow on the virtual detector lies within t_c +- ext: a corner
+        // offset (do, dq) of the projected coordinates (|do|, |dq| <= rho) moves
+        // t = d oc/qe by d |do qe - oc dq| / (qe (qe + dq)) <= d rho (qe + |oc|) / (qe (qe - rho))
         const float oc = fmaf(fx, cT, fy * sT);
         const float qe = fmaf(fx, sT, fmaf(-fy, cT, S.d));
-        const float hc = 0.5f * (cT + sT), hd = 0.5f * (cT - sT);
-        const float t0 = S.d * (oc - hc) * fast_rcp(qe + hd);
-        const float t1 = S.d * (oc + hd) * fast_rcp(qe + hc);
-        const float t2 = S.d * (oc + hc) * fast_rcp(qe - hd);
-        const float t3 = S.d * (oc - hd) * fast_rcp(qe - hc);
-        const float tl = fminf(fminf(t0, t1), fminf(t2, t3));
-        const float tr = fmaxf(fmaxf(t0, t1), fmaxf(t2, t3));
-        const float m = S.hbd + kSupportMargin * S.dSize;
-        blo = (int)ceilf((tl - m) * S.invDSize + (float)S.pC);
-        bhi = (int)floorf((tr + m) * S.invDSize + (float)S.pC);
+        const float rho = 0.5f * (fabsf(cT) + fabsf(sT));
+        const float iq = fast_rcp(qe), iqr = fast_rcp(qe - rho);
+        const float tc = S.d * oc * iq;
+        const float ext = S.d * rho * (qe + fabsf(oc)) * iq * iqr;
+        const float m = ext + S.hbd + kSupportMargin * S.dSize;
+        blo = (int)ceilf((tc - m) * S.invDSize + (float)S.pC);
+        bhi = (int)floorf((tc + m) * S.invDSize + (float)S.pC);
+        if (!(qe - rho > 0.0f)) {  // the source within a pixel: every bin is a candidate
+            blo = S.blo;
+            bhi = S.bhi;
+        }
     } else {
         const Ray r = rays_view[0];
         const float center = fmaf(fx, cT, fy * sT);
@@ -483,15 +487,15 @@ __device__ __forceinline__ Ray ldray(__amdgpu_buffer_rsrc_t r, int voff, int sof
 
 // The same gather for a canonical view off the mirror lines and a compile-time element
 // g' = E2: the reflection of every view row is fixed (elem_flip), so windows and bins are
-// indexed without selects.  Sums go straight into the view's partials; the windows of
-// views beyond np are zeroed (wave-uniform branch), which adds exact zeros.  Fan beam:
+// indexed without selects.  Sums go straight into the view's partials; views beyond np
+// read the zero row (a wave-uniform choice of buffer), which adds exact zeros.  Fan beam:
 // the rays of a window's first three bins are loaded together before the image windows
 // (a pixel meets at most three bins when the detector pitch is at least the pixel's;
 // the fourth is loaded where needed), RR being the slice's ray table and rrow the
 // view's row.
 template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD, int WIN>
 __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* __restrict__ rview,
-                                                __amdgpu_buffer_rsrc_t RS,
+                                                __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
                                                 const int* __restrict__ tab, float cT, float sT,
                                                 int x0, int y0, float* vpart)
@@ -504,14 +508,16 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
     int blo, bhi;
     pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
     if (blo > bhi) return;
+    // rows of views beyond np read the zero row (wave-uniform resource selects)
+    __amdgpu_buffer_rsrc_t rs[8];
     int soff[8];
-    unsigned none = 0;
 #pragma unroll
     for (int h = 0; h < 8; ++h) {
         if (!(HMASK >> h & 1u)) continue;
         const int t = tab[h];
-        soff[h] = (t & kRowNone) ? 0 : (t & ~(kRowFlip | kRowNone));
-        if (t & kRowNone) none |= 1u << h;
+        const bool none = (t & kRowNone) != 0;
+        rs[h] = none ? RZ : RS;
+        soff[h] = none ? 0 : (t & ~(kRowFlip | kRowNone));
     }
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
@@ -529,16 +535,10 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
         Win<WIN> q[8];
 #pragma unroll
         for (int h = 0; h < 8; ++h)
-            if (HMASK >> h & 1u) q[h] = ldbw<WIN>(RS, elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
-        if (none) {  // wave-uniform
-#pragma unroll
-            for (int h = 0; h < 8; ++h)
-                if ((HMASK >> h & 1u) && (none >> h & 1u)) q[h] = Win<WIN>{};
-        }
+            if (HMASK >> h & 1u) q[h] = ldbw<WIN>(rs[h], elem_flip(h, E2, FOLD) ? of : ob, soff[h]);
 #pragma unroll
         for (int j = 0; j < WIN; ++j) {
             const int b = s + j;
-            if (b < s0 || b > bhi) continue;
             Ray r;
             if (FAN) r = j < 3 ? rq[j] : ldray(RR, b * (int)sizeof(Ray), rrow);
             else {
@@ -548,6 +548,9 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
             const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
             float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
             if (FBP && FAN) w *= fw;
+            // bins of the window outside [s0, bhi] (taken by the previous window, or past
+            // the pixel's candidates) add an exact zero
+            w = (b >= s0 && b <= bhi) ? w : 0.0f;
 #pragma unroll
             for (int h = 0; h < 8; ++h)
                 if (HMASK >> h & 1u)
@@ -559,7 +562,7 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
 // All elements of a canonical view off the mirror lines, unrolled at compile time.
 template <bool FAN, bool FBP, int GRP, bool FOLD, int WIN>
 __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* __restrict__ rview,
-                                                __amdgpu_buffer_rsrc_t RS,
+                                                __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
                                                 const int* __restrict__ tv, float cT, float sT,
                                                 int x0, int y0, float* vpart)
@@ -568,7 +571,7 @@ __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* _
     constexpr unsigned GSET = FOLD ? G.fold : G.all;
 #define CTPRJ_E(E2)                                                                        \
     if constexpr ((GSET >> E2) & 1u)                                                        \
-        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RR, rrow, tv + 8 * E2, \
+        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow, tv + 8 * E2, \
                                                         cT, sT, x0, y0, vpart);
     CTPRJ_E(0) CTPRJ_E(1) CTPRJ_E(2) CTPRJ_E(3) CTPRJ_E(4) CTPRJ_E(5) CTPRJ_E(6) CTPRJ_E(7)
 #undef CTPRJ_E
@@ -617,11 +620,11 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
         } else if (GRP != kGrpZ2 && S.fold) {
             // a half-turn parallel scan takes the half set that maps the canonical range
             // into [0, 180) and reads the rest folded
-            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2, WIN>(S, rview, RS, RR, rrow, tv, T.x,
-                                                              T.y, x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2, WIN>(S, rview, RS, RZ, RR, rrow, tv,
+                                                              T.x, T.y, x0, y0, vpart);
         } else {
-            adj_sym_view_ct<FAN, FBP, GRP, false, WIN>(S, rview, RS, RR, rrow, tv, T.x, T.y, x0,
-                                                       y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, false, WIN>(S, rview, RS, RZ, RR, rrow, tv, T.x, T.y,
+                                                       x0, y0, vpart);
         }
 #pragma unroll
         for (int h = 0; h < 8; ++h)
