@@ -135,17 +135,25 @@ def pmc_summary(fetch_dir, write_dir, kernel_substr, tag, command):
     print(json.dumps(out))
 
 
-VALU_PEAK_CLOCK_GHZ = 2.4   # MI355X peak engine clock
-VALU_LANES_PER_CU = 64      # 4 SIMDs x 16 lanes: one non-packed FP32 lane-op each per clock
-VALU_CUS = 256
+CLOCK_GHZ = 2.4             # MI355X peak engine clock (GRBM_GUI_ACTIVE / kernel time agrees)
+CUS = 256
+VALU_LANES_PER_CU_CLK = 128  # 4 SIMDs; two VALU instructions can issue per quad-cycle
+L1_BYTES_PER_CU_CLK = 64     # vector memory (TA/TD) data path per CU
 
 
 def valu_peak_lane_ops():
-    """Non-packed VALU issue slots per second of the whole GPU (lane-ops/s)."""
-    return VALU_CUS * VALU_LANES_PER_CU * VALU_PEAK_CLOCK_GHZ * 1e9
+    """VALU lane-ops per second of the whole GPU: 256 CUs x 128 lanes x 2.4 GHz, the
+    FP32 vector FMA rate of the data sheet (157.3 TFLOP/s / 2)."""
+    return CUS * VALU_LANES_PER_CU_CLK * CLOCK_GHZ * 1e9
 
 
-def valu_summary(sq_dir, kernel_substr, tag, command):
+def l1_peak_bytes():
+    """Vector-memory (TA/TD) data rate of the whole GPU: 64 B/clk/CU (a 64-lane 16-byte
+    buffer load occupies the TA for 16 cycles)."""
+    return CUS * L1_BYTES_PER_CU_CLK * CLOCK_GHZ * 1e9
+
+
+def valu_summary(sq_dir, kernel_substr, tag, command, tune_json=None, suffix="valu"):
     """Reduce a rocprofv3 --pmc run of SQ counters (-f csv) to profiles/<tag>_valu.json:
     per kernel of one operator call ('+'-separated names, each at its trailing dispatches
     of the last grid size: the timed shape) the mean per dispatch of every counter
@@ -158,6 +166,8 @@ def valu_summary(sq_dir, kernel_substr, tag, command):
     rows = [r for f in files for r in _csv_rows(f)]
     parts = [k for k in kernel_substr.split("+") if k]
     out = {"command": command, "kernel": kernel_substr, "per_kernel": {}}
+    if tune_json:
+        out["tune"] = json.loads(tune_json)
     totals = {}
     for part in parts:
         kern = [r for r in rows
@@ -183,19 +193,71 @@ def valu_summary(sq_dir, kernel_substr, tag, command):
     out["call"] = totals
     if "SQ_INSTS_VALU" in totals:
         out["lane_ops_per_call"] = totals["SQ_INSTS_VALU"] * 64.0
-    path = os.path.join(ROOT, "profiles", f"{tag}_valu.json")
+    path = os.path.join(ROOT, "profiles", f"{tag}_{suffix}.json")
     with open(path, "w") as f:
         json.dump(out, f, indent=1)
     print(json.dumps(out))
 
 
-def valu_from_profiles(tag):
-    """Lane-ops per call from profiles/<tag>_valu.json, or None."""
-    path = os.path.join(ROOT, "profiles", f"{tag}_valu.json")
+VALU_MIN_OPS_PER_UPDATE = 3.75  # one ~22-op weight shared by 8 images, plus one FMA each
+
+
+def _summary(tag, suffix):
+    path = os.path.join(ROOT, "profiles", f"{tag}_{suffix}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         return json.load(f)
+
+
+def roofline_entry(tag, kernel, kernel_ms, launches, nnz, tune, other_ms):
+    """The dominant call against the ceiling that binds it.  The symmetric kernels reuse
+    every loaded value for 2-8 updates and move little HBM traffic; their counters
+    (committed under profiles/, taken at the same tuned shapes) show the vector-memory
+    pipeline (TA/TD: 64 B/clk/CU) as the busiest unit.  frac = TA busy cycles per call
+    (TA_TA_BUSY summed over the 256 TAs, profiles/<tag>_l1.json) / (256 x the call's live
+    duration x 2.4 GHz); achieved = frac x 39.3 TB/s, the data rate that busy time
+    carries.  Also reported: VALU issue (SQ_INSTS_VALU x 64 lane-ops per call over the
+    live duration, against 78.6 T lane-op/s) and HBM (measured FETCH_SIZE + WRITE_SIZE
+    over the same duration).  Without summaries at these shapes: HBM from the measured
+    traffic, or nothing."""
+    traffic = traffic_from_profiles(tag)
+    sec = kernel_ms * 1e-3
+    hbm = None
+    if traffic is not None:
+        gbs = traffic / sec / 1e9
+        hbm = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_per_call": round(traffic)}
+    base = {"kernel": kernel, "kernel_ms": round(kernel_ms, 4), "launches": launches,
+            "other_kernel_ms": round(other_ms, 4), "traffic": traffic,
+            "updates_per_s": round(nnz / sec, 1)}
+    valu, l1 = _summary(tag, "valu"), _summary(tag, "l1")
+    if (valu is not None and l1 is not None and valu.get("tune") == tune
+            and l1.get("tune") == tune):
+        lane_ops = float(valu["lane_ops_per_call"])
+        vfrac = lane_ops / sec / valu_peak_lane_ops()
+        busy = float(l1["call"]["TA_TA_BUSY"])
+        frac = busy / (CUS * sec * CLOCK_GHZ * 1e9)
+        out = {"bound": "l1", "achieved": round(frac * l1_peak_bytes() / 1e12, 2),
+               "peak": round(l1_peak_bytes() / 1e12, 2), "unit": "TB/s", "frac": round(frac, 4)}
+        out.update(base)
+        out.update({"ta_busy_cycles_per_call": busy,
+                    "valu": {"achieved": round(lane_ops / sec / 1e12, 2),
+                             "peak": round(valu_peak_lane_ops() / 1e12, 2),
+                             "unit": "Tlane-op/s", "frac": round(vfrac, 4),
+                             "lane_ops_per_update": round(lane_ops / nnz, 3),
+                             "min_lane_ops_per_update": VALU_MIN_OPS_PER_UPDATE},
+                    "hbm": hbm, "clock_GHz": CLOCK_GHZ,
+                    "profiles": [f"profiles/{tag}_l1.json", f"profiles/{tag}_valu.json"]})
+        return out
+    if hbm is not None:
+        out = {"bound": "hbm", "achieved": hbm["achieved"], "peak": HBM_PEAK_GBS,
+               "unit": "GB/s", "frac": hbm["frac"]}
+    else:
+        out = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+               "frac": None}
+    out.update(base)
+    return out
 
 
 def cpu_model():
@@ -465,7 +527,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="fan1024", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-extras", action="store_true", help="skip the bhc and batched legs")
+    ap.add_argument("--no-extras", action="store_true", help="only the headline slice")
+    ap.add_argument("--legs", default="bhc,npg,batched,pcie,stack,par256",
+                    help="comma-separated extra legs to run (bhc, npg, batched, pcie, stack, "
+                         "par256); npg and batched also run the bhc set-up")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (PMC scaling reference)")
     ap.add_argument("--pnpg-slices", type=int, default=512,
@@ -478,15 +543,21 @@ def main():
     ap.add_argument("--valu-summary", nargs=3, metavar=("SQ_DIR", "KERNEL", "TAG"),
                     help="reduce a rocprofv3 --pmc csv run of SQ counters to "
                          "profiles/TAG_valu.json")
+    ap.add_argument("--summary-suffix", default="valu",
+                    help="file suffix of a --valu-summary (profiles/TAG_SUFFIX.json)")
+    ap.add_argument("--valu-tune", default=None,
+                    help="JSON of the tuned shapes the counter run used (recorded in the "
+                         "summary; the bench uses a summary only at the same shapes)")
     ap.add_argument("--tune-cache", default=None,
                     help="autotuner cache file: a later run with the same file reuses the "
                          "shapes the first one chose (profiling runs)")
     args = ap.parse_args()
+    legs = set(x.strip() for x in args.legs.split(",") if x.strip())
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.pmc_command)
         return
     if args.valu_summary:
-        valu_summary(*args.valu_summary, args.pmc_command)
+        valu_summary(*args.valu_summary, args.pmc_command, args.valu_tune, args.summary_suffix)
         return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -551,20 +622,13 @@ def main():
     sfx = "_sym" if P.modes()[0] != 0 else ""
     # the forward call also runs the 32x32 transpose of the image (a few microseconds)
     dom_name, dom_ms = (f"k_adjoint{sfx}", t_adj) if t_adj >= t_fwd else (f"k_forward{sfx}", t_fwd)
-    alg_bytes = 4.0 * nnz
-    achieved = alg_bytes / (dom_ms * 1e-3) / 1e9
+    launches = (["k_adjoint_sym"] + (["k_adjoint_sym_combine"] if tune["adj_split"] > 1 else [])
+                if dom_name.startswith("k_adjoint") else
+                ["k_transpose", "k_forward_sym"] + (["k_forward_sym_combine"]
+                                                    if tune["fwd_split"] > 1 else []))
     tag = f"{'adjoint' if dom_name.startswith('k_adjoint') else 'forward'}_{args.workload}"
-    traffic = traffic_from_profiles(tag)
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": traffic, "kernel": dom_name, "kernel_ms": round(dom_ms, 4),
-                "algorithmic_bytes": alg_bytes,
-                "other_kernel_ms": round(t_fwd if dom_name.startswith("k_adjoint") else t_adj, 4),
-                "launches": (["k_adjoint_sym"] + (["k_adjoint_sym_combine"] if tune["adj_split"] > 1
-                                                  else [])
-                             if dom_name.startswith("k_adjoint") else
-                             ["k_transpose", "k_forward_sym"] + (["k_forward_sym_combine"]
-                                                                 if tune["fwd_split"] > 1 else []))}
+    roofline = roofline_entry(tag, dom_name, dom_ms, launches, nnz, tune,
+                              t_fwd if dom_name.startswith("k_adjoint") else t_adj)
 
     # <Ax, y> = <x, A^T y> on seeded Gaussian x, y
     x = torch.from_numpy(gaussian((n, n), 21)).to(dev).reshape(-1)
@@ -587,7 +651,7 @@ def main():
         "roofline": roofline,
     }
 
-    if not args.no_extras:
+    if not args.no_extras and legs & {"bhc", "npg", "batched"}:
         # ---- beam-hardening gradient evaluations (512^2 fan, E = 20)
         log("bhc leg")
         bg = FAN512
@@ -611,13 +675,20 @@ def main():
         bnnz = PB.nnz()
         t_ga = timed(lambda: poissLAlpha(Imea, Ie, a_eval, PB, Ts, knots), 20, stream)
         t_gi = timed(lambda: poissLI(Imea, Ie, PB, Ts, knots, alpha=a_eval), 20, stream)
-        roof_evals = HBM_PEAK_GBS * 1e9 / (8.0 * bnnz + 12.0 * PB.sino_elems)
+        # the same plan's projector pair: the gradient's floor (its ray passes and
+        # reductions come on top of one forward and one adjoint)
+        bs = torch.empty(PB.sino_elems, dtype=torch.float32, device=dev)
+        bb = torch.empty(PB.img_elems, dtype=torch.float32, device=dev)
+        t_pair = timed(lambda: (PB.forward(a_eval, out=bs), PB.adjoint(bs, out=bb)), 20, stream)
+        t_fw = timed(lambda: PB.forward(a_eval, out=bs), 20, stream)
         out["bhc"] = {"workload": "fan 512^2, 360 views x 729 bins, d=2000, E=20",
                       "grad_alpha_evals_per_s": round(1e3 / t_ga, 1),
                       "grad_alpha_ms": round(t_ga, 4),
                       "grad_ie_evals_per_s": round(1e3 / t_gi, 1), "grad_ie_ms": round(t_gi, 4),
-                      "roofline_evals_per_s": round(roof_evals, 1),
-                      "frac": round((1e3 / t_ga) / roof_evals, 4), "nnz": bnnz,
+                      "fwd_adj_ms": round(t_pair, 4), "fwd_ms": round(t_fw, 4),
+                      "grad_alpha_over_fwd_adj": round(t_ga / t_pair, 3),
+                      "grad_ie_over_fwd": round(t_gi / t_fw, 3),
+                      "grad_GUPS": round(2 * bnnz / t_ga / 1e6, 1), "nnz": bnnz,
                       "tune": PB.get_tune()}
 
         # ---- I-epsilon step (20 iterations) and the measurement simulator; both block
@@ -644,119 +715,121 @@ def main():
             "gen_beam_harden_trials": int(round(np.log(sim["Ts"]) / np.log(0.8))) + 1,
             "gen_beam_harden_energies": int(eps_s.size)})
 
-        # ---- the reference's default alpha step: NPG with the wavelet-ADMM prior (Haar,
-        #      6 levels) on a circle support, same 512^2 fan problem; trials synchronise
-        log("npg + wavelet leg")
-        from imgrecsrc_amd.pnpg import PNPG
-        from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
-        nb = bg[0]
-        cc = np.arange(nb) - (nb - 1) / 2
-        XX, YY = np.meshgrid(cc, cc)
-        disc = XX ** 2 + YY ** 2 <= (nb / 2 - 1) ** 2
-        hw = daubcqf(2)[0]
-        wv = Wavelet(nb, nb, hw, 6)
-        wv.set_masks(disc, wvlt_mask(disc, hw, 6))
-        a_w = (alpha * torch.from_numpy(disc.ravel().astype(np.float32)).to(dev)).contiguous()
-        npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
-        npg.set_method("NPG")
-        npg.set_mask(disc)
-        npg.set_wavelet(wv, 1e-3 * npg.t0)
-        ntr, inner = 10, 0
-        npg.step(a_w)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(ntr):
+        if "npg" in legs:
+            # ---- the reference's default alpha step: NPG with the wavelet-ADMM prior (Haar,
+            #      6 levels) on a circle support, same 512^2 fan problem; trials synchronise
+            log("npg + wavelet leg")
+            from imgrecsrc_amd.pnpg import PNPG
+            from imgrecsrc_amd.wavelet import Wavelet, daubcqf, wvlt_mask
+            nb = bg[0]
+            cc = np.arange(nb) - (nb - 1) / 2
+            XX, YY = np.meshgrid(cc, cc)
+            disc = XX ** 2 + YY ** 2 <= (nb / 2 - 1) ** 2
+            hw = daubcqf(2)[0]
+            wv = Wavelet(nb, nb, hw, 6)
+            wv.set_masks(disc, wvlt_mask(disc, hw, 6))
+            a_w = (alpha * torch.from_numpy(disc.ravel().astype(np.float32)).to(dev)).contiguous()
+            npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
+            npg.set_method("NPG")
+            npg.set_mask(disc)
+            npg.set_wavelet(wv, 1e-3 * npg.t0)
+            ntr, inner = 10, 0
             npg.step(a_w)
-            inner += npg.stats()["innerSearch"]
-        torch.cuda.synchronize()
-        t_npg = (time.perf_counter() - t0) * 1e3 / ntr
-        st = npg.stats()
-        out["npg_wavelet"] = {
-            "workload": "NPG + wavelet-ADMM prior (daubcqf(2), 6 levels, circle support), "
-                        "fan 512^2, E=20",
-            "ms_per_trial": round(t_npg, 3), "admm_iterations_per_trial": inner / ntr,
-            "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
-        del npg, wv
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(ntr):
+                npg.step(a_w)
+                inner += npg.stats()["innerSearch"]
+            torch.cuda.synchronize()
+            t_npg = (time.perf_counter() - t0) * 1e3 / ntr
+            st = npg.stats()
+            out["npg_wavelet"] = {
+                "workload": "NPG + wavelet-ADMM prior (daubcqf(2), 6 levels, circle support), "
+                            "fan 512^2, E=20",
+                "ms_per_trial": round(t_npg, 3), "admm_iterations_per_trial": inner / ntr,
+                "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
+            del npg, wv
 
-        # ---- the same step with the isotropic TV prior ('tviso', utils/TV.m denoise)
-        log("npg + tv leg")
-        from imgrecsrc_amd.tv import TV
-        tvp = TV(nb, nb, tv="iso")
-        tvp.set_mask(disc)
-        npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
-        npg.set_method("NPG")
-        npg.set_mask(disc)
-        npg.set_tv(tvp, 1e-3 * npg.t0)
-        inner = 0
-        npg.step(a_w)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(ntr):
+            # ---- the same step with the isotropic TV prior ('tviso', utils/TV.m denoise)
+            log("npg + tv leg")
+            from imgrecsrc_amd.tv import TV
+            tvp = TV(nb, nb, tv="iso")
+            tvp.set_mask(disc)
+            npg = PNPG(PB, Imea, knots, Ie, Ts, a_w)
+            npg.set_method("NPG")
+            npg.set_mask(disc)
+            npg.set_tv(tvp, 1e-3 * npg.t0)
+            inner = 0
             npg.step(a_w)
-            inner += npg.stats()["innerSearch"]
-        torch.cuda.synchronize()
-        t_tv = (time.perf_counter() - t0) * 1e3 / ntr
-        st = npg.stats()
-        out["npg_tv"] = {
-            "workload": "NPG + isotropic TV prior (FGP), circle support, fan 512^2, E=20",
-            "ms_per_trial": round(t_tv, 3), "fgp_iterations_per_trial": inner / ntr,
-            "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
-        del npg, tvp
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(ntr):
+                npg.step(a_w)
+                inner += npg.stats()["innerSearch"]
+            torch.cuda.synchronize()
+            t_tv = (time.perf_counter() - t0) * 1e3 / ntr
+            st = npg.stats()
+            out["npg_tv"] = {
+                "workload": "NPG + isotropic TV prior (FGP), circle support, fan 512^2, E=20",
+                "ms_per_trial": round(t_tv, 3), "fgp_iterations_per_trial": inner / ntr,
+                "iterations": st["iterations"], "trials": st["trials"], "cost": st["cost"]}
+            del npg, tvp
 
-        # ---- 64 ragged slices: one launch per operator vs one launch per slice
-        log("batched leg")
-        geoms = ragged_geoms()
-        PBat = Projector(geoms)
-        PBat.autotune(reps=3)
-        imgs = [torch.from_numpy(ellipse_phantom(gq[0], 100 + i)).to(dev).reshape(-1)
-                for i, gq in enumerate(geoms)]
-        bimg = PBat.pack_images(imgs)
-        bsino = torch.empty(PBat.sino_elems, dtype=torch.float32, device=dev)
-        bback = torch.empty(PBat.img_elems, dtype=torch.float32, device=dev)
-        tot_nnz = PBat.nnz()
-        singles = [Projector([gq]) for gq in geoms]
-        for s1 in singles:
-            s1.autotune(reps=3)
-        # the plain path on the general (per-view) kernels, one slice per launch
-        generals = [Projector([gq], symmetric=False) for gq in geoms]
-        for s1 in generals:
-            s1.autotune(reps=3)
-        s_out = [torch.empty(s.sino_elems, dtype=torch.float32, device=dev) for s in singles]
-        i_out = [torch.empty(s.img_elems, dtype=torch.float32, device=dev) for s in singles]
+        if "batched" in legs:
+            # ---- 64 ragged slices: one launch per operator vs one launch per slice
+            log("batched leg")
+            geoms = ragged_geoms()
+            PBat = Projector(geoms)
+            PBat.autotune(reps=3)
+            imgs = [torch.from_numpy(ellipse_phantom(gq[0], 100 + i)).to(dev).reshape(-1)
+                    for i, gq in enumerate(geoms)]
+            bimg = PBat.pack_images(imgs)
+            bsino = torch.empty(PBat.sino_elems, dtype=torch.float32, device=dev)
+            bback = torch.empty(PBat.img_elems, dtype=torch.float32, device=dev)
+            tot_nnz = PBat.nnz()
+            singles = [Projector([gq]) for gq in geoms]
+            for s1 in singles:
+                s1.autotune(reps=3)
+            # the plain path on the general (per-view) kernels, one slice per launch
+            generals = [Projector([gq], symmetric=False) for gq in geoms]
+            for s1 in generals:
+                s1.autotune(reps=3)
+            s_out = [torch.empty(s.sino_elems, dtype=torch.float32, device=dev) for s in singles]
+            i_out = [torch.empty(s.img_elems, dtype=torch.float32, device=dev) for s in singles]
 
-        def batched():
-            PBat.forward(bimg, out=bsino)
-            PBat.adjoint(bsino, out=bback)
+            def batched():
+                PBat.forward(bimg, out=bsino)
+                PBat.adjoint(bsino, out=bback)
 
-        def plain():
-            for k, s in enumerate(singles):
-                s.forward(imgs[k], out=s_out[k])
-                s.adjoint(s_out[k], out=i_out[k])
+            def plain():
+                for k, s in enumerate(singles):
+                    s.forward(imgs[k], out=s_out[k])
+                    s.adjoint(s_out[k], out=i_out[k])
 
-        def plain_general():
-            for k, s in enumerate(generals):
-                s.forward(imgs[k], out=s_out[k])
-                s.adjoint(s_out[k], out=i_out[k])
+            def plain_general():
+                for k, s in enumerate(generals):
+                    s.forward(imgs[k], out=s_out[k])
+                    s.adjoint(s_out[k], out=i_out[k])
 
-        batched()
-        plain()
-        plain_general()
-        t_b = timed(batched, 3, stream)
-        t_p = timed(plain, 3, stream)
-        t_g = timed(plain_general, 3, stream)
-        groups = PBat.modes()
-        out["batched"] = {"workload": "64 ragged slices, n in {128..1024}, views 90..720, "
-                                      "mixed parallel/fan", "nnz": tot_nnz,
-                          "batched_ms": round(t_b, 3), "plain_ms": round(t_p, 3),
-                          "batched_GUPS": round(2 * tot_nnz / t_b / 1e6, 1),
-                          "plain_GUPS": round(2 * tot_nnz / t_p / 1e6, 1),
-                          "speedup": round(t_p / t_b, 3),
-                          "plain_general_ms": round(t_g, 3),
-                          "plain_general_GUPS": round(2 * tot_nnz / t_g / 1e6, 1),
-                          "speedup_vs_general": round(t_g / t_b, 3),
-                          "slices_per_group": {str(k): groups.count(k) for k in range(4)}}
+            batched()
+            plain()
+            plain_general()
+            t_b = timed(batched, 3, stream)
+            t_p = timed(plain, 3, stream)
+            t_g = timed(plain_general, 3, stream)
+            groups = PBat.modes()
+            out["batched"] = {"workload": "64 ragged slices, n in {128..1024}, views 90..720, "
+                                          "mixed parallel/fan", "nnz": tot_nnz,
+                              "batched_ms": round(t_b, 3), "plain_ms": round(t_p, 3),
+                              "batched_GUPS": round(2 * tot_nnz / t_b / 1e6, 1),
+                              "plain_GUPS": round(2 * tot_nnz / t_p / 1e6, 1),
+                              "speedup": round(t_p / t_b, 3),
+                              "plain_general_ms": round(t_g, 3),
+                              "plain_general_GUPS": round(2 * tot_nnz / t_g / 1e6, 1),
+                              "speedup_vs_general": round(t_g / t_b, 3),
+                              "slices_per_group": {str(k): groups.count(k) for k in range(4)}}
 
-    if not args.no_extras:
+    if not args.no_extras and "pcie" in legs:
         # ---- the reference-compatible boundary: host arrays through prj_run, PCIe included
         import ctypes
         from imgrecsrc_amd._lib import BWD_BIT, FWD_BIT, check, lib
@@ -782,10 +855,10 @@ def main():
                                          "host arrays (the MEX boundary), default shapes"}
 
     check_data = None
-    if not args.no_extras and args.pnpg_slices > 0:
+    if not args.no_extras and args.pnpg_slices > 0 and "stack" in legs:
         out["pnpg"], check_data = pnpg_leg(args.pnpg_slices, args.pnpg_iters, dev, stream)
 
-    if not args.no_extras:
+    if not args.no_extras and "par256" in legs:
         # ---- the 256^2 parallel slice (180 views over 360, 367 bins): plain GPU times
         PC1 = Projector([PAR256])
         PC1.autotune(reps=3)
