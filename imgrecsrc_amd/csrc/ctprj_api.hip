@@ -729,7 +729,8 @@ int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool
     if (n_syma_items) {
         for (int g = 1; g < 4; ++g)
             if (syma_count[g])
-                launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4, tune.adj_split,
+                launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4,
+                                   (tune.flags & 32) != 0, tune.adj_split,
                                    d_split, syma_count[g], d_desc, d_syma_items + syma_start[g],
                                    d_views, d_rays, d_sino, d_zero_row, d_symtab, d_img, st);
     }
@@ -826,7 +827,7 @@ extern "C" int ctprj_plan_set_tune(ctprj_plan* p, const ctprj_tune* t)
         t->fwd_seg != 16)
         return fail(CTPRJ_EINVAL, "fwd_seg must be 1, 2, 4, 8 or 16");
     if (t->adj_tile != 8 && t->adj_tile != 16) return fail(CTPRJ_EINVAL, "adj_tile must be 8 or 16");
-    if (t->flags & ~31) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
+    if (t->flags & ~63) return fail(CTPRJ_EINVAL, "unknown tune flags 0x%x", t->flags);
     if (t->adj_seg != 1 && t->adj_seg != 2 && t->adj_seg != 4 && t->adj_seg != 8)
         return fail(CTPRJ_EINVAL, "adj_seg must be 1, 2, 4 or 8");
     if (t->adj_split < 1 || t->adj_split > 8 || t->fwd_split < 1 || t->fwd_split > 8)
@@ -1033,6 +1034,11 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
         rc = search_split(&ctprj_tune::fwd_seg, &ctprj_tune::fwd_split, fsegs, 5, true,
                           p->n_symf_items);
     if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);
+    if (!rc && p->n_syma_items) {
+        // symmetric adjoint gathers: per lane from L1, or staged per wave in LDS (bit 5)
+        const int stg[2] = {best.flags & ~32, best.flags | 32};
+        rc = search(&ctprj_tune::flags, stg, 2, false);
+    }
     if (!rc && p->n_syma_items)
         rc = search_split(&ctprj_tune::adj_seg, &ctprj_tune::adj_split, asegs, 4, false,
                           p->n_syma_items);

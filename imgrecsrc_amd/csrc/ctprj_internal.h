@@ -121,10 +121,10 @@ void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
                         const SliceDesc* desc, const FwdItem* items, const float4* views,
                         const Ray* rays, const float* img, const float* imgT, float* sino,
                         hipStream_t st);
-void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* part, int nitems,
-                        const SliceDesc* desc, const AdjItem* items, const float4* views,
-                        const Ray* rays, const float* sino, const float* zero_row,
-                        const int* symtab, float* img, hipStream_t st);
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, bool stage, int nsplit, float* part,
+                        int nitems, const SliceDesc* desc, const AdjItem* items,
+                        const float4* views, const Ray* rays, const float* sino,
+                        const float* zero_row, const int* symtab, float* img, hipStream_t st);
 
 // Images held by a wavelet context (rows * cols * batch).
 long long wavelet_elems(const ctprj_wavelet* w);

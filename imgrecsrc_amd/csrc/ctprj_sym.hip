@@ -559,32 +559,191 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
     }
 }
 
+// ------------------------------------------------- LDS-staged adjoint gather ---
+//
+// The direct gather above has every lane load its own 3- or 4-bin windows of the eight
+// sinogram rows and (fan beam) three 32-byte ray records per window, so neighbouring
+// lanes fetch the same bins again through the vector-memory pipeline, the kernel's
+// busiest unit.  The staged gather lets the wave fetch each (canonical view, element g')
+// once: the bins [L0, L0 + kStageBins) that any of its 64 orbit pixels can read (the
+// wave minimum of the pixels' first candidate bins, up to the maximum of their last
+// candidate plus the window width), one load per sinogram row and one for the ray
+// records, into a wave-private LDS area; the pixels then read their windows from LDS.
+// Every pixel still forms the same weights from the same ray records and sums its bins
+// in the same order, so the image is bit-identical.  A wave whose range exceeds
+// kStageBins bins (fine detector pitch) takes the direct gather for that element.
+constexpr int kStageBins = 32;
+
+struct AdjStage {
+    float4 ray[2 * kStageBins];  // fan beam: bin k's record at ray[2k], ray[2k+1]
+    float val[kStageBins][8];    // bin k's value in orbit image h's row
+};
+
+// Wave minimum / maximum of an int: prefix reductions inside each row of 16 lanes
+// (DPP row shifts; lanes with no source keep their own value), then the four rows'
+// last lanes.
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
+    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
+    const int a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const int c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return min(min(a, b), min(c, d));
+}
+
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
+    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
+    const int a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
+    const int c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
+    return max(max(a, b), max(c, d));
+}
+
+// Orders this wave's LDS writes before its later reads (and reads before later writes).
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The gather of adj_sym_elem_ct through the wave's LDS area.  Every lane of the wave
+// takes part (the staging loads are the wave's); lanes with valid == false or an empty
+// candidate range add nothing.
+template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD, int WIN>
+__device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* __restrict__ rview,
+                                                 __amdgpu_buffer_rsrc_t RS,
+                                                 __amdgpu_buffer_rsrc_t RZ,
+                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
+                                                 const int* __restrict__ tab, float cT, float sT,
+                                                 int x0, int y0, bool valid, float* vpart,
+                                                 AdjStage* __restrict__ st)
+{
+    constexpr int K2 = E2 & 3, M2 = E2 >> 2;
+    int px, py;
+    if (M2) apply_px(K2, 1, x0, y0, px, py);
+    else apply_px(4 - K2, 0, x0, y0, px, py);
+    const float fx = (float)px, fy = (float)py;
+    int blo, bhi;
+    pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
+    if (!valid || blo > bhi) {
+        blo = INT_MAX;
+        bhi = INT_MIN;
+    }
+    const int L0 = wave_min_i32(blo);
+    const int Lmax = wave_max_i32(bhi);
+    if (L0 > Lmax) return;  // no lane has a candidate (wave-uniform)
+    if (Lmax + WIN - 1 - L0 >= kStageBins) {  // wave-uniform: too wide to stage
+        if (valid)
+            adj_sym_elem_ct<FAN, FBP, HMASK, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow, tab, cT,
+                                                            sT, x0, y0, vpart);
+        return;
+    }
+    const int lane = (int)threadIdx.x & 63;
+    wave_lds_sync();  // the previous element's reads are done
+    // bins L0 + lane (lane < kStageBins) of each orbit image's row; reflected rows read
+    // bin 2pC - b, whose offset may fall below the row (a negative offset is out of the
+    // buffer and reads 0) -- such bins lie outside the detector and carry no weight
+    if (lane < kStageBins) {
+        const int b = L0 + lane, two_pc = 2 * S.pC;
+#pragma unroll
+        for (int h = 0; h < 8; ++h) {
+            if (!(HMASK >> h & 1u)) continue;
+            const int t = tab[h];
+            const bool none = (t & kRowNone) != 0;
+            const int soff = none ? 0 : (t & ~(kRowFlip | kRowNone));
+            const int bb = elem_flip(h, E2, FOLD) ? two_pc - b : b;
+            st->val[lane][h] = ldb(none ? RZ : RS, bb * 4, soff);
+        }
+    }
+    if (FAN) {  // ray records of bins L0 .. L0 + kStageBins - 1: 16 bytes per lane
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(RR, L0 * (int)sizeof(Ray) + lane * 16,
+                                                             rrow, 0);
+        st->ray[lane] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
+                                    __uint_as_float(v[2]), __uint_as_float(v[3]));
+    }
+    wave_lds_sync();  // staged values visible to every lane
+    if (!valid || blo > bhi) return;
+    const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
+    // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
+    const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
+    for (int s0 = blo; s0 <= bhi; s0 += WIN) {
+        const int k0 = s0 - L0;
+#pragma unroll
+        for (int j = 0; j < WIN; ++j) {
+            const int b = s0 + j, k = k0 + j;
+            Ray r;
+            if (FAN) {
+                const float4 a = st->ray[2 * k], c = st->ray[2 * k + 1];
+                r.cosR = a.x;
+                r.sinR = a.y;
+                r.dc = a.z;
+                r.hbc = a.w;
+                r.h = c.x;
+                r.b1 = c.y;
+                r.s = c.z;
+                r.k = c.w;
+            } else {
+                r = rview[0];
+                r.dc = (float)(b - S.pC) * S.dSize;
+            }
+            const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
+            float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+            if (FBP && FAN) w *= fw;
+            w = b <= bhi ? w : 0.0f;  // bins past the pixel's candidates add an exact zero
+            const float4 lo = *reinterpret_cast<const float4*>(&st->val[k][0]);
+            const float4 hi = *reinterpret_cast<const float4*>(&st->val[k][4]);
+            const float q[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+            for (int h = 0; h < 8; ++h)
+                if (HMASK >> h & 1u) vpart[h] = fmaf(w, q[h], vpart[h]);
+        }
+    }
+}
+
 // All elements of a canonical view off the mirror lines, unrolled at compile time.
-template <bool FAN, bool FBP, int GRP, bool FOLD, int WIN>
+// With STAGE the wave stages each element's bins in LDS (every lane takes part),
+// otherwise the valid lanes gather directly.
+template <bool FAN, bool FBP, int GRP, bool FOLD, int WIN, bool STAGE>
 __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* __restrict__ rview,
                                                 __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RZ,
                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
                                                 const int* __restrict__ tv, float cT, float sT,
-                                                int x0, int y0, float* vpart)
+                                                int x0, int y0, bool valid, float* vpart,
+                                                AdjStage* __restrict__ st)
 {
     constexpr GroupMasks G = group_masks(GRP);
     constexpr unsigned GSET = FOLD ? G.fold : G.all;
-#define CTPRJ_E(E2)                                                                        \
-    if constexpr ((GSET >> E2) & 1u)                                                        \
-        adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow, tv + 8 * E2, \
-                                                        cT, sT, x0, y0, vpart);
+#define CTPRJ_E(E2)                                                                             \
+    if constexpr ((GSET >> E2) & 1u) {                                                           \
+        if (STAGE)                                                                               \
+            adj_sym_elem_lds<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,         \
+                                                             tv + 8 * E2, cT, sT, x0, y0, valid, \
+                                                             vpart, st);                         \
+        else if (valid)                                                                          \
+            adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,          \
+                                                            tv + 8 * E2, cT, sT, x0, y0, vpart); \
+    }
     CTPRJ_E(0) CTPRJ_E(1) CTPRJ_E(2) CTPRJ_E(3) CTPRJ_E(4) CTPRJ_E(5) CTPRJ_E(6) CTPRJ_E(7)
 #undef CTPRJ_E
 }
 
 // Canonical views i = i0, i0 + step, ... for the orbit of (x0, y0) under group GRP.
-template <bool FAN, bool FBP, int GRP, int WIN>
+// Every lane of the wave runs the view loop (the staged gather needs them all); lanes
+// with valid == false add nothing.
+template <bool FAN, bool FBP, int GRP, int WIN, bool STAGE>
 __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* __restrict__ views,
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ sino,
                                               const float* __restrict__ zero_row,
                                               const int* __restrict__ symtab, int x0, int y0,
-                                              int i0, int step, KahanAcc* acc)
+                                              bool valid, int i0, int step, KahanAcc* acc,
+                                              AdjStage* __restrict__ st)
 {
     constexpr GroupMasks G = group_masks(GRP);
     constexpr unsigned REF = G.all & ~G.rot;
@@ -607,24 +766,26 @@ __device__ __forceinline__ void adj_sym_pixel(const SliceDesc& S, const float4* 
             // rotations g' -> rotation orbit pixels, reflections g' -> reflected ones
             const unsigned grot = S.fold ? G.mfold_rot : G.rot;
             const unsigned gref = S.fold ? G.mfold_ref : REF;
+            if (valid) {
 #pragma unroll 1
-            for (int e = 0; e < 8; ++e)
-                if (grot >> e & 1u)
-                    adj_sym_elem<FAN, FBP, G.rot, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
-                                                       e & 3, e >> 2, x0, y0, vpart);
+                for (int e = 0; e < 8; ++e)
+                    if (grot >> e & 1u)
+                        adj_sym_elem<FAN, FBP, G.rot, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
+                                                           e & 3, e >> 2, x0, y0, vpart);
 #pragma unroll 1
-            for (int e = 0; e < 8; ++e)
-                if (gref >> e & 1u)
-                    adj_sym_elem<FAN, FBP, REF, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
-                                                     e & 3, e >> 2, x0, y0, vpart);
+                for (int e = 0; e < 8; ++e)
+                    if (gref >> e & 1u)
+                        adj_sym_elem<FAN, FBP, REF, WIN>(S, rview, RS, RZ, tv + 8 * e, T.x, T.y,
+                                                         e & 3, e >> 2, x0, y0, vpart);
+            }
         } else if (GRP != kGrpZ2 && S.fold) {
             // a half-turn parallel scan takes the half set that maps the canonical range
             // into [0, 180) and reads the rest folded
-            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2, WIN>(S, rview, RS, RZ, RR, rrow, tv,
-                                                              T.x, T.y, x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, GRP != kGrpZ2, WIN, STAGE>(
+                S, rview, RS, RZ, RR, rrow, tv, T.x, T.y, x0, y0, valid, vpart, st);
         } else {
-            adj_sym_view_ct<FAN, FBP, GRP, false, WIN>(S, rview, RS, RZ, RR, rrow, tv, T.x, T.y,
-                                                       x0, y0, vpart);
+            adj_sym_view_ct<FAN, FBP, GRP, false, WIN, STAGE>(S, rview, RS, RZ, RR, rrow, tv, T.x,
+                                                              T.y, x0, y0, valid, vpart, st);
         }
 #pragma unroll
         for (int h = 0; h < 8; ++h)
@@ -679,7 +840,7 @@ __device__ __forceinline__ void adj_sym_store(const SliceDesc& S, int x0, int y0
 // The canonical views are cut into nsplit * SEG strided sets: wave w of this work-group
 // takes i = split * SEG + w + j * nsplit * SEG.  nsplit == 1 writes the image; otherwise
 // the work-group's sums go to part[h * 64 + lane] for k_adjoint_sym_combine.
-template <bool FBP, int SEG, int GRP, int WIN>
+template <bool FBP, int SEG, int GRP, int WIN, bool STAGE>
 __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& it,
                                              const float4* __restrict__ views,
                                              const Ray* __restrict__ rays,
@@ -687,7 +848,8 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
                                              const float* __restrict__ zero_row,
                                              const int* __restrict__ symtab,
                                              float* __restrict__ img, float (*red)[8][64],
-                                             int split, int nsplit, float* __restrict__ part)
+                                             int split, int nsplit, float* __restrict__ part,
+                                             AdjStage* __restrict__ stage)
 {
     constexpr GroupMasks G = group_masks(GRP);
     // the wave index is uniform over the wave: the view loop then runs on scalar registers
@@ -696,14 +858,15 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
     int x0, y0;
     const bool valid = adj_sym_base<GRP>(S, it, lane, x0, y0);
     KahanAcc acc[8];
-    if (valid) {
+    if (STAGE || valid) {
         const int i0 = split * SEG + wave, step = SEG * nsplit;
+        AdjStage* st = STAGE ? stage + wave : nullptr;
         if (S.fan)
-            adj_sym_pixel<true, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
-                                               i0, step, acc);
+            adj_sym_pixel<true, FBP, GRP, WIN, STAGE>(S, views, rays, sino, zero_row, symtab, x0,
+                                                      y0, valid, i0, step, acc, st);
         else
-            adj_sym_pixel<false, FBP, GRP, WIN>(S, views, rays, sino, zero_row, symtab, x0, y0,
-                                                i0, step, acc);
+            adj_sym_pixel<false, FBP, GRP, WIN, STAGE>(S, views, rays, sino, zero_row, symtab, x0,
+                                                       y0, valid, i0, step, acc, st);
     }
     if (SEG > 1) {
 #pragma unroll
@@ -732,8 +895,9 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 // One launch per symmetry group; SEG waves split the canonical views (wave w takes
 // i = w, w + SEG, ...) and then share the final sums; with nsplit > 1, nsplit
 // work-groups share the views of a tile and leave their sums in part (512 floats per
-// work-group) for k_adjoint_sym_combine.
-template <bool FBP, int SEG, int GRP, int WIN>
+// work-group) for k_adjoint_sym_combine.  STAGE: the gathers off the mirror views go
+// through each wave's LDS area (adj_sym_elem_lds).
+template <bool FBP, int SEG, int GRP, int WIN, bool STAGE>
 __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
@@ -745,13 +909,16 @@ __global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __res
                                                           int nsplit, float* __restrict__ part)
 {
     __shared__ float red[SEG][8][64];
+    __shared__ AdjStage stage[STAGE ? SEG : 1];
     int split, idx;
     split_item(nitems, split, idx);
     const AdjItem it = items[idx];
     if (it.slice < 0) return;  // uniform over the work-group
     const SliceDesc S = slices[it.slice];
-    adj_sym_tile<FBP, SEG, GRP, WIN>(S, it, views, rays, sino, zero_row, symtab, img, red, split,
-                                     nsplit, part ? part + (long long)blockIdx.x * 512 : nullptr);
+    adj_sym_tile<FBP, SEG, GRP, WIN, STAGE>(S, it, views, rays, sino, zero_row, symtab, img, red,
+                                            split, nsplit,
+                                            part ? part + (long long)blockIdx.x * 512 : nullptr,
+                                            stage);
 }
 
 // The nsplit partial sums of each orbit image of a tile, added in split order and
@@ -822,21 +989,30 @@ void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
 }
 
 template <bool FBP, int GRP, int WIN>
-static void launch_adjoint_sym_t(int seg, int nsplit, float* part, int nitems,
+static void launch_adjoint_sym_t(int seg, bool stage, int nsplit, float* part, int nitems,
                                  const SliceDesc* desc, const AdjItem* items, const float4* views,
                                  const Ray* rays, const float* sino, const float* zero_row,
                                  const int* symtab, float* img, hipStream_t st)
 {
     float* pp = nsplit > 1 ? part : nullptr;
     const dim3 grid((unsigned)nitems * (unsigned)nsplit);
-#define CTPRJ_LAUNCH(SEG)                                                                      \
-    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN>), grid, dim3(64 * SEG), 0, st, desc,  \
-                       items, views, rays, sino, zero_row, symtab, img, nitems, nsplit, pp)
-    switch (seg) {
-        case 1: CTPRJ_LAUNCH(1); break;
-        case 2: CTPRJ_LAUNCH(2); break;
-        case 4: CTPRJ_LAUNCH(4); break;
-        default: CTPRJ_LAUNCH(8); break;
+#define CTPRJ_LAUNCH(SEG, STAGE)                                                                 \
+    hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN, STAGE>), grid, dim3(64 * SEG), 0, st,  \
+                       desc, items, views, rays, sino, zero_row, symtab, img, nitems, nsplit, pp)
+    if (stage) {
+        switch (seg) {
+            case 1: CTPRJ_LAUNCH(1, true); break;
+            case 2: CTPRJ_LAUNCH(2, true); break;
+            case 4: CTPRJ_LAUNCH(4, true); break;
+            default: CTPRJ_LAUNCH(8, true); break;
+        }
+    } else {
+        switch (seg) {
+            case 1: CTPRJ_LAUNCH(1, false); break;
+            case 2: CTPRJ_LAUNCH(2, false); break;
+            case 4: CTPRJ_LAUNCH(4, false); break;
+            default: CTPRJ_LAUNCH(8, false); break;
+        }
     }
 #undef CTPRJ_LAUNCH
     if (pp)
@@ -845,39 +1021,39 @@ static void launch_adjoint_sym_t(int seg, int nsplit, float* part, int nitems,
 }
 
 template <bool FBP, int WIN>
-static void launch_adjoint_sym_f(int grp, int seg, int nsplit, float* part, int nitems,
+static void launch_adjoint_sym_f(int grp, int seg, bool stage, int nsplit, float* part, int nitems,
                                  const SliceDesc* desc, const AdjItem* items, const float4* views,
                                  const Ray* rays, const float* sino, const float* zero_row,
                                  const int* symtab, float* img, hipStream_t st)
 {
     if (grp == kGrpD4)
-        launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
-                                               sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, stage, nsplit, part, nitems, desc, items,
+                                               views, rays, sino, zero_row, symtab, img, st);
     else if (grp == kGrpV4)
-        launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
-                                               sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, stage, nsplit, part, nitems, desc, items,
+                                               views, rays, sino, zero_row, symtab, img, st);
     else
-        launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, nsplit, part, nitems, desc, items, views, rays,
-                                               sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, stage, nsplit, part, nitems, desc, items,
+                                               views, rays, sino, zero_row, symtab, img, st);
 }
 
-void launch_adjoint_sym(bool fbp, int grp, int seg, int win, int nsplit, float* part, int nitems,
-                        const SliceDesc* desc, const AdjItem* items, const float4* views,
-                        const Ray* rays, const float* sino, const float* zero_row,
-                        const int* symtab, float* img, hipStream_t st)
+void launch_adjoint_sym(bool fbp, int grp, int seg, int win, bool stage, int nsplit, float* part,
+                        int nitems, const SliceDesc* desc, const AdjItem* items,
+                        const float4* views, const Ray* rays, const float* sino,
+                        const float* zero_row, const int* symtab, float* img, hipStream_t st)
 {
     if (fbp && win == 3)
-        launch_adjoint_sym_f<true, 3>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
-                                      sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_f<true, 3>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
+                                      rays, sino, zero_row, symtab, img, st);
     else if (fbp)
-        launch_adjoint_sym_f<true, 4>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
-                                      sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_f<true, 4>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
+                                      rays, sino, zero_row, symtab, img, st);
     else if (win == 3)
-        launch_adjoint_sym_f<false, 3>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
-                                       sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_f<false, 3>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
+                                       rays, sino, zero_row, symtab, img, st);
     else
-        launch_adjoint_sym_f<false, 4>(grp, seg, nsplit, part, nitems, desc, items, views, rays,
-                                       sino, zero_row, symtab, img, st);
+        launch_adjoint_sym_f<false, 4>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
+                                       rays, sino, zero_row, symtab, img, st);
 }
 
 

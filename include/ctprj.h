@@ -114,7 +114,9 @@ typedef struct ctprj_tune {
                         bit1: not the 8-fold group; bit2: not the 4-fold group;
                         bit3: work order keeps small slices whole on one XCD (else every
                         slice is spread over all XCDs);
-                        bit4: the symmetric adjoint reads 3-bin windows (else 4) */
+                        bit4: the symmetric adjoint reads 3-bin windows (else 4);
+                        bit5: the symmetric adjoint's waves stage the bins of each view
+                        and element in LDS (else every lane loads its own windows) */
     int adj_seg;     /* symmetric adjoint: waves per 8x8 tile of orbit base pixels
                         splitting the views: 1, 2, 4 or 8 */
     int fwd_seg;     /* symmetric forward: waves per 64 canonical rays splitting the

@@ -248,6 +248,26 @@ def test_adjoint_window_width_is_bitwise_neutral(torch_dev, geo, flags):
 
 
 @pytest.mark.parametrize("geo,flags", WINDOW_CASES)
+def test_adjoint_lds_staging_is_bitwise_neutral(torch_dev, geo, flags):
+    """The symmetric adjoint's gathers through each wave's LDS area (tune flag bit 5:
+    the wave stages the bins its pixels can read once per view and element) only change
+    where the values come from: the same weights, the same order of sums, so the adjoint
+    and FBP are bit-identical to the per-lane gathers, with 3- and 4-bin windows and
+    with split work items; the fine-pitch case exceeds the staged width and falls back."""
+    import torch
+    from imgrecsrc_amd.phantom import gaussian
+    P = _proj(geo, flags)
+    y = torch.from_numpy(gaussian((geo[2], geo[1]), 17)).cuda().reshape(-1)
+    for win, split in ((0, 1), (16, 1), (16, 3)):
+        P.set_tune(flags=flags | win, adj_split=split)
+        ref, fbp = P.adjoint(y).clone(), P.fbp(y).clone()
+        P.set_tune(flags=flags | win | 32)
+        assert torch.equal(P.adjoint(y), ref), (win, split)
+        assert torch.equal(P.fbp(y), fbp), (win, split)
+        P.set_tune(flags=flags | win, adj_split=1)
+
+
+@pytest.mark.parametrize("geo,flags", WINDOW_CASES)
 def test_split_work_items_match_one_work_group(torch_dev, geo, flags):
     """Work-groups per item (tune adj_split / fwd_split) only regroup the partial sums of
     the views (adjoint) or image rows (forward): forward, adjoint and FBP agree with the

@@ -1,8 +1,8 @@
 """Seeded random geometries (image size, detector width and pitch, views used and per
 turn, effectiveRate, parallel or fan): the device forward and adjoint against the FP64
 oracle at the angles the chosen kernels use, on the symmetric and the general kernels,
-with the 3- and 4-bin adjoint windows and with work items split over several
-work-groups, plus the adjoint identity.  Bars: 2e-6 of the
+with the 3- and 4-bin adjoint windows, per-lane and LDS-staged adjoint gathers, and
+with work items split over several work-groups, plus the adjoint identity.  Bars: 2e-6 of the
 FP64 result's scale per entry, and 1e-6 of sum |x| A^T|y| for <Ax,y> = <x,A^T y>."""
 import numpy as np
 import pytest
@@ -37,7 +37,7 @@ def test_random_geometry_against_fp64(torch_dev, oracle_mod, geo):
     xt = torch.from_numpy(x).cuda().reshape(-1)
     yt = torch.from_numpy(y).cuda().reshape(-1)
     # (flags, adjoint / forward work-groups per item)
-    for flags, split in ((1, 1), (17, 1), (0, 1), (1, 3), (17, 2)):
+    for flags, split in ((1, 1), (17, 1), (0, 1), (1, 3), (17, 2), (33, 1), (49, 2)):
         P = _proj(geo, flags)
         if split > 1:
             P.set_tune(adj_split=split, fwd_split=5 - split)
