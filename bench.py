@@ -35,6 +35,11 @@ from imgrecsrc_amd.workloads import FAN1024, FAN512, PAR256, STACK512, ragged_ge
 WORKLOADS = {"fan1024": FAN1024, "par256": PAR256, "fan512": FAN512, "par512": STACK512}
 
 
+# Autotuner cache of every plan the bench creates (set from --tune-cache).
+DEFAULT_TUNE_CACHE = os.path.join(ROOT, "imgrecsrc_amd", "tunes", "mi355x.tune")
+TUNE_CACHE = None
+
+
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
@@ -213,14 +218,15 @@ def _summary(tag, suffix):
 def roofline_entry(tag, kernel, kernel_ms, launches, nnz, tune, other_ms):
     """The dominant call against the ceiling that binds it.  The symmetric kernels reuse
     every loaded value for 2-8 updates and move little HBM traffic; their counters
-    (committed under profiles/, taken at the same tuned shapes) show the vector-memory
-    pipeline (TA/TD: 64 B/clk/CU) as the busiest unit.  frac = TA busy cycles per call
-    (TA_TA_BUSY summed over the 256 TAs, profiles/<tag>_l1.json) / (256 x the call's live
-    duration x 2.4 GHz); achieved = frac x 39.3 TB/s, the data rate that busy time
-    carries.  Also reported: VALU issue (SQ_INSTS_VALU x 64 lane-ops per call over the
-    live duration, against 78.6 T lane-op/s) and HBM (measured FETCH_SIZE + WRITE_SIZE
-    over the same duration).  Without summaries at these shapes: HBM from the measured
-    traffic, or nothing."""
+    (committed under profiles/, taken at the same tuned shapes) put the vector-memory
+    pipeline (TA address and TD data units, 64 B/clk/CU) or the VALU highest.  Vector
+    memory: busy cycles per call (TA_TA_BUSY or TD_TD_BUSY summed over the 256 units,
+    profiles/<tag>_l1.json) / (256 x the call's live duration x 2.4 GHz), the larger of
+    the two; achieved = that fraction x 39.3 TB/s, the data rate the busy time carries.
+    VALU: SQ_INSTS_VALU x 64 lane-ops per call over the live duration against
+    78.6 T lane-op/s.  `bound` is the busier of the two, `frac` its fraction; the other
+    and HBM (measured FETCH_SIZE + WRITE_SIZE over the same duration) are reported beside
+    it.  Without summaries at these shapes: HBM from the measured traffic, or nothing."""
     traffic = traffic_from_profiles(tag)
     sec = kernel_ms * 1e-3
     hbm = None
@@ -236,18 +242,27 @@ def roofline_entry(tag, kernel, kernel_ms, launches, nnz, tune, other_ms):
             and l1.get("tune") == tune):
         lane_ops = float(valu["lane_ops_per_call"])
         vfrac = lane_ops / sec / valu_peak_lane_ops()
-        busy = float(l1["call"]["TA_TA_BUSY"])
-        frac = busy / (CUS * sec * CLOCK_GHZ * 1e9)
-        out = {"bound": "l1", "achieved": round(frac * l1_peak_bytes() / 1e12, 2),
-               "peak": round(l1_peak_bytes() / 1e12, 2), "unit": "TB/s", "frac": round(frac, 4)}
+        cyc = CUS * sec * CLOCK_GHZ * 1e9
+        ta = float(l1["call"]["TA_TA_BUSY"])
+        td = float(l1["call"].get("TD_TD_BUSY", 0.0))
+        mfrac = max(ta, td) / cyc
+        vmem = {"achieved": round(mfrac * l1_peak_bytes() / 1e12, 2),
+                "peak": round(l1_peak_bytes() / 1e12, 2), "unit": "TB/s",
+                "frac": round(mfrac, 4), "ta_frac": round(ta / cyc, 4),
+                "td_frac": round(td / cyc, 4), "ta_busy_cycles_per_call": ta,
+                "td_busy_cycles_per_call": td}
+        vl = {"achieved": round(lane_ops / sec / 1e12, 2),
+              "peak": round(valu_peak_lane_ops() / 1e12, 2), "unit": "Tlane-op/s",
+              "frac": round(vfrac, 4), "lane_ops_per_update": round(lane_ops / nnz, 3),
+              "min_lane_ops_per_update": VALU_MIN_OPS_PER_UPDATE}
+        if mfrac >= vfrac:
+            out = {"bound": "l1", "achieved": vmem["achieved"], "peak": vmem["peak"],
+                   "unit": "TB/s", "frac": vmem["frac"]}
+        else:
+            out = {"bound": "valu", "achieved": vl["achieved"], "peak": vl["peak"],
+                   "unit": "Tlane-op/s", "frac": vl["frac"]}
         out.update(base)
-        out.update({"ta_busy_cycles_per_call": busy,
-                    "valu": {"achieved": round(lane_ops / sec / 1e12, 2),
-                             "peak": round(valu_peak_lane_ops() / 1e12, 2),
-                             "unit": "Tlane-op/s", "frac": round(vfrac, 4),
-                             "lane_ops_per_update": round(lane_ops / nnz, 3),
-                             "min_lane_ops_per_update": VALU_MIN_OPS_PER_UPDATE},
-                    "hbm": hbm, "clock_GHz": CLOCK_GHZ,
+        out.update({"l1": vmem, "valu": vl, "hbm": hbm, "clock_GHz": CLOCK_GHZ,
                     "profiles": [f"profiles/{tag}_l1.json", f"profiles/{tag}_valu.json"]})
         return out
     if hbm is not None:
@@ -333,7 +348,7 @@ def stack_problem(nslices, dev, tune_reps=2):
     from imgrecsrc_amd.workloads import STACK512, stack_truth
 
     P = Projector([STACK512] * nslices)
-    P.autotune(reps=tune_reps)
+    P.autotune(reps=tune_reps, cache_path=TUNE_CACHE)
     truth = torch.from_numpy(stack_truth(nslices)).to(dev).reshape(-1)
     knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
     Ie = normalize_ie(np.exp(-0.5 * ((np.arange(20) - 7.5) / 3.0) ** 2) + 0.05, knots)
@@ -359,7 +374,7 @@ def stack_batched_vs_plain(P, Imea, knots, Ie, Ts, dev, stream):
 
     ns = P.nslices
     one = Projector([STACK512])
-    one.autotune(reps=2)
+    one.autotune(reps=2, cache_path=TUNE_CACHE)
     ni, nr = one.img_elems, one.sino_elems
     x = torch.from_numpy(stack_truth(ns)).to(dev).reshape(-1)
     y = torch.empty(P.sino_elems, dtype=torch.float32, device=dev)
@@ -548,10 +563,15 @@ def main():
     ap.add_argument("--valu-tune", default=None,
                     help="JSON of the tuned shapes the counter run used (recorded in the "
                          "summary; the bench uses a summary only at the same shapes)")
-    ap.add_argument("--tune-cache", default=None,
-                    help="autotuner cache file: a later run with the same file reuses the "
-                         "shapes the first one chose (profiling runs)")
+    ap.add_argument("--tune-cache", default=DEFAULT_TUNE_CACHE,
+                    help="autotuner cache file (every leg's plans): shapes recorded there "
+                         "for this device and geometry are reused, others are tuned and "
+                         "appended; the default is the committed MI355X cache, whose shapes "
+                         "the committed counter summaries were taken at; 'none' tunes "
+                         "afresh")
     args = ap.parse_args()
+    global TUNE_CACHE
+    TUNE_CACHE = None if args.tune_cache in (None, "", "none") else args.tune_cache
     legs = set(x.strip() for x in args.legs.split(",") if x.strip())
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.pmc_command)
@@ -580,7 +600,7 @@ def main():
     n, W, npj = geo[0], geo[1], geo[2]
     log(f"workload {args.workload} {geo}")
     P = Projector([geo])
-    tune = P.autotune(reps=5, cache_path=args.tune_cache)
+    tune = P.autotune(reps=5, cache_path=TUNE_CACHE)
     log(f"autotuned shapes {tune}")
     img_np = ellipse_phantom(n, 2)
     img = torch.from_numpy(img_np).to(dev).reshape(-1)
@@ -656,7 +676,7 @@ def main():
         log("bhc leg")
         bg = FAN512
         PB = Projector([bg])
-        PB.autotune(reps=3)
+        PB.autotune(reps=3, cache_path=TUNE_CACHE)
         knots = spline_knots(kappa_logspace(20, 0.03, 30.0))
         xs = np.arange(20)
         Ie = normalize_ie(np.exp(-0.5 * ((xs - 7.5) / 3.0) ** 2) + 0.05, knots)
@@ -780,7 +800,7 @@ def main():
             log("batched leg")
             geoms = ragged_geoms()
             PBat = Projector(geoms)
-            PBat.autotune(reps=3)
+            PBat.autotune(reps=3, cache_path=TUNE_CACHE)
             imgs = [torch.from_numpy(ellipse_phantom(gq[0], 100 + i)).to(dev).reshape(-1)
                     for i, gq in enumerate(geoms)]
             bimg = PBat.pack_images(imgs)
@@ -789,11 +809,11 @@ def main():
             tot_nnz = PBat.nnz()
             singles = [Projector([gq]) for gq in geoms]
             for s1 in singles:
-                s1.autotune(reps=3)
+                s1.autotune(reps=3, cache_path=TUNE_CACHE)
             # the plain path on the general (per-view) kernels, one slice per launch
             generals = [Projector([gq], symmetric=False) for gq in geoms]
             for s1 in generals:
-                s1.autotune(reps=3)
+                s1.autotune(reps=3, cache_path=TUNE_CACHE)
             s_out = [torch.empty(s.sino_elems, dtype=torch.float32, device=dev) for s in singles]
             i_out = [torch.empty(s.img_elems, dtype=torch.float32, device=dev) for s in singles]
 
@@ -861,7 +881,7 @@ def main():
     if not args.no_extras and "par256" in legs:
         # ---- the 256^2 parallel slice (180 views over 360, 367 bins): plain GPU times
         PC1 = Projector([PAR256])
-        PC1.autotune(reps=3)
+        PC1.autotune(reps=3, cache_path=TUNE_CACHE)
         c1_img = torch.from_numpy(ellipse_phantom(256, 1)).to(dev).reshape(-1)
         c1_s = PC1.forward(c1_img)
         c1_b = PC1.adjoint(c1_s)

@@ -134,7 +134,7 @@ def _grad_alpha(noise, Imea, Ie, alpha, proj, Ts, knots, want_grad, want_hess, b
     E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
     set_model(proj, basis, noise)
     dev = alpha.device
-    f = torch.zeros(1, dtype=torch.float64, device=dev)
+    f = torch.empty(1, dtype=torch.float64, device=dev)  # written by the call
     g = torch.empty(proj.img_elems, dtype=torch.float32, device=dev) if want_grad else None
     hw = torch.empty(proj.sino_elems, dtype=torch.float32, device=dev) if want_hess else None
     check(lib().ctprj_bhc_grad_alpha(proj.handle, _ptr(alpha), _ptr(Imea), kp, E, ip,
@@ -171,7 +171,7 @@ def alpha_rays(proj: Projector, sraw: torch.Tensor, Imea: torch.Tensor, Ie, knot
     Imea = _f32(Imea.reshape(-1), proj.sino_elems, "Imea")
     E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
     set_model(proj, basis, noise)
-    f = torch.zeros(1, dtype=torch.float64, device=sraw.device)
+    f = torch.empty(1, dtype=torch.float64, device=sraw.device)  # written by the call
     w = torch.empty_like(sraw) if want_w else None
     check(lib().ctprj_bhc_alpha_rays(proj.handle, _ptr(sraw), _ptr(Imea), kp, E, ip, float(Ts),
                                      _ptr(f), _ptr(w) if w is not None else None, None,
@@ -186,9 +186,10 @@ def _grad_ie(noise, Imea, Ie, proj, Ts, knots, alpha, sraw, basis):
     E, kp, _kh, ip, _ih = _knots_ie(knots, Ie, basis)
     set_model(proj, basis, noise)
     dev = Imea.device
-    f = torch.zeros(1, dtype=torch.float64, device=dev)
-    g = torch.zeros(E, dtype=torch.float64, device=dev)
-    H = torch.zeros((E, E), dtype=torch.float64, device=dev)
+    # every entry is written by the call
+    f = torch.empty(1, dtype=torch.float64, device=dev)
+    g = torch.empty(E, dtype=torch.float64, device=dev)
+    H = torch.empty((E, E), dtype=torch.float64, device=dev)
     a = _ptr(_f32(alpha.reshape(-1), proj.img_elems, "alpha")) if alpha is not None else None
     s = _ptr(_f32(sraw.reshape(-1), proj.sino_elems, "sraw")) if sraw is not None else None
     check(lib().ctprj_bhc_grad_ie(proj.handle, a, s, _ptr(Imea), kp, E, ip, float(Ts), _ptr(f),

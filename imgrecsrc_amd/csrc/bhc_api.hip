@@ -49,20 +49,13 @@ static int make_params(const Plan* p, const double* knots, int E, const double* 
     return make_bhc_params(knots, E, Ie, p->bhc_basis, p->bhc_noise, P);
 }
 
-static int ensure_red(Plan* p, long long doubles)
+// The plan holds the reduction partials of every beam-hardening pass from its creation
+// (bhc_red_doubles); a call never allocates.
+static int check_red(const Plan* p, long long doubles)
 {
-    if (p->red_cap >= doubles) return 0;
-    if (p->d_red) (void)hipFree(p->d_red);
-    p->d_red = nullptr;
-    p->red_cap = 0;
-    hipError_t e = hipMalloc((void**)&p->d_red, sizeof(double) * (size_t)doubles);
-    if (e != hipSuccess) return fail(CTPRJ_ENOMEM, "reduction workspace: %s", hipGetErrorString(e));
-    p->red_cap = doubles;
-    if (!p->d_red_out) {
-        e = hipMalloc((void**)&p->d_red_out, sizeof(double) * (size_t)bhc_ie_nvals(CTPRJ_MAX_E));
-        if (e != hipSuccess) return fail(CTPRJ_ENOMEM, "reduction output: %s", hipGetErrorString(e));
-    }
-    return 0;
+    if (p->red_cap >= doubles && p->d_red && p->d_red_out) return 0;
+    return fail(CTPRJ_ESTATE, "reduction workspace: %lld doubles needed, %lld held", doubles,
+                p->red_cap);
 }
 
 static int last_launch(const char* what)
@@ -130,7 +123,7 @@ extern "C" int ctprj_bhc_grad_alpha(ctprj_plan* p, const float* d_alpha, const f
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
-    if ((rc = ensure_red(p, blocks))) return rc;
+    if ((rc = check_red(p, blocks))) return rc;
     // s = A alpha  (Ts applied inside the ray pass)
     if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
     float* w = d_grad ? p->d_sino_ws : nullptr;
@@ -153,7 +146,7 @@ extern "C" int ctprj_bhc_alpha_rays(ctprj_plan* p, const float* d_sraw, const fl
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
-    if ((rc = ensure_red(p, blocks))) return rc;
+    if ((rc = check_red(p, blocks))) return rc;
     launch_bhc_alpha_ray(P, d_sraw, Ts, d_Imea, rays, Ts, d_w, d_hw, p->d_red, st);
     if ((rc = last_launch("bhc alpha ray launch"))) return rc;
     launch_reduce_partials(p->d_red, blocks, 1, d_f, st);
@@ -238,9 +231,9 @@ extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const floa
     int rc = make_params(p, knots, E, Ie, &P);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
+    const long long rays = p->sino_elems, groups = bhc_ie_groups(rays);
     const int nv = bhc_ie_nvals(E);
-    if ((rc = ensure_red(p, blocks * nv))) return rc;
+    if ((rc = check_red(p, groups * nv))) return rc;
     const float* s = d_s;
     if (!s) {
         if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
@@ -248,7 +241,7 @@ extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const floa
     }
     launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st);
     if ((rc = last_launch("bhc ie ray launch"))) return rc;
-    launch_reduce_partials(p->d_red, blocks, nv, p->d_red_out, st);
+    launch_reduce_partials(p->d_red, groups, nv, p->d_red_out, st);
     if ((rc = last_launch("bhc ie reduce launch"))) return rc;
     launch_unpack_ie(p->d_red_out, E, d_f, d_g, d_H, st);
     return last_launch("bhc ie unpack launch");
@@ -301,9 +294,9 @@ extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* 
     int rc = make_params(p, knots, E, Ie, &P);
     if (rc) return rc;
     hipStream_t st = (hipStream_t)stream;
-    const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
+    const long long rays = p->sino_elems, blocks = bhc_blocks(rays), groups = bhc_ie_groups(rays);
     const int nv = bhc_ie_nvals(E);
-    if ((rc = ensure_red(p, blocks * nv))) return rc;
+    if ((rc = check_red(p, std::max(blocks, groups * nv)))) return rc;
     const float* s = d_sraw;
     if (!s) {  // the projection stays fixed over the I-epsilon step
         if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
@@ -322,7 +315,7 @@ extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* 
     auto full = [&](double* f, std::vector<double>* g, std::vector<double>* H) -> int {
         set_ie(x);
         launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st);
-        launch_reduce_partials(p->d_red, blocks, nv, p->d_red_out, st);
+        launch_reduce_partials(p->d_red, groups, nv, p->d_red_out, st);
         hipError_t e = hipMemcpyAsync(red.data(), p->d_red_out, sizeof(double) * nv,
                                       hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);

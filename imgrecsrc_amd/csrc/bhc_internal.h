@@ -39,7 +39,8 @@ void launch_poly_iout(const BhcParams& P, const float* s, long long count, float
 void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                           long long count, float Ts, float* w, float* hw, double* partials,
                           hipStream_t st);
-// per-ray I-epsilon pass: per work-group partial [H upper, g, f]
+// per-ray I-epsilon pass: per work-group partial [H upper, g, f], bhc_ie_groups(count)
+// work-groups
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                        long long count, double* partials, hipStream_t st);
 // cost only of the I-epsilon pass: one FP64 partial per work-group
@@ -52,5 +53,24 @@ void launch_reduce_partials(const double* partials, long long nparts, int nvals,
 void launch_unpack_ie(const double* red, int E, double* f, double* g, double* H, hipStream_t st);
 
 inline long long bhc_blocks(long long count) { return (count + kBhcBlock - 1) / kBhcBlock; }
+
+// Work-groups of the I-epsilon pass: at most kIeMaxGroups, each taking the same number
+// of kBhcBlock-ray tiles (but the last).
+constexpr long long kIeMaxGroups = 4096;
+inline long long bhc_ie_groups(long long count)
+{
+    const long long tiles = bhc_blocks(count);
+    if (tiles <= 0) return 0;
+    const long long per = (tiles + kIeMaxGroups - 1) / kIeMaxGroups;
+    return (tiles + per - 1) / per;
+}
+
+// Doubles of reduction partials a plan of `rays` rays holds for its beam-hardening calls:
+// one per kBhcBlock rays (NLL and cost passes) or the I-epsilon pass's groups x values.
+inline long long bhc_red_doubles(long long rays)
+{
+    const long long a = bhc_blocks(rays), b = bhc_ie_groups(rays) * bhc_ie_nvals(CTPRJ_MAX_E);
+    return a > b ? a : b;
+}
 
 }  // namespace ctprj

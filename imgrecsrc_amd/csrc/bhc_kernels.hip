@@ -41,16 +41,17 @@ struct Moments {
     float m0, m1, m2, m3;
 };
 
-// The first NM (1..4) moments by the series (|z| <= 2 only).
-template <int NM>
+// The first NM (1..4) moments by their first TERMS series terms (|z| <= 2: 14 terms;
+// |z| <= 1: 11 terms, truncation below 2e-9 of M_n).
+template <int NM, int TERMS = kMomentTerms>
 __device__ __forceinline__ Moments moments_series(float z)
 {
     float m[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
     for (int n = 0; n < NM; ++n) {
-        float a = kMoment.c[n][kMomentTerms - 1];
+        float a = kMoment.c[n][TERMS - 1];
 #pragma unroll
-        for (int k = kMomentTerms - 2; k >= 0; --k) a = fmaf(a, z, kMoment.c[n][k]);
+        for (int k = TERMS - 2; k >= 0; --k) a = fmaf(a, z, kMoment.c[n][k]);
         m[n] = a;
     }
     return Moments{m[0], m[1], m[2], m[3]};
@@ -71,14 +72,18 @@ __device__ __forceinline__ Moments moments(float z)
 
 // Weighted moments W_n = w M_n(z) of b1 segment j, w = Delta_j e_j, z = Delta_j s, from
 // the exponentials e_j = exp(-k_j s) and e_{j+1} = exp(-k_{j+1} s) of its two knots
-// (e^-z = e_{j+1}/e_j): the series for |z| <= 2, else the recurrence
+// (e^-z = e_{j+1}/e_j): the series for small |z|, else the recurrence
 // W_0 = Delta_j (e_j - e_{j+1})/z, W_n = (n W_{n-1} - Delta_j e_{j+1})/z, which needs no
-// exponential of its own.  NM = moments needed (2..4).
+// exponential of its own.  NM = moments needed (2..4).  Up to M_2 the series switches
+// at |z| = 1 (11 terms; the recurrence then stays within 1.1e-6 of M_2), M_3 (second
+// derivatives) keeps the switch at 2 (14 terms), where its recurrence error is 8e-7.
 template <int NM>
 __device__ __forceinline__ Moments weighted_moments(float z, float dj, float ej, float ej1)
 {
-    if (fabsf(z) <= 2.0f) {
-        const Moments M = moments_series<NM>(z);
+    constexpr float kSwitch = NM >= 4 ? 2.0f : 1.0f;
+    constexpr int kTerms = NM >= 4 ? kMomentTerms : 11;
+    if (fabsf(z) <= kSwitch) {
+        const Moments M = moments_series<NM, kTerms>(z);
         const float w = dj * ej;
         return Moments{w * M.m0, w * M.m1, w * M.m2, w * M.m3};
     }
@@ -256,18 +261,21 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 
 // ------------------------------------------------------------ I-epsilon pass ---
 
-// Each work-group takes kBhcBlock rays: the basis values phi_ij (FP32) and the ray
-// weights q_i, r_i, f_i (FP64) go to LDS, then the group forms [H upper triangle
-// (row-major j<=k), g, f] over its rays:
+// A work-group takes tiles of kBhcBlock rays (tile = group, group + G, ...): the basis
+// values phi_ij (FP32) and the ray weights q_i, r_i, f_i (FP64) of a tile go to LDS, and
+// the group adds the tile's contribution to its sums [H upper triangle (row-major j<=k),
+// g, f]:
 //   H_jk = sum_i phi_ij phi_ik q_i,  g_j = sum_i phi_ij r_i,  f = sum_i f_i,
 //   Ir_i = sum_j Ie_j phi_ij (FP64), with
 //   Poisson  (bhc/poissLI.m):  q = Imea/Ir^2,       r = 1 - Imea/Ir,  f_i = nll(Ir, Imea)
 //   Gaussian (bhc/gaussLI.m):  q = (1 - Err)/Ir^2,  r = Err/Ir,       f_i = Err^2/2.
 // Register blocking: the outputs are cut into 4x4 blocks of H (upper block triangle)
 // and 4-entry blocks of g (the last one also carries f); thread t takes block
-// slot = t % S over the rays lane, lane + L, ... with lane = t / S (S = slots rounded up
-// to a power of two, L = kBhcBlock / S), so each LDS read of 4 basis values feeds 4 or
-// 16 FP64 FMAs.  The L lane sums of every output are then added in lane order.
+// slot = t % S over the rays lane, lane + L, ... of every tile with lane = t / S (S =
+// slots rounded up to a power of two, L = kBhcBlock / S), so each LDS read of 4 basis
+// values feeds 4 or 16 FP64 FMAs.  The L lane sums of every output are then added in
+// lane order.  The grid is capped (bhc_ie_groups), so the partials take at most
+// kIeMaxGroups * nvals doubles whatever the ray count.
 constexpr int kIeStride = ((CTPRJ_MAX_E + 3) / 4) * 4;  // phi row: a multiple of 4 floats
 
 __host__ __device__ inline int ie_blocks(int E) { return (E + 3) / 4; }
@@ -282,7 +290,7 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
                                                           long long count, float sscale,
                                                           double* __restrict__ partials)
 {
-    // the basis values, then (after a barrier) the lane sums share one LDS area
+    // the basis values, then (after the last tile) the lane sums share one LDS area
     constexpr size_t kPhiBytes = sizeof(float) * kBhcBlock * kIeStride;
     constexpr size_t kSumBytes = sizeof(double) * kBhcBlock * 17;
     __shared__ __attribute__((aligned(16))) char area[kPhiBytes > kSumBytes ? kPhiBytes : kSumBytes];
@@ -290,70 +298,77 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
     auto phi = reinterpret_cast<float(*)[kIeStride]>(area);
     auto lanesum = reinterpret_cast<double(*)[17]>(area);  // [thread][16 block values + f]
     const int t = threadIdx.x;
-    const long long i = (long long)blockIdx.x * kBhcBlock + t;
     const int E = P.E;
-    for (int j = 0; j < kIeStride; ++j) phi[t][j] = 0.0f;
-    if (i < count) {
-        const float si = sscale * s[i];
-        const double im = (double)Imea[i];
-        double ir = 0.0;
-        spline_basis<1>(P, si, [&](int j, float p, float, float) {
-            phi[t][j] = p;
-            ir = fma(P.Ied[j], (double)p, ir);
-        });
-        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
-            const double err = gauss_err(ir, im);
-            q[t] = (1.0 - err) / (ir * ir);
-            r[t] = err / ir;
-            ft[t] = 0.5 * err * err;
-        } else {
-            q[t] = im / (ir * ir);
-            r[t] = 1.0 - im / ir;
-            ft[t] = nll_term(ir, im);
-        }
-    } else {
-        q[t] = 0.0;
-        r[t] = 0.0;
-        ft[t] = 0.0;
-    }
-    __syncthreads();
     const int nb = ie_blocks(E), nhb = nb * (nb + 1) / 2, nslot = nhb + nb;
     int S = 1;
     while (S < nslot) S <<= 1;  // nslot <= 44 for E <= 32, so S <= 64
     const int L = kBhcBlock / S, slot = t % S, lane = t / S;
+    int bi = 0, bj = 0;
+    if (slot < nhb) {
+        int rem = slot;  // block row bi of the upper triangle holds nb-bi blocks
+        while (rem >= nb - bi) { rem -= nb - bi; ++bi; }
+        bj = bi + rem;
+    } else {
+        bj = slot - nhb;
+    }
+    const bool last = slot >= nhb && bj == nb - 1;
     double acc[17];
 #pragma unroll
     for (int v = 0; v < 17; ++v) acc[v] = 0.0;
-    if (slot < nhb) {
-        int bi = 0, rem = slot;  // block row bi of the upper triangle holds nb-bi blocks
-        while (rem >= nb - bi) { rem -= nb - bi; ++bi; }
-        const int bj = bi + rem;
-        for (int m = lane; m < kBhcBlock; m += L) {
-            const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bi]);
-            const float4 b = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
-            const double w = q[m];
-            const double ad[4] = {(double)a.x * w, (double)a.y * w, (double)a.z * w,
-                                  (double)a.w * w};
-            const double bd[4] = {(double)b.x, (double)b.y, (double)b.z, (double)b.w};
-#pragma unroll
-            for (int x = 0; x < 4; ++x)
-#pragma unroll
-                for (int y = 0; y < 4; ++y) acc[4 * x + y] = fma(ad[x], bd[y], acc[4 * x + y]);
+    const long long ntiles = (count + kBhcBlock - 1) / kBhcBlock;
+    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const long long i = tile * kBhcBlock + t;
+        for (int j = 0; j < kIeStride; ++j) phi[t][j] = 0.0f;
+        if (i < count) {
+            const float si = sscale * s[i];
+            const double im = (double)Imea[i];
+            double ir = 0.0;
+            spline_basis<1>(P, si, [&](int j, float p, float, float) {
+                phi[t][j] = p;
+                ir = fma(P.Ied[j], (double)p, ir);
+            });
+            if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
+                const double err = gauss_err(ir, im);
+                q[t] = (1.0 - err) / (ir * ir);
+                r[t] = err / ir;
+                ft[t] = 0.5 * err * err;
+            } else {
+                q[t] = im / (ir * ir);
+                r[t] = 1.0 - im / ir;
+                ft[t] = nll_term(ir, im);
+            }
+        } else {
+            q[t] = 0.0;
+            r[t] = 0.0;
+            ft[t] = 0.0;
         }
-    } else if (slot < nslot) {
-        const int bj = slot - nhb;
-        const bool last = bj == nb - 1;
-        for (int m = lane; m < kBhcBlock; m += L) {
-            const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
-            const double w = r[m];
-            acc[0] = fma((double)a.x, w, acc[0]);
-            acc[1] = fma((double)a.y, w, acc[1]);
-            acc[2] = fma((double)a.z, w, acc[2]);
-            acc[3] = fma((double)a.w, w, acc[3]);
-            if (last) acc[16] += ft[m];
+        __syncthreads();
+        if (slot < nhb) {
+            for (int m = lane; m < kBhcBlock; m += L) {
+                const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bi]);
+                const float4 b = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+                const double w = q[m];
+                const double ad[4] = {(double)a.x * w, (double)a.y * w, (double)a.z * w,
+                                      (double)a.w * w};
+                const double bd[4] = {(double)b.x, (double)b.y, (double)b.z, (double)b.w};
+#pragma unroll
+                for (int x = 0; x < 4; ++x)
+#pragma unroll
+                    for (int y = 0; y < 4; ++y) acc[4 * x + y] = fma(ad[x], bd[y], acc[4 * x + y]);
+            }
+        } else if (slot < nslot) {
+            for (int m = lane; m < kBhcBlock; m += L) {
+                const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+                const double w = r[m];
+                acc[0] = fma((double)a.x, w, acc[0]);
+                acc[1] = fma((double)a.y, w, acc[1]);
+                acc[2] = fma((double)a.z, w, acc[2]);
+                acc[3] = fma((double)a.w, w, acc[3]);
+                if (last) acc[16] += ft[m];
+            }
         }
+        __syncthreads();  // every basis value of the tile read
     }
-    __syncthreads();  // every basis value read: the area now takes the lane sums
 #pragma unroll
     for (int v = 0; v < 17; ++v) lanesum[t][v] = acc[v];
     __syncthreads();
@@ -365,8 +380,8 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
             int j = 0, rem = v;  // row j of the upper triangle holds E-j entries
             while (rem >= E - j) { rem -= E - j; ++j; }
             const int k = j + rem;
-            const int bi = j >> 2, bj = k >> 2;
-            sl = bi * nb - bi * (bi - 1) / 2 + (bj - bi);
+            const int ri = j >> 2, rj = k >> 2;
+            sl = ri * nb - ri * (ri - 1) / 2 + (rj - ri);
             idx = 4 * (j & 3) + (k & 3);
         } else if (v < nh + E) {
             const int j = v - nh;
@@ -479,9 +494,9 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                        long long count, double* partials, hipStream_t st)
 {
-    const long long blocks = bhc_blocks(count);
-    if (blocks <= 0) return;
-    hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s, Imea,
+    const long long groups = bhc_ie_groups(count);
+    if (groups <= 0) return;
+    hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)groups), dim3(kBhcBlock), 0, st, P, s, Imea,
                        count, sscale, partials);
 }
 

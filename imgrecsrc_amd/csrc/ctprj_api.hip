@@ -15,6 +15,7 @@
 #include <string>
 #include <vector>
 
+#include "bhc_internal.h"
 #include "ctprj_internal.h"
 
 namespace ctprj {
@@ -653,6 +654,7 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
         for (int a = 0; a < geoms[s].np; ++a) fbpi.push_back(make_int2(s, a));
     p->n_fbp_items = (int)fbpi.size();
     std::vector<float> zeros((size_t)max_W + 1, 0.0f);
+    p->red_cap = bhc_red_doubles(sino);
     if ((rc = upload(&p->d_fbp_items, fbpi, "fbp work list")) ||
         (rc = upload(&p->d_fbp_kern, kern, "fbp taps")) ||
         (rc = upload(&p->d_fbp_kern_off, kern_off, "fbp tap offsets")) ||
@@ -661,6 +663,12 @@ static int plan_new(const ctprj_geom* geoms, int nslices, int flags, Plan** out)
         (rc = upload(&p->d_zero_row, zeros, "zero row")) ||
         (rc = dev_alloc((void**)&p->d_imgT, sizeof(float) * (size_t)img, "transposed image")) ||
         (rc = dev_alloc((void**)&p->d_sino_ws, sizeof(float) * (size_t)sino, "sinogram workspace")) ||
+        // the beam-hardening passes' reduction partials and outputs: their calls
+        // allocate nothing (and can be captured into a graph)
+        (rc = dev_alloc((void**)&p->d_red, sizeof(double) * (size_t)p->red_cap,
+                        "reduction workspace")) ||
+        (rc = dev_alloc((void**)&p->d_red_out, sizeof(double) * (size_t)bhc_ie_nvals(CTPRJ_MAX_E),
+                        "reduction output")) ||
         (rc = p->build_geometry(p->tune.flags, *p)) || (rc = p->build_fwd_items(p->tune, *p)) ||
         (rc = p->build_adj_items(p->tune, *p))) {
         delete p;
@@ -901,20 +909,35 @@ extern "C" int ctprj_debug_fail_alloc(int k)
     return 0;
 }
 
+// Cache key of a plan: the device arch, then its slices' geometries in order, a run of
+// k > 1 equal consecutive slices written once as "k*" + geometry.
 static std::string tune_key(const ctprj_plan* p, const char* arch)
 {
     std::ostringstream os;
     os << arch;
-    for (const ctprj_geom& g : p->geoms)
-        os << "|" << g.n << "," << g.prjWidth << "," << g.np << "," << g.prjFull << ","
-           << g.dSize << "," << g.effectiveRate << "," << g.d;
+    const auto& gs = p->geoms;
+    auto same = [](const ctprj_geom& a, const ctprj_geom& b) {
+        return a.n == b.n && a.prjWidth == b.prjWidth && a.np == b.np && a.prjFull == b.prjFull &&
+               a.dSize == b.dSize && a.effectiveRate == b.effectiveRate && a.d == b.d;
+    };
+    for (size_t i = 0; i < gs.size();) {
+        size_t k = 1;
+        while (i + k < gs.size() && same(gs[i], gs[i + k])) ++k;
+        const ctprj_geom& g = gs[i];
+        os << "|";
+        if (k > 1) os << k << "*";
+        os << g.n << "," << g.prjWidth << "," << g.np << "," << g.prjFull << "," << g.dSize << ","
+           << g.effectiveRate << "," << g.d;
+        i += k;
+    }
     return os.str();
 }
 
 // Times the kernel shapes on this plan's geometry: general forward threads per group,
 // symmetric forward waves and work-groups per ray run, general adjoint tile edge,
-// symmetric adjoint waves and work-groups per tile and bin window width (flag bit 4), and
-// (several slices) the work order of tune flag bit 3; keeps the symmetry flags (bits 0-2).
+// symmetric adjoint waves and work-groups per tile for each bin window width (flag bit 4)
+// and gather (LDS staging, bit 5), and (several slices) the work order of tune flag bit
+// 3; keeps the symmetry flags (bits 0-2).
 extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_path, void* stream)
 {
     if (!p) return fail(CTPRJ_EINVAL, "plan is NULL");
@@ -985,7 +1008,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
     // than one work-group per item only while the items are too few to fill the device
     // several times over
     auto search_split = [&](int ctprj_tune::*seg, int ctprj_tune::*split, const int* segs,
-                            int ns, bool fwd, int nitems) -> int {
+                            int ns, bool fwd, int nitems, float* out_ms = nullptr) -> int {
         float best_ms = 1e30f;
         const int splits[7] = {1, 2, 3, 4, 5, 6, 8};
         const int nsplits = nitems < 8192 ? 7 : 1;
@@ -1008,6 +1031,7 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
                     best.*split = k;
                 }
             }
+        if (out_ms) *out_ms = best_ms;
         return ctprj_plan_set_tune(p, &best);
     };
     const int blocks[3] = {64, 128, 256}, tiles[2] = {8, 16};
@@ -1035,17 +1059,26 @@ extern "C" int ctprj_plan_autotune(ctprj_plan* p, int reps, const char* cache_pa
                           p->n_symf_items);
     if (!rc && p->n_adj_items) rc = search(&ctprj_tune::adj_tile, tiles, 2, false);
     if (!rc && p->n_syma_items) {
-        // symmetric adjoint gathers: per lane from L1, or staged per wave in LDS (bit 5)
-        const int stg[2] = {best.flags & ~32, best.flags | 32};
-        rc = search(&ctprj_tune::flags, stg, 2, false);
-    }
-    if (!rc && p->n_syma_items)
-        rc = search_split(&ctprj_tune::adj_seg, &ctprj_tune::adj_split, asegs, 4, false,
-                          p->n_syma_items);
-    if (!rc && p->n_syma_items) {
-        // symmetric adjoint bin windows: 4 wide, or 3 wide (flag bit 4)
-        const int wins[2] = {best.flags & ~16, best.flags | 16};
-        rc = search(&ctprj_tune::flags, wins, 2, false);
+        // symmetric adjoint: bin windows 4 or 3 wide (flag bit 4) and gathers per lane
+        // from L1 or staged per wave in LDS (bit 5), each variant with its best waves and
+        // work-groups per tile (the variants differ in registers, so in the shape they
+        // want)
+        const ctprj_tune start = best;
+        ctprj_tune winner = best;
+        float win_ms = 1e30f;
+        for (int v = 0; v < 4 && !rc; ++v) {
+            best = start;
+            best.flags = (start.flags & ~48) | ((v & 1) ? 16 : 0) | ((v & 2) ? 32 : 0);
+            float ms = 1e30f;
+            rc = search_split(&ctprj_tune::adj_seg, &ctprj_tune::adj_split, asegs, 4, false,
+                              p->n_syma_items, &ms);
+            if (!rc && ms < win_ms) {
+                win_ms = ms;
+                winner = best;
+            }
+        }
+        best = winner;
+        if (!rc) rc = ctprj_plan_set_tune(p, &best);
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

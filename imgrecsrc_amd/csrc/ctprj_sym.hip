@@ -576,7 +576,9 @@ constexpr int kStageBins = 32;
 
 struct AdjStage {
     float4 ray[2 * kStageBins];  // fan beam: bin k's record at ray[2k], ray[2k+1]
-    float val[kStageBins][8];    // bin k's value in orbit image h's row
+    // bin k's value in orbit image h's row at val[k][h]; the 48-byte stride spreads a
+    // staging store's lanes (bins 4 q + j of rows h) over the LDS banks
+    float val[kStageBins][12];
 };
 
 // Wave minimum / maximum of an int: prefix reductions inside each row of 16 lanes
@@ -620,9 +622,9 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
                                                  __amdgpu_buffer_rsrc_t RS,
                                                  __amdgpu_buffer_rsrc_t RZ,
                                                  __amdgpu_buffer_rsrc_t RR, int rrow,
-                                                 const int* __restrict__ tab, float cT, float sT,
-                                                 int x0, int y0, bool valid, float* vpart,
-                                                 AdjStage* __restrict__ st)
+                                                 const int* __restrict__ tab, int tl, float cT,
+                                                 float sT, int x0, int y0, bool valid,
+                                                 float* vpart, AdjStage* __restrict__ st)
 {
     constexpr int K2 = E2 & 3, M2 = E2 >> 2;
     int px, py;
@@ -645,28 +647,40 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
         return;
     }
     const int lane = (int)threadIdx.x & 63;
+    // lane 8 ql + hl fetches bins L0 + 4 ql .. L0 + 4 ql + 3 of orbit image hl's row with
+    // one 16-byte load, so one wave load brings the eight rows' kStageBins bins.  Reflected
+    // rows read the reversed block at 2pC - b; its offsets may fall outside the row (into
+    // a neighbouring row, or below the array, which reads 0) only for bins beyond the
+    // detector or the pixels' candidates, whose weights are exact zeros.  Rows of views
+    // beyond np, and images outside HMASK, give 0.
+    const int hl = lane & 7, ql = lane >> 3;
     wave_lds_sync();  // the previous element's reads are done
-    // bins L0 + lane (lane < kStageBins) of each orbit image's row; reflected rows read
-    // bin 2pC - b, whose offset may fall below the row (a negative offset is out of the
-    // buffer and reads 0) -- such bins lie outside the detector and carry no weight
-    if (lane < kStageBins) {
-        const int b = L0 + lane, two_pc = 2 * S.pC;
+    // lane hl's row entry tab[hl] = entry 8 E2 + hl of the view, which lane 8 E2 + hl
+    // holds in tl (images outside the group have kRowNone entries)
+    const int t = __builtin_amdgcn_ds_bpermute((8 * E2 + hl) << 2, tl);
+    // the row's reflection from its entry (equal to elem_flip, checked when the table is
+    // built): a per-view value keeps the store addresses below from being hoisted out of
+    // the view loop as eight per-element sets
+    const bool fl = (t & kRowFlip) != 0;
+    const int b0 = L0 + 4 * ql;
+    float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (!(t & kRowNone)) {
+        const int voff = (t & ~(kRowFlip | kRowNone)) + 4 * (fl ? 2 * S.pC - b0 - 3 : b0);
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(RS, voff, 0, 0);
 #pragma unroll
-        for (int h = 0; h < 8; ++h) {
-            if (!(HMASK >> h & 1u)) continue;
-            const int t = tab[h];
-            const bool none = (t & kRowNone) != 0;
-            const int soff = none ? 0 : (t & ~(kRowFlip | kRowNone));
-            const int bb = elem_flip(h, E2, FOLD) ? two_pc - b : b;
-            st->val[lane][h] = ldb(none ? RZ : RS, bb * 4, soff);
-        }
+        for (int i = 0; i < 4; ++i) x[i] = __uint_as_float(v[i]);
     }
+    float4 rv;
     if (FAN) {  // ray records of bins L0 .. L0 + kStageBins - 1: 16 bytes per lane
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(RR, L0 * (int)sizeof(Ray) + lane * 16,
                                                              rrow, 0);
-        st->ray[lane] = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
-                                    __uint_as_float(v[2]), __uint_as_float(v[3]));
+        rv = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                         __uint_as_float(v[3]));
     }
+    // x[j] is bin b0 + j, or b0 + 3 - j of a reflected row
+#pragma unroll
+    for (int j = 0; j < 4; ++j) st->val[4 * ql + (fl ? 3 - j : j)][hl] = x[j];
+    if (FAN) st->ray[lane] = rv;
     wave_lds_sync();  // staged values visible to every lane
     if (!valid || blo > bhi) return;
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
@@ -719,12 +733,15 @@ __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* _
 {
     constexpr GroupMasks G = group_masks(GRP);
     constexpr unsigned GSET = FOLD ? G.fold : G.all;
+    // the view's 64 row entries, entry l in lane l (the staged gathers pick theirs with a
+    // lane permute)
+    const int tl = STAGE ? tv[threadIdx.x & 63] : 0;
 #define CTPRJ_E(E2)                                                                             \
     if constexpr ((GSET >> E2) & 1u) {                                                           \
         if (STAGE)                                                                               \
             adj_sym_elem_lds<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,         \
-                                                             tv + 8 * E2, cT, sT, x0, y0, valid, \
-                                                             vpart, st);                         \
+                                                             tv + 8 * E2, tl, cT, sT, x0, y0,    \
+                                                             valid, vpart, st);                  \
         else if (valid)                                                                          \
             adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,          \
                                                             tv + 8 * E2, cT, sT, x0, y0, vpart); \
@@ -896,9 +913,10 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 // i = w, w + SEG, ...) and then share the final sums; with nsplit > 1, nsplit
 // work-groups share the views of a tile and leave their sums in part (512 floats per
 // work-group) for k_adjoint_sym_combine.  STAGE: the gathers off the mirror views go
-// through each wave's LDS area (adj_sym_elem_lds).
+// through each wave's LDS area (adj_sym_elem_lds).  At least four waves per SIMD: the
+// register budget (128) the compiler would otherwise exceed for the staged D4 variants.
 template <bool FBP, int SEG, int GRP, int WIN, bool STAGE>
-__global__ __launch_bounds__(64 * SEG) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
+__global__ __launch_bounds__(64 * SEG) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
                                                           const Ray* __restrict__ rays,

@@ -84,9 +84,12 @@ int prj_get_geom(ctprj_geom* out);
 /* Plan for `nslices` independent slices (ragged sizes and mixed parallel/fan allowed).
  * Images of all slices are packed back to back in one float array, likewise sinograms;
  * see ctprj_plan_layout.  Allocates the plan's device tables and workspace on the
- * current HIP device.  The calls of one plan share its workspace (the transposed image,
- * the filtered sinogram, partial sums): calls on one plan that overlap on different
- * streams race; use one stream per plan, or one plan per stream. */
+ * current HIP device, including the FP64 reduction partials of the beam-hardening calls
+ * (ctprj_bhc_grad_alpha, _alpha_rays, _grad_ie, ctprj_ie_solve), which therefore never
+ * allocate and can be captured in a HIP graph from their first call.  The calls of one
+ * plan share its workspace (the transposed image, the filtered sinogram, partial sums):
+ * calls on one plan that overlap on different streams race; use one stream per plan, or
+ * one plan per stream. */
 int ctprj_plan_create(const ctprj_geom* geoms, int nslices, ctprj_plan** plan);
 int ctprj_plan_destroy(ctprj_plan* plan);
 
@@ -193,10 +196,9 @@ int ctprj_bhc_grad_alpha(ctprj_plan* plan, const float* d_alpha, const float* d_
 /* ctprj_bhc_grad_alpha for callers that cannot allocate device memory (code under HIP
  * graph capture, serving loops, harnesses that forbid hipMalloc).  No counterpart in the
  * reference.  Same arguments, results and launches as ctprj_bhc_grad_alpha, but the FP64
- * reduction partials live in a static device array of the library (512 KiB per device,
- * held until the process ends) instead of the plan's lazily allocated workspace, so the
- * call allocates nothing, also on its first call; the plan's own buffers (created with
- * the plan) serve as before.  At most 16,777,216 rays (np*prjWidth summed over the
+ * reduction partials live in a static device array of the library (512 KiB per device, held until the process ends) instead of the plan's
+ * reduction workspace, so the call allocates nothing and leaves the plan's workspace
+ * untouched; the plan's other buffers (created with the plan) serve as before.  At most 16,777,216 rays (np*prjWidth summed over the
  * plan's slices) per call: larger plans get CTPRJ_EINVAL.  Every call on one device
  * shares that array, so calls that overlap on one device (other streams or threads)
  * race and must be serialised by the caller. */
