@@ -640,7 +640,7 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
     const int L0 = wave_min_i32(blo);
     const int Lmax = wave_max_i32(bhi);
     if (L0 > Lmax) return;  // no lane has a candidate (wave-uniform)
-    if (Lmax + WIN - 1 - L0 >= kStageBins) {  // wave-uniform: too wide to stage
+    if (Lmax - L0 >= kStageBins) {  // wave-uniform: too wide to stage
         if (valid)
             adj_sym_elem_ct<FAN, FBP, HMASK, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow, tab, cT,
                                                             sT, x0, y0, vpart);
@@ -686,37 +686,35 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
     const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
-    for (int s0 = blo; s0 <= bhi; s0 += WIN) {
-        const int k0 = s0 - L0;
-#pragma unroll
-        for (int j = 0; j < WIN; ++j) {
-            const int b = s0 + j, k = k0 + j;
-            Ray r;
-            if (FAN) {
-                const float4 a = st->ray[2 * k], c = st->ray[2 * k + 1];
-                r.cosR = a.x;
-                r.sinR = a.y;
-                r.dc = a.z;
-                r.hbc = a.w;
-                r.h = c.x;
-                r.b1 = c.y;
-                r.s = c.z;
-                r.k = c.w;
-            } else {
-                r = rview[0];
-                r.dc = (float)(b - S.pC) * S.dSize;
-            }
-            const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
-            float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
-            if (FBP && FAN) w *= fw;
-            w = b <= bhi ? w : 0.0f;  // bins past the pixel's candidates add an exact zero
-            const float4 lo = *reinterpret_cast<const float4*>(&st->val[k][0]);
-            const float4 hi = *reinterpret_cast<const float4*>(&st->val[k][4]);
-            const float q[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-#pragma unroll
-            for (int h = 0; h < 8; ++h)
-                if (HMASK >> h & 1u) vpart[h] = fmaf(w, q[h], vpart[h]);
+    // exactly the pixel's candidate bins, in increasing order: the per-lane gathers add
+    // the same products plus exact zeros (bins of a window past bhi), which change no sum
+    // (a partial sum that starts at +0 never becomes -0)
+    for (int b = blo; b <= bhi; ++b) {
+        const int k = b - L0;
+        Ray r;
+        if (FAN) {
+            const float4 a = st->ray[2 * k], c = st->ray[2 * k + 1];
+            r.cosR = a.x;
+            r.sinR = a.y;
+            r.dc = a.z;
+            r.hbc = a.w;
+            r.h = c.x;
+            r.b1 = c.y;
+            r.s = c.z;
+            r.k = c.w;
+        } else {
+            r = rview[0];
+            r.dc = (float)(b - S.pC) * S.dSize;
         }
+        const float dist = ray_dist(fx, fy, r.cosR, r.sinR, r.dc);
+        float w = strip_weight(dist, FAN ? r.hbc * qd : r.hbc, r);
+        if (FBP && FAN) w *= fw;
+        const float4 lo = *reinterpret_cast<const float4*>(&st->val[k][0]);
+        const float4 hi = *reinterpret_cast<const float4*>(&st->val[k][4]);
+        const float q[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+        for (int h = 0; h < 8; ++h)
+            if (HMASK >> h & 1u) vpart[h] = fmaf(w, q[h], vpart[h]);
     }
 }
 
@@ -913,10 +911,13 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 // i = w, w + SEG, ...) and then share the final sums; with nsplit > 1, nsplit
 // work-groups share the views of a tile and leave their sums in part (512 floats per
 // work-group) for k_adjoint_sym_combine.  STAGE: the gathers off the mirror views go
-// through each wave's LDS area (adj_sym_elem_lds).  At least four waves per SIMD: the
-// register budget (128) the compiler would otherwise exceed for the staged D4 variants.
+// through each wave's LDS area (adj_sym_elem_lds).  The staged 4-bin-window variants ask
+// for at least four waves per SIMD: a register budget of 128 that the compiler would
+// otherwise exceed for D4 (145 VGPRs, three waves); the others keep the compiler's own
+// schedule (the same request made the staged 3-bin D4 variant slower at 127 VGPRs).
 template <bool FBP, int SEG, int GRP, int WIN, bool STAGE>
-__global__ __launch_bounds__(64 * SEG) __attribute__((amdgpu_waves_per_eu(4))) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
+__global__ __launch_bounds__(64 * SEG)
+__attribute__((amdgpu_waves_per_eu((STAGE && WIN == 4) ? 4 : 1))) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
                                                           const Ray* __restrict__ rays,
