@@ -124,6 +124,18 @@ extern "C" int ctprj_bhc_grad_alpha(ctprj_plan* p, const float* d_alpha, const f
     hipStream_t st = (hipStream_t)stream;
     const long long rays = p->sino_elems, blocks = bhc_blocks(rays);
     if ((rc = check_red(p, blocks))) return rc;
+    // all slices on the symmetric kernels and no Hessian weights: the ray pass runs in the
+    // forward projector's final store (one NLL partial per forward work item)
+    const long long fused = d_hw ? -1 : p->bhc_fused_parts();
+    if (fused >= 0 && fused <= p->red_cap) {
+        if ((rc = p->launch_forward_bhc(d_alpha, p->d_sino_ws, BhcStore{P, d_Imea, p->d_red, Ts, 0},
+                                        st)))
+            return rc;
+        launch_reduce_partials(p->d_red, fused, 1, d_f, st);
+        if ((rc = last_launch("bhc reduce launch"))) return rc;
+        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false);
+        return 0;
+    }
     // s = A alpha  (Ts applied inside the ray pass)
     if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
     float* w = d_grad ? p->d_sino_ws : nullptr;
@@ -207,8 +219,19 @@ extern "C" int ctprj_bhc_grad_alpha_noalloc(ctprj_plan* p, const float* d_alpha,
     double* part = nullptr;
     if ((rc = noalloc_scratch(&part))) return rc;
     hipStream_t st = (hipStream_t)stream;
-    // the same launches as ctprj_bhc_grad_alpha; every partial is written before the
-    // reduction reads it, so nothing needs clearing
+    // the same launches as ctprj_bhc_grad_alpha (with the fused store when its partials fit
+    // the static array, as they fit the plan's workspace there); every partial is written
+    // before the reduction reads it, so nothing needs clearing
+    const long long fused = d_hw ? -1 : p->bhc_fused_parts();
+    if (fused >= 0 && fused <= kNoallocParts && fused <= p->red_cap) {
+        if ((rc = p->launch_forward_bhc(d_alpha, p->d_sino_ws, BhcStore{P, d_Imea, part, Ts, 0},
+                                        st)))
+            return rc;
+        launch_reduce_partials(part, fused, 1, d_f, st);
+        if ((rc = last_launch("bhc reduce launch"))) return rc;
+        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false);
+        return 0;
+    }
     if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
     float* w = d_grad ? p->d_sino_ws : nullptr;
     launch_bhc_alpha_ray(P, p->d_sino_ws, Ts, d_Imea, rays, Ts, w, d_hw, part, st);

@@ -22,6 +22,17 @@ struct BhcParams {
     double Ied[CTPRJ_MAX_E];        // basis coefficients (FP64 I-epsilon sums)
 };
 
+// The alpha pass fused into the symmetric forward projector's final store: each sinogram
+// entry is stored as its weight w (ray value Ts times the ray sum) and its FP64 NLL term
+// goes into the storing work-group's partial, partials[part_off + work-group].
+struct BhcStore {
+    BhcParams P;
+    const float* Imea;  // measured intensities, plan sinogram layout
+    double* partials;
+    float Ts;
+    int part_off;
+};
+
 // Validates the knots of `basis` (E+2, or E+1 for b0, increasing) and fills P.
 int make_bhc_params(const double* knots, int E, const double* Ie, int basis, int noise,
                     BhcParams* P);

@@ -724,7 +724,31 @@ int Plan::launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool
         if (symf_count[g])
             launch_forward_sym(g, tune.fwd_seg, tune.fwd_split, d_split, symf_count[g], d_desc,
                                d_symf_items + symf_start[g], d_views, d_rays, d_img, d_imgT,
-                               d_sino, st);
+                               d_sino, nullptr, st);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
+}
+
+long long Plan::bhc_fused_parts() const
+{
+    if (n_fwd_items) return -1;
+    long long n = 0;
+    for (int g = 1; g < 4; ++g) n += symf_count[g];
+    return n;
+}
+
+int Plan::launch_forward_bhc(const float* d_img, float* d_w, BhcStore bhc, hipStream_t st)
+{
+    if (n_fwd_items) return fail(CTPRJ_EINVAL, "internal: fused store needs symmetric slices");
+    launch_transpose(n_tr_items, d_desc, d_tr_items, d_img, d_imgT, st);
+    bhc.part_off = 0;
+    for (int g = 1; g < 4; ++g)
+        if (symf_count[g]) {
+            launch_forward_sym(g, tune.fwd_seg, tune.fwd_split, d_split, symf_count[g], d_desc,
+                               d_symf_items + symf_start[g], d_views, d_rays, d_img, d_imgT,
+                               d_w, &bhc, st);
+            bhc.part_off += symf_count[g];
+        }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
 }

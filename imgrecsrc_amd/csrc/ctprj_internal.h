@@ -11,6 +11,8 @@
 
 namespace ctprj {
 
+struct BhcStore;  // bhc_internal.h
+
 struct FwdItem {
     int slice, view, bin0, pad;
 };
@@ -93,6 +95,12 @@ struct Plan : Tables {
     // partial-sum floats the split symmetric launches of tune t need on tables G
     size_t split_need(const ctprj_tune& t, const Tables& fwd, const Tables& adj) const;
     int launch_forward(const float* d_img, float* d_sino, hipStream_t st, bool count);
+    // Number of NLL partials of launch_forward_bhc (the symmetric forward items), or -1
+    // when the plan has slices on the general kernels (no fused store there).
+    long long bhc_fused_parts() const;
+    // Forward projection with the alpha pass fused into the store (bhc->partials gets
+    // bhc_fused_parts() values from offset 0; bhc->part_off is set here).
+    int launch_forward_bhc(const float* d_img, float* d_w, BhcStore bhc, hipStream_t st);
     int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp);
     int launch_fbp(const float* d_sino, float* d_img, hipStream_t st);
 };
@@ -116,11 +124,12 @@ void launch_fbp_filter(int nitems, size_t shmem, const SliceDesc* desc, const in
 void launch_scale(float* p, long long n, float s, hipStream_t st);
 // symmetric kernels (ctprj_sym.hip), one launch per symmetry group; nsplit > 1 work-groups
 // per item leave partial sums in part (nitems * nsplit * 512 floats), added by a second
-// launch
+// launch.  bhc != NULL: the beam-hardening alpha pass fused into the store (sino receives
+// w; one NLL partial per item at bhc->partials[bhc->part_off + item])
 void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
                         const SliceDesc* desc, const FwdItem* items, const float4* views,
                         const Ray* rays, const float* img, const float* imgT, float* sino,
-                        hipStream_t st);
+                        const BhcStore* bhc, hipStream_t st);
 void launch_adjoint_sym(bool fbp, int grp, int seg, int win, bool stage, int nsplit, float* part,
                         int nitems, const SliceDesc* desc, const AdjItem* items,
                         const float4* views, const Ray* rays, const float* sino,

@@ -18,6 +18,7 @@
 // into [0, 180) and folds, the forward takes half the detector with all the images.
 // |dist| in the weight makes W(a0, b, p) == W(a0, 2pC - b, R^2 p) exactly.
 #include "ctprj_internal.h"
+#include "bhc_device.h"
 
 namespace ctprj {
 
@@ -228,9 +229,37 @@ __device__ __forceinline__ void fwd_sym_rows(const SliceDesc& S, const Ray& r, f
     }
 }
 
-// The sum t of image e of canonical ray (it, lane) written to its view g_e a0.
-__device__ __forceinline__ void fwd_sym_store(const SliceDesc& S, const FwdItem& it, int a0, int e,
-                                              int lane, float t, float* __restrict__ sino)
+// Kernel argument of the forward launches: empty, or (BHC) the beam-hardening ray pass
+// fused into the final store (ctprj_bhc_grad_alpha): every sinogram entry is stored
+// once, as w of bhc_ray at Ts times its ray sum, and its FP64 NLL term goes into the
+// storing work-group's partial at partials[part_off + work-group].
+template <bool BHC>
+struct FwdBhc {};
+template <>
+struct FwdBhc<true> : BhcStore {};
+
+// Sum of v over the work-group (a multiple of 64 threads, at most 16 waves): a butterfly
+// over each wave's lanes, then the waves' sums in wave order; every thread must call it.
+__device__ __forceinline__ double group_sum(double v)
+{
+    __shared__ double sh[16];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int wave = (int)threadIdx.x >> 6, nw = (int)blockDim.x >> 6;
+    __syncthreads();  // an earlier call's reads are done
+    if ((threadIdx.x & 63) == 0) sh[wave] = v;
+    __syncthreads();
+    double r = sh[0];
+    for (int w = 1; w < nw; ++w) r += sh[w];
+    return r;
+}
+
+// The sum t of image e of canonical ray (it, lane) written to its view g_e a0; with BHC
+// the entry's w instead, and the NLL term is returned (0 where nothing is stored).
+template <bool BHC>
+__device__ __forceinline__ double fwd_sym_store(const SliceDesc& S, const FwdItem& it, int a0,
+                                                int e, int lane, float t, float* __restrict__ sino,
+                                                const FwdBhc<BHC>& B)
 {
     const int b = it.bin0 + lane;
     const int bmax = S.fold ? S.pC : S.bhi;
@@ -238,25 +267,44 @@ __device__ __forceinline__ void fwd_sym_store(const SliceDesc& S, const FwdItem&
     float* out = sino + S.sino_off;
     int sg, off;
     const int v = map_view(S, e & 3, e >> 2, a0, sg, off);
-    if (v >= S.np) return;
+    if (v >= S.np) return 0.0;
     // folded scans: images e and e ^ 2 (R^2 apart) meet at the centre bin, whose two sums
-    // are equal up to rounding; one writer (k < 2) keeps the result deterministic
-    if (S.fold && (e & 2) && b == S.pC) return;
-    if (b < S.blo) out[(long long)v * S.W] = 0.0f;  // even detector: bin 0 carries no ray
-    else if (active) out[(long long)v * S.W + off + sg * b] = t;
+    // are equal up to rounding, and both reach bin 0 of an even detector; one writer
+    // (k < 2) keeps the result deterministic and counts each entry once
+    if (S.fold && (e & 2) && (b == S.pC || b < S.blo)) return 0.0;
+    long long idx;
+    float val = t;
+    if (b < S.blo) {  // even detector: bin 0 carries no ray
+        idx = (long long)v * S.W;
+        val = 0.0f;
+    } else if (active) {
+        idx = (long long)v * S.W + off + sg * b;
+    } else {
+        return 0.0;
+    }
+    if constexpr (!BHC) {
+        out[idx] = val;
+        return 0.0;
+    } else {
+        float w, hw;
+        const double f = bhc_ray<false>(B.P, B.Ts * val, B.Imea[S.sino_off + idx], B.Ts, &w, &hw);
+        out[idx] = w;
+        return f;
+    }
 }
 
 // One canonical ray run over the images in MASK (the group, or its rotations on a
 // mirror view).  The image rows are cut into nsplit * SEG segments: this work-group
 // walks segments split * SEG + wave.  nsplit == 1 writes the sinogram; otherwise the
 // work-group's sums go to part[e * 64 + lane] for k_forward_sym_combine.
-template <bool FAN, unsigned MASK, bool COLS, int SEG>
+template <bool FAN, unsigned MASK, bool COLS, int SEG, bool BHC>
 __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& it, float cT,
                                             float sT, int a0, const Ray* __restrict__ rays,
                                             const float* __restrict__ img,
                                             const float* __restrict__ imgT,
                                             float* __restrict__ sino, float (*red)[8][64],
-                                            int split, int nsplit, float* __restrict__ part)
+                                            int split, int nsplit, float* __restrict__ part,
+                                            float (*tb)[64])
 {
     // the wave index is uniform over the wave: the row loop then runs on scalar registers
     const int lane = (int)threadIdx.x & 63;
@@ -283,7 +331,8 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
         __syncthreads();
     }
     // wave w finalises the images e = w (mod SEG): the sum of the waves' partials in wave
-    // order
+    // order.  BHC: the sums wait in tb[e][lane] for the kernel's one fused store loop (a
+    // single inlined copy of the spectrum sums)
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         if (!(MASK >> e & 1u) || (SEG > 1 && e % SEG != seg)) continue;
@@ -294,26 +343,28 @@ __device__ __forceinline__ void fwd_sym_run(const SliceDesc& S, const FwdItem& i
             for (int q = 1; q < SEG; ++q) t += red[q][e][lane];
         }
         if (part) part[e * 64 + lane] = t;
-        else fwd_sym_store(S, it, a0, e, lane, t, sino);
+        else if (BHC) tb[e][lane] = t;
+        else fwd_sym_store<false>(S, it, a0, e, lane, t, sino, FwdBhc<false>{});
     }
 }
 
-template <bool FAN, int SEG, int GRP>
+template <bool FAN, int SEG, int GRP, bool BHC>
 __device__ __forceinline__ void fwd_sym_group(const SliceDesc& S, const FwdItem& it, float cT,
                                               float sT, int a0, bool mirror,
                                               const Ray* __restrict__ rays,
                                               const float* __restrict__ img,
                                               const float* __restrict__ imgT,
                                               float* __restrict__ sino, float (*red)[8][64],
-                                              int split, int nsplit, float* __restrict__ part)
+                                              int split, int nsplit, float* __restrict__ part,
+                                              float (*tb)[64])
 {
     // D4's canonical views lie in [0, 45] degrees and always walk rows; V4's reach 90 and
     // Z2's 180 degrees, and the steep ones walk columns
     constexpr GroupMasks GM = group_masks(GRP);
     const bool cols = GRP != kGrpD4 && fabsf(cT) < fabsf(sT);
 #define CTPRJ_RUN(MASK, COLS)                                                                 \
-    fwd_sym_run<FAN, MASK, COLS, SEG>(S, it, cT, sT, a0, rays, img, imgT, sino, red, split, \
-                                      nsplit, part)
+    fwd_sym_run<FAN, MASK, COLS, SEG, BHC>(S, it, cT, sT, a0, rays, img, imgT, sino, red, split, \
+                                           nsplit, part, tb)
     if (GRP != kGrpD4 && cols) {
         if (mirror) CTPRJ_RUN(GM.rot, GRP != kGrpD4);
         else CTPRJ_RUN(GM.all, GRP != kGrpD4);
@@ -337,7 +388,7 @@ __device__ __forceinline__ void split_item(int nitems, int& split, int& idx)
 // waves share the 64 rays, each walking 1/SEG of the work-group's image rows, and then
 // share the final sums; with nsplit > 1, nsplit work-groups share the rows of an item
 // and leave their sums in part (512 floats per work-group) for k_forward_sym_combine.
-template <int SEG, int GRP>
+template <int SEG, int GRP, bool BHC>
 __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __restrict__ slices,
                                                           const FwdItem* __restrict__ items,
                                                           const float4* __restrict__ views,
@@ -345,13 +396,18 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
                                                           const float* __restrict__ img,
                                                           const float* __restrict__ imgT,
                                                           float* __restrict__ sino, int nitems,
-                                                          int nsplit, float* __restrict__ part)
+                                                          int nsplit, float* __restrict__ part,
+                                                          const FwdBhc<BHC> B)
 {
     __shared__ float red[SEG][8][64];
     int split, idx;
     split_item(nitems, split, idx);
     const FwdItem it = items[idx];
-    if (it.slice < 0) return;  // uniform over the work-group
+    if (it.slice < 0) {  // uniform over the work-group: a padding item
+        if constexpr (BHC)
+            if (!part && threadIdx.x == 0) B.partials[B.part_off + blockIdx.x] = 0.0;
+        return;
+    }
     const SliceDesc S = slices[it.slice];
     const float4 T = views[S.view_off + it.view];
     const int a0 = __float_as_int(T.z);
@@ -359,37 +415,63 @@ __global__ __launch_bounds__(64 * SEG) void k_forward_sym(const SliceDesc* __res
     float* mine = part ? part + (long long)blockIdx.x * 512 : nullptr;
     // (fan, mirror, walk) are uniform over the work-group: the barrier inside is reached
     // by all of its waves
+    __shared__ float tb[BHC ? 8 : 1][64];  // BHC: each lane's final sums, by image
     if (S.fan)
-        fwd_sym_group<true, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red,
-                                      split, nsplit, mine);
+        fwd_sym_group<true, SEG, GRP, BHC>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino,
+                                           red, split, nsplit, mine, tb);
     else
-        fwd_sym_group<false, SEG, GRP>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino, red,
-                                       split, nsplit, mine);
+        fwd_sym_group<false, SEG, GRP, BHC>(S, it, T.x, T.y, a0, mirror, rays, img, imgT, sino,
+                                            red, split, nsplit, mine, tb);
+    if constexpr (BHC) {
+        if (!mine) {  // uniform: the fused store of this lane's images, then the NLL partial
+            const int lane = (int)threadIdx.x & 63;
+            const int seg = __builtin_amdgcn_readfirstlane((int)threadIdx.x >> 6);
+            const unsigned mask = mirror ? group_masks(GRP).rot : group_masks(GRP).all;
+            double f = 0.0;
+            // each lane reads back only its own entries: no barrier
+#pragma unroll 1
+            for (int e = seg; e < 8; e += SEG)
+                if (mask >> e & 1u) f += fwd_sym_store<true>(S, it, a0, e, lane, tb[e][lane], sino, B);
+            const double tot = group_sum(f);
+            if (threadIdx.x == 0) B.partials[B.part_off + blockIdx.x] = tot;
+        }
+    }
 }
 
 // The nsplit partial sums of each canonical ray and image of an item, added in split
 // order and written like the one-work-group path (512 threads: image e = thread / 64 of
 // ray lane = thread % 64).
-template <int GRP>
+template <int GRP, bool BHC>
 __global__ __launch_bounds__(512) void k_forward_sym_combine(const SliceDesc* __restrict__ slices,
                                                              const FwdItem* __restrict__ items,
                                                              const float4* __restrict__ views,
                                                              int nitems, int nsplit,
                                                              const float* __restrict__ part,
-                                                             float* __restrict__ sino)
+                                                             float* __restrict__ sino,
+                                                             const FwdBhc<BHC> B)
 {
     const FwdItem it = items[blockIdx.x];
-    if (it.slice < 0) return;
+    if (it.slice < 0) {  // uniform: a padding item
+        if constexpr (BHC)
+            if (threadIdx.x == 0) B.partials[B.part_off + blockIdx.x] = 0.0;
+        return;
+    }
     const SliceDesc S = slices[it.slice];
     const int a0 = __float_as_int(views[S.view_off + it.view].z);
     constexpr GroupMasks GM = group_masks(GRP);
     const unsigned mask = (a0 == 0 || GM.q * a0 == S.P) ? GM.rot : GM.all;
     const int e = (int)threadIdx.x >> 6, lane = (int)threadIdx.x & 63;
-    if (!(mask >> e & 1u)) return;
-    const float* p = part + (long long)blockIdx.x * 512 + threadIdx.x;
-    float t = p[0];
-    for (int k = 1; k < nsplit; ++k) t += p[(long long)k * nitems * 512];
-    fwd_sym_store(S, it, a0, e, lane, t, sino);
+    double f = 0.0;
+    if (mask >> e & 1u) {
+        const float* p = part + (long long)blockIdx.x * 512 + threadIdx.x;
+        float t = p[0];
+        for (int k = 1; k < nsplit; ++k) t += p[(long long)k * nitems * 512];
+        f = fwd_sym_store<BHC>(S, it, a0, e, lane, t, sino, B);
+    }
+    if constexpr (BHC) {  // the work-group's NLL partial (every thread takes part)
+        const double tot = group_sum(f);
+        if (threadIdx.x == 0) B.partials[B.part_off + blockIdx.x] = tot;
+    }
 }
 
 // ------------------------------------------------------------------ adjoint ---
@@ -967,17 +1049,17 @@ __global__ __launch_bounds__(512) void k_adjoint_sym_combine(const SliceDesc* __
 
 // ----------------------------------------------------------------- launchers ---
 
-template <int GRP>
+template <int GRP, bool BHC>
 static void launch_forward_sym_g(int seg, int nsplit, float* part, int nitems,
                                  const SliceDesc* desc, const FwdItem* items, const float4* views,
                                  const Ray* rays, const float* img, const float* imgT, float* sino,
-                                 hipStream_t st)
+                                 const FwdBhc<BHC>& B, hipStream_t st)
 {
     float* pp = nsplit > 1 ? part : nullptr;
     const dim3 grid((unsigned)nitems * (unsigned)nsplit);
-#define CTPRJ_LAUNCH(SEG)                                                                      \
-    hipLaunchKernelGGL((k_forward_sym<SEG, GRP>), grid, dim3(64 * SEG), 0, st, desc, items,     \
-                       views, rays, img, imgT, sino, nitems, nsplit, pp)
+#define CTPRJ_LAUNCH(SEG)                                                                       \
+    hipLaunchKernelGGL((k_forward_sym<SEG, GRP, BHC>), grid, dim3(64 * SEG), 0, st, desc, items, \
+                       views, rays, img, imgT, sino, nitems, nsplit, pp, B)
     switch (seg) {
         case 1: CTPRJ_LAUNCH(1); break;
         case 2: CTPRJ_LAUNCH(2); break;
@@ -987,24 +1069,41 @@ static void launch_forward_sym_g(int seg, int nsplit, float* part, int nitems,
     }
 #undef CTPRJ_LAUNCH
     if (pp)
-        hipLaunchKernelGGL((k_forward_sym_combine<GRP>), dim3((unsigned)nitems), dim3(512), 0, st,
-                           desc, items, views, nitems, nsplit, pp, sino);
+        hipLaunchKernelGGL((k_forward_sym_combine<GRP, BHC>), dim3((unsigned)nitems), dim3(512), 0,
+                           st, desc, items, views, nitems, nsplit, pp, sino, B);
+}
+
+template <bool BHC>
+static void launch_forward_sym_b(int grp, int seg, int nsplit, float* part, int nitems,
+                                 const SliceDesc* desc, const FwdItem* items, const float4* views,
+                                 const Ray* rays, const float* img, const float* imgT, float* sino,
+                                 const FwdBhc<BHC>& B, hipStream_t st)
+{
+    if (grp == kGrpD4)
+        launch_forward_sym_g<kGrpD4, BHC>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                          img, imgT, sino, B, st);
+    else if (grp == kGrpV4)
+        launch_forward_sym_g<kGrpV4, BHC>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                          img, imgT, sino, B, st);
+    else
+        launch_forward_sym_g<kGrpZ2, BHC>(seg, nsplit, part, nitems, desc, items, views, rays,
+                                          img, imgT, sino, B, st);
 }
 
 void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
                         const SliceDesc* desc, const FwdItem* items, const float4* views,
                         const Ray* rays, const float* img, const float* imgT, float* sino,
-                        hipStream_t st)
+                        const BhcStore* bhc, hipStream_t st)
 {
-    if (grp == kGrpD4)
-        launch_forward_sym_g<kGrpD4>(seg, nsplit, part, nitems, desc, items, views, rays, img,
-                                     imgT, sino, st);
-    else if (grp == kGrpV4)
-        launch_forward_sym_g<kGrpV4>(seg, nsplit, part, nitems, desc, items, views, rays, img,
-                                     imgT, sino, st);
-    else
-        launch_forward_sym_g<kGrpZ2>(seg, nsplit, part, nitems, desc, items, views, rays, img,
-                                     imgT, sino, st);
+    if (bhc) {
+        FwdBhc<true> B;
+        static_cast<BhcStore&>(B) = *bhc;
+        launch_forward_sym_b<true>(grp, seg, nsplit, part, nitems, desc, items, views, rays, img,
+                                   imgT, sino, B, st);
+    } else {
+        launch_forward_sym_b<false>(grp, seg, nsplit, part, nitems, desc, items, views, rays, img,
+                                    imgT, sino, FwdBhc<false>{}, st);
+    }
 }
 
 template <bool FBP, int GRP, int WIN>

@@ -254,14 +254,17 @@ def test_alpha_objective_variants(problem, oracle_mod, basis, noise):
     fn = gaussLAlpha if noise == "gaussian" else poissLAlpha
     f, g, hw = fn(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], knots, want_hess=True,
                   basis=basis)
+    # without Hessian weights the ray pass runs in the forward projector's store
+    f2, g2 = fn(P["Imea_t"], P["Ie"], P["alpha_t"], proj, P["Ts"], knots, basis=basis)
     cf, Ts = P["cf"], P["Ts"]
     Phi = lambda a: Ts * oracle_mod.forward(cf, a.reshape(GEO[0], GEO[0]))[0].ravel()  # noqa: E731
     Phit = lambda y: Ts * oracle_mod.adjoint(cf, y.reshape(GEO[2], GEO[1]))[0].ravel()  # noqa: E731
     ofn = B.gauss_l_alpha if noise == "gaussian" else B.poiss_l_alpha
     f64, g64, hw64 = ofn(P["Imea"], P["Ie"], P["alpha"].ravel(), Phi, Phit, pol, knots)
-    ef, eg = abs(f.item() - f64) / abs(f64), rel_l2(g.cpu().numpy(), g64)
-    print(f"{noise}/{basis} alpha objective: f rel {ef:.2e}, grad rel L2 {eg:.2e}")
-    assert ef <= 1e-5 and eg <= 1e-5
+    for ff, gg, how in ((f, g, "ray pass"), (f2, g2, "fused store")):
+        ef, eg = abs(ff.item() - f64) / abs(f64), rel_l2(gg.cpu().numpy(), g64)
+        print(f"{noise}/{basis} alpha objective ({how}): f rel {ef:.2e}, grad rel L2 {eg:.2e}")
+        assert ef <= 1e-5 and eg <= 1e-5
     assert rel_l2(hw.cpu().numpy(), hw64) <= 1e-4
 
 
@@ -342,3 +345,44 @@ def test_ie_solve_device(problem, oracle_mod, noise):
             print(f"   L-BFGS-B f {fr:.9e}, x {np.round(xr, 5)}, device x {np.round(x, 5)}")
             assert abs(f_dev - fr) <= 1e-5 * abs(fr)
             assert np.allclose(x, xr, rtol=1e-3, atol=1e-4 * np.abs(xr).max())
+
+
+# (geometry, tune fields): the symmetry groups (D4, V4, Z2), a folded half-turn parallel
+# scan, odd and even detectors, one and several work-groups and waves per forward item
+FUSED_CASES = [
+    ((64, 91, 72, 72, 1.0, 1.0, 250.0), dict(fwd_seg=4, fwd_split=1)),
+    ((64, 91, 72, 72, 1.0, 1.0, 250.0), dict(fwd_seg=2, fwd_split=3)),
+    ((48, 70, 60, 120, 1.0, 1.0, 0.0), dict(fwd_seg=1, fwd_split=1)),
+    ((48, 70, 60, 120, 1.0, 1.0, 0.0), dict(fwd_seg=8, fwd_split=4)),
+    ((50, 73, 70, 70, 0.9, 1.0, 300.0), dict(fwd_seg=2, fwd_split=1)),
+    ((40, 61, 45, 45, 1.0, 1.0, 200.0), dict(fwd_seg=4, fwd_split=2)),
+    ((33, 52, 34, 34, 1.2, 1.0, 160.0), dict(fwd_seg=16, fwd_split=1)),
+]
+
+
+@pytest.mark.parametrize("noise", ["poisson", "gaussian"])
+@pytest.mark.parametrize("case", FUSED_CASES)
+def test_fused_store_matches_ray_pass(problem, case, noise):
+    """The alpha gradient with the ray pass fused into the forward projector's store
+    against the separate ray pass (ctprj_bhc_alpha_rays) on the same plan's projection
+    followed by its adjoint: the gradients bit for bit (the same weights w, the same
+    adjoint), the NLL up to the FP64 summation order."""
+    import torch
+    from imgrecsrc_amd.bhc import alpha_rays, gaussLAlpha, poissLAlpha
+    from imgrecsrc_amd.phantom import ellipse_phantom
+    from imgrecsrc_amd.projector import Projector
+    geo, tune = case
+    proj = Projector([geo])
+    assert proj.modes()[0] > 0, "case meant for the symmetric kernels"
+    proj.set_tune(**tune)
+    rng = np.random.default_rng(geo[0] + geo[2])
+    alpha = torch.from_numpy(0.05 + ellipse_phantom(geo[0], 7).ravel()).cuda()
+    Imea = torch.from_numpy(rng.uniform(0.05, 1.0, proj.sino_elems).astype(np.float32)).cuda()
+    fn = gaussLAlpha if noise == "gaussian" else poissLAlpha
+    Ts = 0.02
+    f2, g2 = fn(Imea, problem["Ie"], alpha, proj, Ts, problem["knots"])
+    f1, w = alpha_rays(proj, proj.forward(alpha), Imea, problem["Ie"], problem["knots"], Ts,
+                       True, noise=noise)
+    g1 = proj.adjoint(w)
+    assert torch.equal(g1, g2), (geo, tune, noise)
+    assert abs(f1.item() - f2.item()) <= 1e-12 * abs(f1.item()), (f1.item(), f2.item())

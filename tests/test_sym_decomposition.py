@@ -8,7 +8,8 @@ transposes.  Keys related by the exact half-turn identity W(a0, b, p) == W(a0, 2
 of a folded parallel scan are identified; no other geometric equivalence is assumed (at
 45 degrees the diagonal mirror is not exact in FP32).  A planted mistake in the model
 (all elements on a mirror view, a missing fold element, a hole in the fundamental
-domain, a dropped dedup, two writers of a folded scan's centre bin) makes these checks
+domain, a dropped dedup, two writers of a folded scan's centre bin or of an even
+detector's empty bin 0) makes these checks
 fail, which the planted-mistake test
 confirms.  The model checks the decomposition's consistency only: that each view map
 is the right geometric one is pinned by the GPU comparison against the FP64 oracle.
@@ -101,12 +102,16 @@ def decompose(n, W, np_, P, fan, flags=1, plant=None):
     writes = defaultdict(int)
     for a0 in cs:
         els = bits(rot if mirror(a0) and plant != "mirror" else allm)
-        for b in range(blo, bmax + 1):
+        # items start at bin 0: b = 0 < blo on an even detector stores the empty bin 0
+        for b in range(0, bmax + 1):
             for e in els:
                 v, sg, off = mv(P, pC, fold, e & 3, e >> 2, a0)
                 if v >= np_: continue
-                # the kernel's single writer of a folded scan's centre bin
-                if fold and (e & 2) and b == pC and plant != "centre":
+                # the kernel's single writer of a folded scan's centre bin and bin 0
+                if fold and (e & 2) and plant != "centre" and (b == pC or (b < blo and plant != "zero")):
+                    continue
+                if b < blo:
+                    writes[(v, 0)] += 1
                     continue
                 writes[(v, off + sg * b)] += 1
                 for p in pixels:
@@ -204,7 +209,7 @@ def test_each_entry_once_with_one_key(case):
     ("mirror", (8, 12, 16, 16, True, 1)), ("mirror", (7, 12, 13, 13, True, 1)),
     ("domain", (9, 14, 18, 18, True, 1)), ("fold", (9, 14, 20, 40, False, 1)),
     ("fold", (8, 13, 9, 18, False, 1)), ("dedup", (7, 12, 13, 13, True, 1)),
-    ("dedup", (9, 14, 18, 18, True, 1)),
+    ("dedup", (9, 14, 18, 18, True, 1)), ("zero", (9, 14, 20, 40, False, 1)),
 ])
 def test_planted_mistakes_are_caught(plant, case):
     try:
