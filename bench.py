@@ -57,6 +57,25 @@ def timed(fn, reps, stream):
     return e0.elapsed_time(e1) / reps
 
 
+def graph_timed(fn, reps, stream):
+    """fn() captured once as a HIP graph (torch.cuda.CUDAGraph) and replayed: average ms
+    per replay on `stream`, without host launch overhead; also the captured call's
+    outputs (valid after each replay)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    cap = torch.cuda.Stream()
+    with torch.cuda.graph(g, stream=cap):
+        out = fn()
+    torch.cuda.synchronize()
+    g.replay()
+    torch.cuda.synchronize()
+    t = timed(g.replay, reps, stream)
+    torch.cuda.synchronize()
+    return t, out, g
+
+
 def traffic_from_profiles(kernel_tag):
     """HBM bytes per launch from profiles/<tag>_pmc.json (FETCH_SIZE and WRITE_SIZE of
     two rocprofv3 --pmc runs, KiB per dispatch, each divided by the counter's reading
@@ -701,6 +720,20 @@ def main():
         bb = torch.empty(PB.img_elems, dtype=torch.float32, device=dev)
         t_pair = timed(lambda: (PB.forward(a_eval, out=bs), PB.adjoint(bs, out=bb)), 20, stream)
         t_fw = timed(lambda: PB.forward(a_eval, out=bs), 20, stream)
+        # the same calls replayed from HIP graphs: device time without the host's launch
+        # overhead (every entry point here allocates nothing, so each call captures)
+        g_ga, ga_out, gga = graph_timed(lambda: poissLAlpha(Imea, Ie, a_eval, PB, Ts, knots), 50,
+                                        stream)
+        g_pair, _, gp = graph_timed(lambda: (PB.forward(a_eval, out=bs), PB.adjoint(bs, out=bb)),
+                                    50, stream)
+        g_gi, _, ggi = graph_timed(lambda: poissLI(Imea, Ie, PB, Ts, knots, alpha=a_eval), 50,
+                                   stream)
+        g_fw, _, gfw = graph_timed(lambda: PB.forward(a_eval, out=bs), 50, stream)
+        f_eager = poissLAlpha(Imea, Ie, a_eval, PB, Ts, knots)[0].item()
+        gga.replay()
+        torch.cuda.synchronize()
+        f_graph = ga_out[0].item()
+        del gga, gp, ggi, gfw
         out["bhc"] = {"workload": "fan 512^2, 360 views x 729 bins, d=2000, E=20",
                       "grad_alpha_evals_per_s": round(1e3 / t_ga, 1),
                       "grad_alpha_ms": round(t_ga, 4),
@@ -709,6 +742,13 @@ def main():
                       "grad_alpha_over_fwd_adj": round(t_ga / t_pair, 3),
                       "grad_ie_over_fwd": round(t_gi / t_fw, 3),
                       "grad_GUPS": round(2 * bnnz / t_ga / 1e6, 1), "nnz": bnnz,
+                      "graph_replay": {"grad_alpha_ms": round(g_ga, 4),
+                                       "fwd_adj_ms": round(g_pair, 4),
+                                       "grad_alpha_over_fwd_adj": round(g_ga / g_pair, 3),
+                                       "grad_ie_ms": round(g_gi, 4), "fwd_ms": round(g_fw, 4),
+                                       "grad_ie_over_fwd": round(g_gi / g_fw, 3),
+                                       "grad_alpha_evals_per_s": round(1e3 / g_ga, 1),
+                                       "f_replay_equals_eager": f_graph == f_eager},
                       "tune": PB.get_tune()}
 
         # ---- I-epsilon step (20 iterations) and the measurement simulator; both block

@@ -131,10 +131,11 @@ extern "C" int ctprj_bhc_grad_alpha(ctprj_plan* p, const float* d_alpha, const f
         if ((rc = p->launch_forward_bhc(d_alpha, p->d_sino_ws, BhcStore{P, d_Imea, p->d_red, Ts, 0},
                                         st)))
             return rc;
-        launch_reduce_partials(p->d_red, fused, 1, d_f, st);
-        if ((rc = last_launch("bhc reduce launch"))) return rc;
-        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false);
-        return 0;
+        // the NLL partials are added by the adjoint's first wave, or (cost only) on their own
+        const RedJob job{p->d_red, fused, d_f};
+        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false, job);
+        launch_wave_sum(job, st);
+        return last_launch("bhc sum launch");
     }
     // s = A alpha  (Ts applied inside the ray pass)
     if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
@@ -227,10 +228,10 @@ extern "C" int ctprj_bhc_grad_alpha_noalloc(ctprj_plan* p, const float* d_alpha,
         if ((rc = p->launch_forward_bhc(d_alpha, p->d_sino_ws, BhcStore{P, d_Imea, part, Ts, 0},
                                         st)))
             return rc;
-        launch_reduce_partials(part, fused, 1, d_f, st);
-        if ((rc = last_launch("bhc reduce launch"))) return rc;
-        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false);
-        return 0;
+        const RedJob job{part, fused, d_f};
+        if (d_grad) return p->launch_adjoint(p->d_sino_ws, d_grad, st, false, job);
+        launch_wave_sum(job, st);
+        return last_launch("bhc sum launch");
     }
     if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
     float* w = d_grad ? p->d_sino_ws : nullptr;

@@ -753,18 +753,25 @@ int Plan::launch_forward_bhc(const float* d_img, float* d_w, BhcStore bhc, hipSt
     return e == hipSuccess ? 0 : hip_fail(e, "forward launch");
 }
 
-int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp)
+int Plan::launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp,
+                         const RedJob& job)
 {
+    if (job.out && !n_syma_items)
+        return fail(CTPRJ_EINVAL, "internal: a side sum needs a symmetric adjoint launch");
     if (n_adj_items)
         launch_adjoint_kernel(tune.adj_tile, fbp, n_adj_items, d_desc, d_adj_items, d_views,
                               d_rays, d_sino, d_img, st);
     if (n_syma_items) {
+        bool first = true;
         for (int g = 1; g < 4; ++g)
-            if (syma_count[g])
+            if (syma_count[g]) {
                 launch_adjoint_sym(fbp, g, tune.adj_seg, (tune.flags & 16) ? 3 : 4,
                                    (tune.flags & 32) != 0, tune.adj_split,
                                    d_split, syma_count[g], d_desc, d_syma_items + syma_start[g],
-                                   d_views, d_rays, d_sino, d_zero_row, d_symtab, d_img, st);
+                                   d_views, d_rays, d_sino, d_zero_row, d_symtab, d_img,
+                                   first ? job : RedJob{}, st);
+                first = false;
+            }
     }
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? 0 : hip_fail(e, "adjoint launch");

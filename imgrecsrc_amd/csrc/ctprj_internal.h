@@ -13,6 +13,14 @@ namespace ctprj {
 
 struct BhcStore;  // bhc_internal.h
 
+// A sum the symmetric adjoint's first wave adds on the side (*out = sum of in[0..n) in a
+// fixed order), so that a reduction of the previous launches costs no launch of its own.
+struct RedJob {
+    const double* in = nullptr;
+    long long n = 0;
+    double* out = nullptr;
+};
+
 struct FwdItem {
     int slice, view, bin0, pad;
 };
@@ -101,7 +109,11 @@ struct Plan : Tables {
     // Forward projection with the alpha pass fused into the store (bhc->partials gets
     // bhc_fused_parts() values from offset 0; bhc->part_off is set here).
     int launch_forward_bhc(const float* d_img, float* d_w, BhcStore bhc, hipStream_t st);
-    int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp);
+    // job: a sum done by the first symmetric launch (none if job.out is NULL; the plan
+    // must then have a symmetric adjoint launch, see has_sym_adjoint)
+    int launch_adjoint(const float* d_sino, float* d_img, hipStream_t st, bool fbp,
+                       const RedJob& job = RedJob{});
+    bool has_sym_adjoint() const { return n_syma_items > 0; }
     int launch_fbp(const float* d_sino, float* d_img, hipStream_t st);
 };
 
@@ -130,10 +142,13 @@ void launch_forward_sym(int grp, int seg, int nsplit, float* part, int nitems,
                         const SliceDesc* desc, const FwdItem* items, const float4* views,
                         const Ray* rays, const float* img, const float* imgT, float* sino,
                         const BhcStore* bhc, hipStream_t st);
+// The same sum as a symmetric adjoint's side job, in a launch of its own.
+void launch_wave_sum(const RedJob& job, hipStream_t st);
 void launch_adjoint_sym(bool fbp, int grp, int seg, int win, bool stage, int nsplit, float* part,
                         int nitems, const SliceDesc* desc, const AdjItem* items,
                         const float4* views, const Ray* rays, const float* sino,
-                        const float* zero_row, const int* symtab, float* img, hipStream_t st);
+                        const float* zero_row, const int* symtab, float* img, const RedJob& job,
+                        hipStream_t st);
 
 // Images held by a wavelet context (rows * cols * batch).
 long long wavelet_elems(const ctprj_wavelet* w);

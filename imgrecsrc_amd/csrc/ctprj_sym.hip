@@ -94,6 +94,25 @@ __device__ __forceinline__ void pixel_bins(const SliceDesc& S, const Ray* rays_v
     bhi = min(bhi, S.bhi);
 }
 
+// The sum of a RedJob by one wave: lane l adds in[l], in[l + 64], ... in order, then a
+// butterfly over the lanes; the order is fixed, so every caller gets the same value.
+__device__ __forceinline__ void wave_sum_job(const RedJob& job)
+{
+    const int lane = (int)threadIdx.x & 63;
+    double acc = 0.0;
+    for (long long q = lane; q < job.n; q += 64) acc += job.in[q];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) *job.out = acc;
+}
+
+__global__ __launch_bounds__(64) void k_wave_sum(const RedJob job) { wave_sum_job(job); }
+
+void launch_wave_sum(const RedJob& job, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wave_sum, dim3(1), dim3(64), 0, st, job);
+}
+
 // ------------------------------------------------------------------ forward ---
 
 // Rows are summed in plain FP32 over blocks of kRowBlock rows, and the block sums are
@@ -1007,10 +1026,12 @@ __attribute__((amdgpu_waves_per_eu((STAGE && WIN == 4) ? 4 : 1))) void k_adjoint
                                                           const float* __restrict__ zero_row,
                                                           const int* __restrict__ symtab,
                                                           float* __restrict__ img, int nitems,
-                                                          int nsplit, float* __restrict__ part)
+                                                          int nsplit, float* __restrict__ part,
+                                                          const RedJob job)
 {
     __shared__ float red[SEG][8][64];
     __shared__ AdjStage stage[STAGE ? SEG : 1];
+    if (job.out && blockIdx.x == 0 && threadIdx.x < 64) wave_sum_job(job);  // on the side
     int split, idx;
     split_item(nitems, split, idx);
     const AdjItem it = items[idx];
@@ -1110,13 +1131,14 @@ template <bool FBP, int GRP, int WIN>
 static void launch_adjoint_sym_t(int seg, bool stage, int nsplit, float* part, int nitems,
                                  const SliceDesc* desc, const AdjItem* items, const float4* views,
                                  const Ray* rays, const float* sino, const float* zero_row,
-                                 const int* symtab, float* img, hipStream_t st)
+                                 const int* symtab, float* img, const RedJob& job, hipStream_t st)
 {
     float* pp = nsplit > 1 ? part : nullptr;
     const dim3 grid((unsigned)nitems * (unsigned)nsplit);
 #define CTPRJ_LAUNCH(SEG, STAGE)                                                                 \
     hipLaunchKernelGGL((k_adjoint_sym<FBP, SEG, GRP, WIN, STAGE>), grid, dim3(64 * SEG), 0, st,  \
-                       desc, items, views, rays, sino, zero_row, symtab, img, nitems, nsplit, pp)
+                       desc, items, views, rays, sino, zero_row, symtab, img, nitems, nsplit, pp, \
+                       job)
     if (stage) {
         switch (seg) {
             case 1: CTPRJ_LAUNCH(1, true); break;
@@ -1142,36 +1164,37 @@ template <bool FBP, int WIN>
 static void launch_adjoint_sym_f(int grp, int seg, bool stage, int nsplit, float* part, int nitems,
                                  const SliceDesc* desc, const AdjItem* items, const float4* views,
                                  const Ray* rays, const float* sino, const float* zero_row,
-                                 const int* symtab, float* img, hipStream_t st)
+                                 const int* symtab, float* img, const RedJob& job, hipStream_t st)
 {
     if (grp == kGrpD4)
         launch_adjoint_sym_t<FBP, kGrpD4, WIN>(seg, stage, nsplit, part, nitems, desc, items,
-                                               views, rays, sino, zero_row, symtab, img, st);
+                                               views, rays, sino, zero_row, symtab, img, job, st);
     else if (grp == kGrpV4)
         launch_adjoint_sym_t<FBP, kGrpV4, WIN>(seg, stage, nsplit, part, nitems, desc, items,
-                                               views, rays, sino, zero_row, symtab, img, st);
+                                               views, rays, sino, zero_row, symtab, img, job, st);
     else
         launch_adjoint_sym_t<FBP, kGrpZ2, WIN>(seg, stage, nsplit, part, nitems, desc, items,
-                                               views, rays, sino, zero_row, symtab, img, st);
+                                               views, rays, sino, zero_row, symtab, img, job, st);
 }
 
 void launch_adjoint_sym(bool fbp, int grp, int seg, int win, bool stage, int nsplit, float* part,
                         int nitems, const SliceDesc* desc, const AdjItem* items,
                         const float4* views, const Ray* rays, const float* sino,
-                        const float* zero_row, const int* symtab, float* img, hipStream_t st)
+                        const float* zero_row, const int* symtab, float* img, const RedJob& job,
+                        hipStream_t st)
 {
     if (fbp && win == 3)
         launch_adjoint_sym_f<true, 3>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
-                                      rays, sino, zero_row, symtab, img, st);
+                                      rays, sino, zero_row, symtab, img, job, st);
     else if (fbp)
         launch_adjoint_sym_f<true, 4>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
-                                      rays, sino, zero_row, symtab, img, st);
+                                      rays, sino, zero_row, symtab, img, job, st);
     else if (win == 3)
         launch_adjoint_sym_f<false, 3>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
-                                       rays, sino, zero_row, symtab, img, st);
+                                       rays, sino, zero_row, symtab, img, job, st);
     else
         launch_adjoint_sym_f<false, 4>(grp, seg, stage, nsplit, part, nitems, desc, items, views,
-                                       rays, sino, zero_row, symtab, img, st);
+                                       rays, sino, zero_row, symtab, img, job, st);
 }
 
 

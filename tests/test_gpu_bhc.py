@@ -366,7 +366,8 @@ def test_fused_store_matches_ray_pass(problem, case, noise):
     """The alpha gradient with the ray pass fused into the forward projector's store
     against the separate ray pass (ctprj_bhc_alpha_rays) on the same plan's projection
     followed by its adjoint: the gradients bit for bit (the same weights w, the same
-    adjoint), the NLL up to the FP64 summation order."""
+    adjoint), the NLL up to the FP64 summation order (the fused path's NLL partials are
+    summed by the adjoint's first wave)."""
     import torch
     from imgrecsrc_amd.bhc import alpha_rays, gaussLAlpha, poissLAlpha
     from imgrecsrc_amd.phantom import ellipse_phantom
@@ -386,3 +387,7 @@ def test_fused_store_matches_ray_pass(problem, case, noise):
     g1 = proj.adjoint(w)
     assert torch.equal(g1, g2), (geo, tune, noise)
     assert abs(f1.item() - f2.item()) <= 1e-12 * abs(f1.item()), (f1.item(), f2.item())
+    # cost only (no adjoint, the NLL partials summed by a launch of their own): the same
+    # value bit for bit
+    f3 = fn(Imea, problem["Ie"], alpha, proj, Ts, problem["knots"], want_grad=False)
+    assert f3.item() == f2.item()
