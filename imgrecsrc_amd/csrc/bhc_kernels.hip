@@ -90,39 +90,46 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 //   Poisson  (bhc/poissLI.m):  q = Imea/Ir^2,       r = 1 - Imea/Ir,  f_i = nll(Ir, Imea)
 //   Gaussian (bhc/gaussLI.m):  q = (1 - Err)/Ir^2,  r = Err/Ir,       f_i = Err^2/2.
 // Register blocking: the outputs are cut into 4x4 blocks of H (upper block triangle)
-// and 4-entry blocks of g (the last one also carries f); thread t takes block
-// slot = t % S over the rays lane, lane + L, ... of every tile with lane = t / S (S =
-// slots rounded up to a power of two, L = kBhcBlock / S), so each LDS read of 4 basis
-// values feeds 4 or 16 FP64 FMAs.  The L lane sums of every output are then added in
-// lane order.  The grid is capped (bhc_ie_groups), so the partials take at most
-// kIeMaxGroups * nvals doubles whatever the ray count.
-constexpr int kIeStride = ((CTPRJ_MAX_E + 3) / 4) * 4;  // phi row: a multiple of 4 floats
-
+// and 4-entry blocks of g (the last one also carries f); the S slots (a power of two at
+// least the number of blocks) take L = kBhcBlock / S consecutive threads each, thread
+// lane l of a slot summing the rays l, l + L, ... of every tile, so each LDS read of 4
+// basis values feeds 4 or 16 FP64 FMAs.  At the end the L lanes of a slot add their sums
+// by a butterfly inside the wave (a fixed order) and lane 0 writes them.  The basis rows
+// are E values padded to an odd number of 16-byte units (the L lanes of a slot read L
+// consecutive rows at the same column without bank conflicts), in dynamic LDS sized for
+// E: at E = 20, 30 KiB per group, so five groups fit a CU.  The grid is capped
+// (bhc_ie_groups), so the partials take at most kIeMaxGroups * nvals doubles whatever the
+// ray count.
 __host__ __device__ inline int ie_blocks(int E) { return (E + 3) / 4; }
 __host__ __device__ inline int ie_slots(int E)
 {
     const int nb = ie_blocks(E);
     return nb * (nb + 1) / 2 + nb;
 }
-
-__global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s,
-                                                          const float* __restrict__ Imea,
-                                                          long long count, float sscale,
-                                                          double* __restrict__ partials)
+// floats per LDS basis row: 4 per 4-block, an odd number of 16-byte units
+__host__ __device__ inline int ie_row(int E) { return 4 * (ie_blocks(E) | 1); }
+__host__ __device__ inline size_t ie_lds_bytes(int E)
 {
-    // the basis values, then (after the last tile) the lane sums share one LDS area
-    constexpr size_t kPhiBytes = sizeof(float) * kBhcBlock * kIeStride;
-    constexpr size_t kSumBytes = sizeof(double) * kBhcBlock * 17;
-    __shared__ __attribute__((aligned(16))) char area[kPhiBytes > kSumBytes ? kPhiBytes : kSumBytes];
-    __shared__ double q[kBhcBlock], r[kBhcBlock], ft[kBhcBlock];
-    auto phi = reinterpret_cast<float(*)[kIeStride]>(area);
-    auto lanesum = reinterpret_cast<double(*)[17]>(area);  // [thread][16 block values + f]
+    return sizeof(float) * kBhcBlock * (size_t)ie_row(E) + 3 * sizeof(double) * kBhcBlock;
+}
+
+// Five waves per SIMD (90 VGPRs instead of 104): five groups per CU, so the 1026 tiles of
+// a 512^2 fan sinogram run in one round instead of two.
+__global__ __launch_bounds__(kBhcBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restrict__ Imea,
+                  long long count, float sscale, double* __restrict__ partials)
+{
+    extern __shared__ __attribute__((aligned(16))) char ie_lds[];
+    const int E = P.E, row = ie_row(E);
+    float* phi = reinterpret_cast<float*>(ie_lds);  // [kBhcBlock][row]
+    double* q = reinterpret_cast<double*>(ie_lds + sizeof(float) * kBhcBlock * row);
+    double* r = q + kBhcBlock;
+    double* ft = r + kBhcBlock;
     const int t = threadIdx.x;
-    const int E = P.E;
     const int nb = ie_blocks(E), nhb = nb * (nb + 1) / 2, nslot = nhb + nb;
-    int S = 1;
+    int S = 4;  // at most 64 lanes per slot: a slot's lanes stay inside one wave
     while (S < nslot) S <<= 1;  // nslot <= 44 for E <= 32, so S <= 64
-    const int L = kBhcBlock / S, slot = t % S, lane = t / S;
+    const int L = kBhcBlock / S, slot = t / L, lane = t % L;
     int bi = 0, bj = 0;
     if (slot < nhb) {
         int rem = slot;  // block row bi of the upper triangle holds nb-bi blocks
@@ -138,13 +145,14 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
     const long long ntiles = (count + kBhcBlock - 1) / kBhcBlock;
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long i = tile * kBhcBlock + t;
-        for (int j = 0; j < kIeStride; ++j) phi[t][j] = 0.0f;
+        float* my = phi + t * row;
+        for (int j = 0; j < row; ++j) my[j] = 0.0f;
         if (i < count) {
             const float si = sscale * s[i];
             const double im = (double)Imea[i];
             double ir = 0.0;
             spline_basis<1>(P, si, [&](int j, float p, float, float) {
-                phi[t][j] = p;
+                my[j] = p;
                 ir = fma(P.Ied[j], (double)p, ir);
             });
             if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
@@ -165,8 +173,8 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
         __syncthreads();
         if (slot < nhb) {
             for (int m = lane; m < kBhcBlock; m += L) {
-                const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bi]);
-                const float4 b = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+                const float4 a = *reinterpret_cast<const float4*>(phi + m * row + 4 * bi);
+                const float4 b = *reinterpret_cast<const float4*>(phi + m * row + 4 * bj);
                 const double w = q[m];
                 const double ad[4] = {(double)a.x * w, (double)a.y * w, (double)a.z * w,
                                       (double)a.w * w};
@@ -178,7 +186,7 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
             }
         } else if (slot < nslot) {
             for (int m = lane; m < kBhcBlock; m += L) {
-                const float4 a = *reinterpret_cast<const float4*>(&phi[m][4 * bj]);
+                const float4 a = *reinterpret_cast<const float4*>(phi + m * row + 4 * bj);
                 const double w = r[m];
                 acc[0] = fma((double)a.x, w, acc[0]);
                 acc[1] = fma((double)a.y, w, acc[1]);
@@ -189,31 +197,26 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_ray(BhcParams P, const flo
         }
         __syncthreads();  // every basis value of the tile read
     }
+    // the slot's L lane sums: a butterfly over its consecutive lanes (inside one wave)
+    for (int off = L >> 1; off > 0; off >>= 1)
 #pragma unroll
-    for (int v = 0; v < 17; ++v) lanesum[t][v] = acc[v];
-    __syncthreads();
+        for (int v = 0; v < 17; ++v) acc[v] += __shfl_xor(acc[v], off, 64);
+    if (lane != 0 || slot >= nslot) return;
     const int nh = E * (E + 1) / 2, nv = nh + E + 1;
     double* out = partials + (long long)blockIdx.x * nv;
-    for (int v = t; v < nv; v += kBhcBlock) {
-        int sl, idx;
-        if (v < nh) {
-            int j = 0, rem = v;  // row j of the upper triangle holds E-j entries
-            while (rem >= E - j) { rem -= E - j; ++j; }
-            const int k = j + rem;
-            const int ri = j >> 2, rj = k >> 2;
-            sl = ri * nb - ri * (ri - 1) / 2 + (rj - ri);
-            idx = 4 * (j & 3) + (k & 3);
-        } else if (v < nh + E) {
-            const int j = v - nh;
-            sl = nhb + (j >> 2);
-            idx = j & 3;
-        } else {
-            sl = nslot - 1;
-            idx = 16;
-        }
-        double tot = 0.0;
-        for (int l = 0; l < L; ++l) tot += lanesum[l * S + sl][idx];
-        out[v] = tot;
+    if (slot < nhb) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int j = 4 * bi + x, k = 4 * bj + y;  // H_jk, stored for j <= k < E
+                if (j <= k && k < E) out[j * E - j * (j - 1) / 2 + (k - j)] = acc[4 * x + y];
+            }
+    } else {
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+            if (4 * bj + x < E) out[nh + 4 * bj + x] = acc[x];
+        if (last) out[nv - 1] = acc[16];
     }
 }
 
@@ -316,8 +319,8 @@ void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const f
 {
     const long long groups = bhc_ie_groups(count);
     if (groups <= 0) return;
-    hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)groups), dim3(kBhcBlock), 0, st, P, s, Imea,
-                       count, sscale, partials);
+    hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)groups), dim3(kBhcBlock), ie_lds_bytes(P.E), st,
+                       P, s, Imea, count, sscale, partials);
 }
 
 void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
