@@ -887,6 +887,38 @@ def main():
             t_p = timed(plain, 3, stream)
             t_g = timed(plain_general, 3, stream)
             groups = PBat.modes()
+            # per symmetry group: the group's slices in one plan against one plan per slice
+            # (the groups differ in how often a weight is reused: D4 8, V4 4, Z2 2 times)
+            per_group = {}
+            for gid, gname in ((1, "D4"), (2, "V4"), (3, "Z2"), (0, "general")):
+                idx = [k for k, m in enumerate(groups) if m == gid]
+                if not idx:
+                    continue
+                Pg = Projector([geoms[k] for k in idx])
+                Pg.autotune(reps=3, cache_path=TUNE_CACHE)
+                gimg = Pg.pack_images([imgs[k] for k in idx])
+                gs = torch.empty(Pg.sino_elems, dtype=torch.float32, device=dev)
+                gb = torch.empty(Pg.img_elems, dtype=torch.float32, device=dev)
+
+                def gbat():
+                    Pg.forward(gimg, out=gs)
+                    Pg.adjoint(gs, out=gb)
+
+                def gplain():
+                    for k in idx:
+                        singles[k].forward(imgs[k], out=s_out[k])
+                        singles[k].adjoint(s_out[k], out=i_out[k])
+
+                gbat()
+                gplain()
+                tb_g, tp_g = timed(gbat, 3, stream), timed(gplain, 3, stream)
+                gnnz = Pg.nnz()
+                per_group[gname] = {"slices": len(idx), "batched_ms": round(tb_g, 3),
+                                    "plain_ms": round(tp_g, 3),
+                                    "batched_GUPS": round(2 * gnnz / tb_g / 1e6, 1),
+                                    "plain_GUPS": round(2 * gnnz / tp_g / 1e6, 1),
+                                    "speedup": round(tp_g / tb_g, 3)}
+                del Pg, gimg, gs, gb
             out["batched"] = {"workload": "64 ragged slices, n in {128..1024}, views 90..720, "
                                           "mixed parallel/fan", "nnz": tot_nnz,
                               "batched_ms": round(t_b, 3), "plain_ms": round(t_p, 3),
@@ -896,7 +928,8 @@ def main():
                               "plain_general_ms": round(t_g, 3),
                               "plain_general_GUPS": round(2 * tot_nnz / t_g / 1e6, 1),
                               "speedup_vs_general": round(t_g / t_b, 3),
-                              "slices_per_group": {str(k): groups.count(k) for k in range(4)}}
+                              "slices_per_group": {str(k): groups.count(k) for k in range(4)},
+                              "per_group": per_group}
 
     if not args.no_extras and "pcie" in legs:
         # ---- the reference-compatible boundary: host arrays through prj_run, PCIe included

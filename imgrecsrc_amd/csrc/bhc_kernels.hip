@@ -146,7 +146,8 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
     for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const long long i = tile * kBhcBlock + t;
         float* my = phi + t * row;
-        for (int j = 0; j < row; ++j) my[j] = 0.0f;
+        // spline_basis writes columns 0..E-1 of a ray's row; the padding columns feed only
+        // block entries beyond E, which are never stored, so they are left as they are
         if (i < count) {
             const float si = sscale * s[i];
             const double im = (double)Imea[i];
@@ -165,7 +166,8 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
                 r[t] = 1.0 - im / ir;
                 ft[t] = nll_term(ir, im);
             }
-        } else {
+        } else {  // past the last ray: finite zeros (times a zero weight)
+            for (int j = 0; j < row; ++j) my[j] = 0.0f;
             q[t] = 0.0;
             r[t] = 0.0;
             ft[t] = 0.0;
