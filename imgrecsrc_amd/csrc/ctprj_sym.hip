@@ -718,35 +718,15 @@ __device__ __forceinline__ void wave_lds_sync()
 // The gather of adj_sym_elem_ct through the wave's LDS area.  Every lane of the wave
 // takes part (the staging loads are the wave's); lanes with valid == false or an empty
 // candidate range add nothing.
-template <bool FAN, bool FBP, unsigned HMASK, int E2, bool FOLD, int WIN>
-__device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* __restrict__ rview,
-                                                 __amdgpu_buffer_rsrc_t RS,
-                                                 __amdgpu_buffer_rsrc_t RZ,
-                                                 __amdgpu_buffer_rsrc_t RR, int rrow,
-                                                 const int* __restrict__ tab, int tl, float cT,
-                                                 float sT, int x0, int y0, bool valid,
-                                                 float* vpart, AdjStage* __restrict__ st)
+// One chunk of adj_sym_elem_lds: bins [L0, L0 + kStageBins) staged by the wave, then this
+// lane's candidates blo..bhi inside the chunk summed.  Every lane of the wave takes part.
+template <bool FAN, bool FBP, unsigned HMASK>
+__device__ __forceinline__ void adj_sym_chunk(const SliceDesc& S, const Ray* __restrict__ rview,
+                                              __amdgpu_buffer_rsrc_t RS, __amdgpu_buffer_rsrc_t RR,
+                                              int rrow, int e2, int tl, float cT, float sT,
+                                              float fx, float fy, int blo, int bhi, int L0,
+                                              float* vpart, AdjStage* __restrict__ st)
 {
-    constexpr int K2 = E2 & 3, M2 = E2 >> 2;
-    int px, py;
-    if (M2) apply_px(K2, 1, x0, y0, px, py);
-    else apply_px(4 - K2, 0, x0, y0, px, py);
-    const float fx = (float)px, fy = (float)py;
-    int blo, bhi;
-    pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
-    if (!valid || blo > bhi) {
-        blo = INT_MAX;
-        bhi = INT_MIN;
-    }
-    const int L0 = wave_min_i32(blo);
-    const int Lmax = wave_max_i32(bhi);
-    if (L0 > Lmax) return;  // no lane has a candidate (wave-uniform)
-    if (Lmax - L0 >= kStageBins) {  // wave-uniform: too wide to stage
-        if (valid)
-            adj_sym_elem_ct<FAN, FBP, HMASK, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow, tab, cT,
-                                                            sT, x0, y0, vpart);
-        return;
-    }
     const int lane = (int)threadIdx.x & 63;
     // lane 8 ql + hl fetches bins L0 + 4 ql .. L0 + 4 ql + 3 of orbit image hl's row with
     // one 16-byte load, so one wave load brings the eight rows' kStageBins bins.  Reflected
@@ -756,9 +736,9 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
     // beyond np, and images outside HMASK, give 0.
     const int hl = lane & 7, ql = lane >> 3;
     wave_lds_sync();  // the previous element's reads are done
-    // lane hl's row entry tab[hl] = entry 8 E2 + hl of the view, which lane 8 E2 + hl
+    // lane hl's row entry tab[hl] = entry 8 e2 + hl of the view, which lane 8 e2 + hl
     // holds in tl (images outside the group have kRowNone entries)
-    const int t = __builtin_amdgcn_ds_bpermute((8 * E2 + hl) << 2, tl);
+    const int t = __builtin_amdgcn_ds_bpermute((8 * e2 + hl) << 2, tl);
     // the row's reflection from its entry (equal to elem_flip, checked when the table is
     // built): a per-view value keeps the store addresses below from being hoisted out of
     // the view loop as eight per-element sets
@@ -783,7 +763,7 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
     for (int j = 0; j < 4; ++j) st->val[4 * ql + (fl ? 3 - j : j)][hl] = x[j];
     if (FAN) st->ray[lane] = rv;
     wave_lds_sync();  // staged values visible to every lane
-    if (!valid || blo > bhi) return;
+    if (blo > bhi) return;  // no candidate of this lane in the chunk
     const float qd = FAN ? fan_depth_scale(fx, fy, cT, sT, S.invd) : 1.0f;
     // FBP fan weight d^2/qe^2 (prj/cpuPrj.c:258), folded into each weight
     const float fw = (FBP && FAN) ? 1.0f / (qd * qd) : 1.0f;
@@ -819,6 +799,39 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
     }
 }
 
+// The gather of adj_sym_elem_ct through the wave's LDS area, for element g' = e2 (a
+// canonical view off the mirror lines) and the orbit images in HMASK.  Every lane of the
+// wave takes part; lanes with valid == false or no candidate add nothing.  A range wider
+// than kStageBins bins (fine detector pitch) is staged and summed in chunks of kStageBins
+// bins, in increasing order, so the sums are bit-identical to the per-lane gathers'.
+template <bool FAN, bool FBP, unsigned HMASK>
+__device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* __restrict__ rview,
+                                                 __amdgpu_buffer_rsrc_t RS,
+                                                 __amdgpu_buffer_rsrc_t RR, int rrow, int e2,
+                                                 int tl, float cT, float sT, int x0, int y0,
+                                                 bool valid, float* vpart,
+                                                 AdjStage* __restrict__ st)
+{
+    const int k2 = e2 & 3, m2 = e2 >> 2;
+    int px, py;
+    if (m2) apply_px(k2, 1, x0, y0, px, py);
+    else apply_px(4 - k2, 0, x0, y0, px, py);
+    const float fx = (float)px, fy = (float)py;
+    int blo, bhi;
+    pixel_bins<FAN>(S, rview, fx, fy, cT, sT, blo, bhi);
+    if (!valid || blo > bhi) {
+        blo = INT_MAX;
+        bhi = INT_MIN;
+    }
+    const int L0 = wave_min_i32(blo);
+    const int Lmax = wave_max_i32(bhi);
+    // wave-uniform; one chunk unless the range is wider than kStageBins bins
+    for (int c0 = L0; c0 <= Lmax; c0 += kStageBins)
+        adj_sym_chunk<FAN, FBP, HMASK>(S, rview, RS, RR, rrow, e2, tl, cT, sT, fx, fy,
+                                       max(blo, c0), min(bhi, c0 + kStageBins - 1), c0, vpart,
+                                       st);
+}
+
 // All elements of a canonical view off the mirror lines, unrolled at compile time.
 // With STAGE the wave stages each element's bins in LDS (every lane takes part),
 // otherwise the valid lanes gather directly.
@@ -838,9 +851,8 @@ __device__ __forceinline__ void adj_sym_view_ct(const SliceDesc& S, const Ray* _
 #define CTPRJ_E(E2)                                                                             \
     if constexpr ((GSET >> E2) & 1u) {                                                           \
         if (STAGE)                                                                               \
-            adj_sym_elem_lds<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,         \
-                                                             tv + 8 * E2, tl, cT, sT, x0, y0,    \
-                                                             valid, vpart, st);                  \
+            adj_sym_elem_lds<FAN, FBP, G.all>(S, rview, RS, RR, rrow, E2, tl, cT, sT, x0, y0,     \
+                                              valid, vpart, st);                                 \
         else if (valid)                                                                          \
             adj_sym_elem_ct<FAN, FBP, G.all, E2, FOLD, WIN>(S, rview, RS, RZ, RR, rrow,          \
                                                             tv + 8 * E2, cT, sT, x0, y0, vpart); \
@@ -1013,12 +1025,12 @@ __device__ __forceinline__ void adj_sym_tile(const SliceDesc& S, const AdjItem& 
 // work-groups share the views of a tile and leave their sums in part (512 floats per
 // work-group) for k_adjoint_sym_combine.  STAGE: the gathers off the mirror views go
 // through each wave's LDS area (adj_sym_elem_lds).  The staged 4-bin-window variants ask
-// for at least four waves per SIMD: a register budget of 128 that the compiler would
-// otherwise exceed for D4 (145 VGPRs, three waves); the others keep the compiler's own
-// schedule (the same request made the staged 3-bin D4 variant slower at 127 VGPRs).
+// for at least five waves per SIMD (96 VGPRs; D4 would take 99, four waves); the same
+// request spills on the staged 3-bin D4 variants, which keep the compiler's own schedule
+// (the window width matters there only on the mirror views' per-lane gathers).
 template <bool FBP, int SEG, int GRP, int WIN, bool STAGE>
 __global__ __launch_bounds__(64 * SEG)
-__attribute__((amdgpu_waves_per_eu((STAGE && WIN == 4) ? 4 : 1))) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
+__attribute__((amdgpu_waves_per_eu((STAGE && WIN == 4) ? 5 : 1))) void k_adjoint_sym(const SliceDesc* __restrict__ slices,
                                                           const AdjItem* __restrict__ items,
                                                           const float4* __restrict__ views,
                                                           const Ray* __restrict__ rays,
