@@ -886,6 +886,11 @@ def main():
             t_b = timed(batched, 3, stream)
             t_p = timed(plain, 3, stream)
             t_g = timed(plain_general, 3, stream)
+            # both paths captured as HIP graphs (the batched one is a handful of launches,
+            # the plain one 128 pairs of launch sequences) and replayed
+            tg_b, _, gb_ = graph_timed(batched, 5, stream)
+            tg_p, _, gp_ = graph_timed(plain, 5, stream)
+            del gb_, gp_
             groups = PBat.modes()
             # per symmetry group: the group's slices in one plan against one plan per slice
             # (the groups differ in how often a weight is reused: D4 8, V4 4, Z2 2 times)
@@ -928,6 +933,10 @@ def main():
                               "plain_general_ms": round(t_g, 3),
                               "plain_general_GUPS": round(2 * tot_nnz / t_g / 1e6, 1),
                               "speedup_vs_general": round(t_g / t_b, 3),
+                              "graph_replay": {"batched_ms": round(tg_b, 3),
+                                               "plain_ms": round(tg_p, 3),
+                                               "batched_GUPS": round(2 * tot_nnz / tg_b / 1e6, 1),
+                                               "speedup": round(tg_p / tg_b, 3)},
                               "slices_per_group": {str(k): groups.count(k) for k in range(4)},
                               "per_group": per_group}
 
