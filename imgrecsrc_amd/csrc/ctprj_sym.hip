@@ -682,29 +682,26 @@ struct AdjStage {
     float val[kStageBins][12];
 };
 
-// Wave minimum / maximum of an int: prefix reductions inside each row of 16 lanes
-// (DPP row shifts; lanes with no source keep their own value), then the four rows'
-// last lanes.
-__device__ __forceinline__ int wave_min_i32(int v)
+// Wave minimum of lo and maximum of hi: prefix reductions inside each row of 16 lanes
+// (DPP row shifts; lanes with no source read the operation's identity, which lets the
+// compiler fold each shift into the min / max), then the four rows' last lanes.  The
+// two chains are interleaved so that each DPP read finds its operand already written.
+__device__ __forceinline__ void wave_range_i32(int lo, int hi, int& wlo, int& whi)
 {
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
-    v = min(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
-    const int a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
-    const int c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
-    return min(min(a, b), min(c, d));
-}
-
-__device__ __forceinline__ int wave_max_i32(int v)
-{
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x111, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x112, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x114, 0xf, 0xf, false));
-    v = max(v, __builtin_amdgcn_update_dpp(v, v, 0x118, 0xf, 0xf, false));
-    const int a = __builtin_amdgcn_readlane(v, 15), b = __builtin_amdgcn_readlane(v, 31);
-    const int c = __builtin_amdgcn_readlane(v, 47), d = __builtin_amdgcn_readlane(v, 63);
-    return max(max(a, b), max(c, d));
+    lo = min(lo, __builtin_amdgcn_update_dpp(INT_MAX, lo, 0x111, 0xf, 0xf, false));
+    hi = max(hi, __builtin_amdgcn_update_dpp(INT_MIN, hi, 0x111, 0xf, 0xf, false));
+    lo = min(lo, __builtin_amdgcn_update_dpp(INT_MAX, lo, 0x112, 0xf, 0xf, false));
+    hi = max(hi, __builtin_amdgcn_update_dpp(INT_MIN, hi, 0x112, 0xf, 0xf, false));
+    lo = min(lo, __builtin_amdgcn_update_dpp(INT_MAX, lo, 0x114, 0xf, 0xf, false));
+    hi = max(hi, __builtin_amdgcn_update_dpp(INT_MIN, hi, 0x114, 0xf, 0xf, false));
+    lo = min(lo, __builtin_amdgcn_update_dpp(INT_MAX, lo, 0x118, 0xf, 0xf, false));
+    hi = max(hi, __builtin_amdgcn_update_dpp(INT_MIN, hi, 0x118, 0xf, 0xf, false));
+    const int a = __builtin_amdgcn_readlane(lo, 15), b = __builtin_amdgcn_readlane(lo, 31);
+    const int c = __builtin_amdgcn_readlane(lo, 47), d = __builtin_amdgcn_readlane(lo, 63);
+    const int e = __builtin_amdgcn_readlane(hi, 15), f = __builtin_amdgcn_readlane(hi, 31);
+    const int g = __builtin_amdgcn_readlane(hi, 47), h = __builtin_amdgcn_readlane(hi, 63);
+    wlo = min(min(a, b), min(c, d));
+    whi = max(max(e, f), max(g, h));
 }
 
 // Orders this wave's LDS writes before its later reads (and reads before later writes).
@@ -823,8 +820,8 @@ __device__ __forceinline__ void adj_sym_elem_lds(const SliceDesc& S, const Ray* 
         blo = INT_MAX;
         bhi = INT_MIN;
     }
-    const int L0 = wave_min_i32(blo);
-    const int Lmax = wave_max_i32(bhi);
+    int L0, Lmax;
+    wave_range_i32(blo, bhi, L0, Lmax);
     // wave-uniform; one chunk unless the range is wider than kStageBins bins
     for (int c0 = L0; c0 <= Lmax; c0 += kStageBins)
         adj_sym_chunk<FAN, FBP, HMASK>(S, rview, RS, RR, rrow, e2, tl, cT, sT, fx, fy,
