@@ -265,10 +265,8 @@ extern "C" int ctprj_bhc_grad_ie(ctprj_plan* p, const float* d_alpha, const floa
     }
     launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st);
     if ((rc = last_launch("bhc ie ray launch"))) return rc;
-    launch_reduce_partials(p->d_red, groups, nv, p->d_red_out, st);
-    if ((rc = last_launch("bhc ie reduce launch"))) return rc;
-    launch_unpack_ie(p->d_red_out, E, d_f, d_g, d_H, st);
-    return last_launch("bhc ie unpack launch");
+    launch_reduce_ie(p->d_red, groups, E, d_f, d_g, d_H, st);
+    return last_launch("bhc ie reduce launch");
 }
 
 // ------------------------------------------------------------- I-epsilon solve ---

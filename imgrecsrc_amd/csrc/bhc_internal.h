@@ -51,17 +51,19 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
                           long long count, float Ts, float* w, float* hw, double* partials,
                           hipStream_t st);
 // per-ray I-epsilon pass: per work-group partial [H upper, g, f], bhc_ie_groups(count)
-// work-groups
+// = G work-groups, value v of group p at partials[v*G + p]
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                        long long count, double* partials, hipStream_t st);
 // cost only of the I-epsilon pass: one FP64 partial per work-group
 void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
                         long long count, double* partials, hipStream_t st);
-// out[v] = sum_p partials[p*nvals + v] in a fixed order
+// out[v] = sum_p partials[v*nparts + p] in a fixed order, v < nvals
 void launch_reduce_partials(const double* partials, long long nparts, int nvals, double* out,
                             hipStream_t st);
-// unpack [H upper, g, f] -> f, g, H (full)
-void launch_unpack_ie(const double* red, int E, double* f, double* g, double* H, hipStream_t st);
+// the I-epsilon pass's partials [H upper, g, f] reduced as launch_reduce_partials does,
+// straight into f, g and H (full); NULL outputs are skipped
+void launch_reduce_ie(const double* partials, long long nparts, int E, double* f, double* g,
+                      double* H, hipStream_t st);
 
 inline long long bhc_blocks(long long count) { return (count + kBhcBlock - 1) / kBhcBlock; }
 

@@ -82,30 +82,31 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 // ------------------------------------------------------------ I-epsilon pass ---
 
 // A work-group takes tiles of kBhcBlock rays (tile = group, group + G, ...): the basis
-// values phi_ij (FP32) and the ray weights q_i, r_i, f_i (FP64) of a tile go to LDS, and
-// the group adds the tile's contribution to its sums [H upper triangle (row-major j<=k),
-// g, f]:
+// values phi_ij (FP32) and the ray weights q_i, r_i (FP64) of a tile go to LDS, and the
+// group adds the tile's contribution to its sums [H upper triangle (row-major j<=k), g,
+// f]:
 //   H_jk = sum_i phi_ij phi_ik q_i,  g_j = sum_i phi_ij r_i,  f = sum_i f_i,
 //   Ir_i = sum_j Ie_j phi_ij (FP64), with
 //   Poisson  (bhc/poissLI.m):  q = Imea/Ir^2,       r = 1 - Imea/Ir,  f_i = nll(Ir, Imea)
 //   Gaussian (bhc/gaussLI.m):  q = (1 - Err)/Ir^2,  r = Err/Ir,       f_i = Err^2/2.
-// Register blocking: the outputs are cut into 4x4 blocks of H (upper block triangle)
-// and 4-entry blocks of g (the last one also carries f); the S slots (a power of two at
-// least the number of blocks) take L = kBhcBlock / S consecutive threads each, thread
-// lane l of a slot summing the rays l, l + L, ... of every tile, so each LDS read of 4
-// basis values feeds 4 or 16 FP64 FMAs.  At the end the L lanes of a slot add their sums
-// by a butterfly inside the wave (a fixed order) and lane 0 writes them.  The basis rows
-// are E values padded to an odd number of 16-byte units (the L lanes of a slot read L
-// consecutive rows at the same column without bank conflicts), in dynamic LDS sized for
-// E: at E = 20, 30 KiB per group, so five groups fit a CU.  The grid is capped
-// (bhc_ie_groups), so the partials take at most kIeMaxGroups * nvals doubles whatever the
-// ray count.
+// Register blocking: H is cut into 4x4 blocks (upper block triangle); the S slots (a
+// power of two at least the number of blocks, at least 4) take L = kBhcBlock / S
+// consecutive lanes of one wave each, lane l of a slot summing the rays l, l + L, ... of
+// every tile, so each LDS read of 4 + 4 basis values feeds 16 FP64 FMAs.  The slots of
+// the diagonal blocks also sum g over their 4 entries (the other slots run the same
+// instructions and drop those sums, so that no wave splits into two paths), and every
+// thread sums f over its own rays (in LDS).  At E = 20 the 15 blocks take 15 slots of 16 lanes:
+// each of the four waves holds four slots (the last one three) and runs 16 iterations
+// per tile.  At the end the L lanes of a slot add their sums in the order of an XOR
+// butterfly (offsets L/2, ..., 1), the offsets below 16 as DPP row shifts that leave the
+// total in the slot's last lane, which writes it; f is added by block_sum.  The basis
+// rows are E values padded to an odd number of 16-byte units (consecutive rows at one
+// column spread over the LDS banks), in dynamic LDS sized for E: at E = 20, 26 KiB per
+// group.  The grid is capped (bhc_ie_groups), so the partials take at most
+// kIeMaxGroups * nvals doubles whatever the ray count; they are stored value-major (value
+// v of group p at v * G + p), so the reduction reads each value's G partials
+// contiguously.
 __host__ __device__ inline int ie_blocks(int E) { return (E + 3) / 4; }
-__host__ __device__ inline int ie_slots(int E)
-{
-    const int nb = ie_blocks(E);
-    return nb * (nb + 1) / 2 + nb;
-}
 // floats per LDS basis row: 4 per 4-block, an odd number of 16-byte units
 __host__ __device__ inline int ie_row(int E) { return 4 * (ie_blocks(E) | 1); }
 __host__ __device__ inline size_t ie_lds_bytes(int E)
@@ -113,112 +114,163 @@ __host__ __device__ inline size_t ie_lds_bytes(int E)
     return sizeof(float) * kBhcBlock * (size_t)ie_row(E) + 3 * sizeof(double) * kBhcBlock;
 }
 
-// Five waves per SIMD (90 VGPRs instead of 104): five groups per CU, so the 1026 tiles of
-// a 512^2 fan sinogram run in one round instead of two.
-__global__ __launch_bounds__(kBhcBlock) __attribute__((amdgpu_waves_per_eu(5)))
+// a + (a from the lane OFF below, in the same row of 16 lanes), for FP64 a; lanes with
+// no such source add +0 (bound_ctrl; their sums are not used)
+template <int OFF>
+__device__ __forceinline__ double add_from_below(double a)
+{
+    const unsigned long long b = (unsigned long long)__double_as_longlong(a);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)b, 0x110 + OFF, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x110 + OFF, 0xf, 0xf, true);
+    return a + __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                                (unsigned)lo));
+}
+
+// One tile of the pass: the basis rows, q_i, r_i of rays tile * kBhcBlock + t into LDS,
+// f_i added to the thread's LDS sum.
+__device__ __forceinline__ void ie_tile_rows(const BhcParams& P, const float* __restrict__ s,
+                                             const float* __restrict__ Imea, long long count,
+                                             float sscale, long long tile, int row, float* phi,
+                                             double2* qr, double* fs)
+{
+    const int t = threadIdx.x;
+    const long long i = tile * kBhcBlock + t;
+    float* my = phi + t * row;
+    // spline_basis writes columns 0..E-1 of a ray's row; the padding columns feed only
+    // block entries beyond E, which are never stored, so they are left as they are
+    if (i < count) {
+        const float si = sscale * s[i];
+        const double im = (double)Imea[i];
+        double ir = 0.0;
+        spline_basis<1>(P, si, [&](int j, float p, float, float) {
+            my[j] = p;
+            ir = fma(P.Ied[j], (double)p, ir);
+        });
+        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
+            const double err = gauss_err(ir, im);
+            qr[t] = make_double2((1.0 - err) / (ir * ir), err / ir);
+            fs[t] += 0.5 * err * err;
+        } else {
+            qr[t] = make_double2(im / (ir * ir), 1.0 - im / ir);
+            fs[t] += nll_term(ir, im);
+        }
+    } else {  // past the last ray: finite zeros (times a zero weight)
+        for (int j = 0; j < row; ++j) my[j] = 0.0f;
+        qr[t] = make_double2(0.0, 0.0);
+    }
+}
+
+// One tile's contribution to a slot lane's sums: block (bi, bj) of H in acc[0..15], g of
+// block bj in acc[16..19].
+__device__ __forceinline__ void ie_tile_sums(int lane, int L, int row, int bi, int bj,
+                                             const float* phi, const double2* qr, double* acc)
+{
+    for (int m = lane; m < kBhcBlock; m += L) {
+        const float4 a = *reinterpret_cast<const float4*>(phi + m * row + 4 * bi);
+        const float4 b = *reinterpret_cast<const float4*>(phi + m * row + 4 * bj);
+        const double2 w = qr[m];
+        const double ad[4] = {(double)a.x * w.x, (double)a.y * w.x, (double)a.z * w.x,
+                              (double)a.w * w.x};
+        const double bd[4] = {(double)b.x, (double)b.y, (double)b.z, (double)b.w};
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[4 * x + y] = fma(ad[x], bd[y], acc[4 * x + y]);
+        // g of block bj (kept by the diagonal slots, where block bj is block bi)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[16 + y] = fma(bd[y], w.y, acc[16 + y]);
+    }
+}
+
+// ONE: one tile per group (gridDim.x = tiles), so the sums start after the tile's basis
+// rows and are not live across them; then five waves per SIMD fit without spills: five
+// groups per CU, so the 1026 tiles of a 512^2 fan sinogram run in one round instead of
+// two.  With several tiles per group (more than kIeMaxGroups tiles) the grid takes
+// several rounds anyway, and four waves per SIMD leave room for the live sums.
+template <bool ONE>
+__global__ __launch_bounds__(kBhcBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4)))
 void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restrict__ Imea,
                   long long count, float sscale, double* __restrict__ partials)
 {
+    constexpr int kAcc = 20;  // 16 H entries, 4 g entries
     extern __shared__ __attribute__((aligned(16))) char ie_lds[];
     const int E = P.E, row = ie_row(E);
     float* phi = reinterpret_cast<float*>(ie_lds);  // [kBhcBlock][row]
-    double* q = reinterpret_cast<double*>(ie_lds + sizeof(float) * kBhcBlock * row);
-    double* r = q + kBhcBlock;
-    double* ft = r + kBhcBlock;
+    double2* qr = reinterpret_cast<double2*>(ie_lds + sizeof(float) * kBhcBlock * row);
+    double* fs = reinterpret_cast<double*>(qr + kBhcBlock);  // each thread's f sum
     const int t = threadIdx.x;
-    const int nb = ie_blocks(E), nhb = nb * (nb + 1) / 2, nslot = nhb + nb;
+    const int nb = ie_blocks(E), nhb = nb * (nb + 1) / 2;
     int S = 4;  // at most 64 lanes per slot: a slot's lanes stay inside one wave
-    while (S < nslot) S <<= 1;  // nslot <= 44 for E <= 32, so S <= 64
+    while (S < nhb) S <<= 1;  // nhb <= 36 for E <= 32, so S <= 64
     const int L = kBhcBlock / S, slot = t / L, lane = t % L;
     int bi = 0, bj = 0;
     if (slot < nhb) {
         int rem = slot;  // block row bi of the upper triangle holds nb-bi blocks
         while (rem >= nb - bi) { rem -= nb - bi; ++bi; }
         bj = bi + rem;
-    } else {
-        bj = slot - nhb;
     }
-    const bool last = slot >= nhb && bj == nb - 1;
-    double acc[17];
-#pragma unroll
-    for (int v = 0; v < 17; ++v) acc[v] = 0.0;
-    const long long ntiles = (count + kBhcBlock - 1) / kBhcBlock;
-    for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        const long long i = tile * kBhcBlock + t;
-        float* my = phi + t * row;
-        // spline_basis writes columns 0..E-1 of a ray's row; the padding columns feed only
-        // block entries beyond E, which are never stored, so they are left as they are
-        if (i < count) {
-            const float si = sscale * s[i];
-            const double im = (double)Imea[i];
-            double ir = 0.0;
-            spline_basis<1>(P, si, [&](int j, float p, float, float) {
-                my[j] = p;
-                ir = fma(P.Ied[j], (double)p, ir);
-            });
-            if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
-                const double err = gauss_err(ir, im);
-                q[t] = (1.0 - err) / (ir * ir);
-                r[t] = err / ir;
-                ft[t] = 0.5 * err * err;
-            } else {
-                q[t] = im / (ir * ir);
-                r[t] = 1.0 - im / ir;
-                ft[t] = nll_term(ir, im);
-            }
-        } else {  // past the last ray: finite zeros (times a zero weight)
-            for (int j = 0; j < row; ++j) my[j] = 0.0f;
-            q[t] = 0.0;
-            r[t] = 0.0;
-            ft[t] = 0.0;
-        }
+    fs[t] = 0.0;
+    double acc[kAcc];
+    if constexpr (ONE) {
+        ie_tile_rows(P, s, Imea, count, sscale, blockIdx.x, row, phi, qr, fs);
         __syncthreads();
-        if (slot < nhb) {
-            for (int m = lane; m < kBhcBlock; m += L) {
-                const float4 a = *reinterpret_cast<const float4*>(phi + m * row + 4 * bi);
-                const float4 b = *reinterpret_cast<const float4*>(phi + m * row + 4 * bj);
-                const double w = q[m];
-                const double ad[4] = {(double)a.x * w, (double)a.y * w, (double)a.z * w,
-                                      (double)a.w * w};
-                const double bd[4] = {(double)b.x, (double)b.y, (double)b.z, (double)b.w};
 #pragma unroll
-                for (int x = 0; x < 4; ++x)
-#pragma unroll
-                    for (int y = 0; y < 4; ++y) acc[4 * x + y] = fma(ad[x], bd[y], acc[4 * x + y]);
-            }
-        } else if (slot < nslot) {
-            for (int m = lane; m < kBhcBlock; m += L) {
-                const float4 a = *reinterpret_cast<const float4*>(phi + m * row + 4 * bj);
-                const double w = r[m];
-                acc[0] = fma((double)a.x, w, acc[0]);
-                acc[1] = fma((double)a.y, w, acc[1]);
-                acc[2] = fma((double)a.z, w, acc[2]);
-                acc[3] = fma((double)a.w, w, acc[3]);
-                if (last) acc[16] += ft[m];
-            }
-        }
+        for (int v = 0; v < kAcc; ++v) acc[v] = 0.0;
+        if (slot < nhb) ie_tile_sums(lane, L, row, bi, bj, phi, qr, acc);
         __syncthreads();  // every basis value of the tile read
-    }
-    // the slot's L lane sums: a butterfly over its consecutive lanes (inside one wave)
-    for (int off = L >> 1; off > 0; off >>= 1)
-#pragma unroll
-        for (int v = 0; v < 17; ++v) acc[v] += __shfl_xor(acc[v], off, 64);
-    if (lane != 0 || slot >= nslot) return;
-    const int nh = E * (E + 1) / 2, nv = nh + E + 1;
-    double* out = partials + (long long)blockIdx.x * nv;
-    if (slot < nhb) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x)
-#pragma unroll
-            for (int y = 0; y < 4; ++y) {
-                const int j = 4 * bi + x, k = 4 * bj + y;  // H_jk, stored for j <= k < E
-                if (j <= k && k < E) out[j * E - j * (j - 1) / 2 + (k - j)] = acc[4 * x + y];
-            }
     } else {
 #pragma unroll
-        for (int x = 0; x < 4; ++x)
-            if (4 * bj + x < E) out[nh + 4 * bj + x] = acc[x];
-        if (last) out[nv - 1] = acc[16];
+        for (int v = 0; v < kAcc; ++v) acc[v] = 0.0;
+        const long long ntiles = (count + kBhcBlock - 1) / kBhcBlock;
+        for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+            ie_tile_rows(P, s, Imea, count, sscale, tile, row, phi, qr, fs);
+            __syncthreads();
+            if (slot < nhb) ie_tile_sums(lane, L, row, bi, bj, phi, qr, acc);
+            __syncthreads();  // every basis value of the tile read
+        }
+    }
+    // f: every thread's sum over its rays, in a fixed order
+    const double ftot = block_sum(fs[t], reinterpret_cast<double*>(ie_lds));
+    // the slot's L lane sums, in the order of an XOR butterfly over its consecutive
+    // lanes (inside one wave): offsets of 16 and more by shuffles, then row shifts of 8,
+    // 4, 2, 1 towards the last lane.  The lanes a shift level reads have the offset's bit
+    // set, so lane x - off is lane x ^ off, and the last lane ends with the butterfly's
+    // value (the same in every lane of a butterfly).
+    for (int off = L >> 1; off >= 16; off >>= 1)
+#pragma unroll
+        for (int v = 0; v < kAcc; ++v) acc[v] += __shfl_xor(acc[v], off, 64);
+    if (L > 8) {
+#pragma unroll
+        for (int v = 0; v < kAcc; ++v) acc[v] = add_from_below<8>(acc[v]);
+    }
+    if (L > 4) {
+#pragma unroll
+        for (int v = 0; v < kAcc; ++v) acc[v] = add_from_below<4>(acc[v]);
+    }
+    if (L > 2) {
+#pragma unroll
+        for (int v = 0; v < kAcc; ++v) acc[v] = add_from_below<2>(acc[v]);
+    }
+    if (L > 1) {
+#pragma unroll
+        for (int v = 0; v < kAcc; ++v) acc[v] = add_from_below<1>(acc[v]);
+    }
+    const int nh = E * (E + 1) / 2, nv = nh + E + 1;
+    const long long G = gridDim.x;
+    double* out = partials + blockIdx.x;  // value v at out[v * G]
+    if (t == 0) out[(nv - 1) * G] = ftot;
+    if (lane != L - 1 || slot >= nhb) return;
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+            const int j = 4 * bi + x, k = 4 * bj + y;  // H_jk, stored for j <= k < E
+            if (j <= k && k < E) out[(j * E - j * (j - 1) / 2 + (k - j)) * G] = acc[4 * x + y];
+        }
+    if (bi == bj) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y)
+            if (4 * bj + y < E) out[(nh + 4 * bj + y) * G] = acc[16 + y];
     }
 }
 
@@ -252,35 +304,53 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_cost(BhcParams P, const fl
 
 // out[v] = sum over p of partials[p*nvals + v]: one work-group per value, each thread
 // sums a fixed stride of partials in order, then a fixed LDS tree.
+// Sum of value v = blockIdx.x over the nparts partials stored at partials[v * nparts + p]
+// (thread-strided partial sums, then block_sum: a fixed order).
+__device__ __forceinline__ double reduce_value(const double* __restrict__ partials, long long nparts,
+                                               double* sh)
+{
+    const double* col = partials + (long long)blockIdx.x * nparts;
+    double acc = 0.0;
+    for (long long p = threadIdx.x; p < nparts; p += kBhcBlock) acc += col[p];
+    return block_sum(acc, sh);
+}
+
 __global__ __launch_bounds__(kBhcBlock) void k_reduce_partials(const double* __restrict__ partials,
-                                                               long long nparts, int nvals,
+                                                               long long nparts,
                                                                double* __restrict__ out)
 {
     __shared__ double sh[kBhcBlock];
-    const int v = blockIdx.x;
-    double acc = 0.0;
-    for (long long p = threadIdx.x; p < nparts; p += kBhcBlock) acc += partials[p * nvals + v];
-    const double tot = block_sum(acc, sh);
-    if (threadIdx.x == 0) out[v] = tot;
+    const double tot = reduce_value(partials, nparts, sh);
+    if (threadIdx.x == 0) out[blockIdx.x] = tot;
 }
 
-__global__ void k_unpack_ie(const double* __restrict__ red, int E, double* __restrict__ f,
-                            double* __restrict__ g, double* __restrict__ H)
+// The I-epsilon pass's reduction straight into f, g and H (full): value v of
+// [H upper (row-major j <= k), g, f] goes to H[j][k] and H[k][j], g[v - nh] or *f
+// (NULL outputs are skipped).
+__global__ __launch_bounds__(kBhcBlock) void k_reduce_ie(const double* __restrict__ partials,
+                                                         long long nparts, int E,
+                                                         double* __restrict__ f,
+                                                         double* __restrict__ g,
+                                                         double* __restrict__ H)
 {
-    const int nh = E * (E + 1) / 2;
-    for (int v = threadIdx.x; v < nh + E + 1; v += blockDim.x) {
-        if (v < nh) {
-            if (!H) continue;
-            int j = 0, rem = v;
-            while (rem >= E - j) { rem -= E - j; ++j; }
-            const int k = j + rem;
-            H[j * E + k] = red[v];
-            H[k * E + j] = red[v];
-        } else if (v < nh + E) {
-            if (g) g[v - nh] = red[v];
-        } else if (f) {
-            *f = red[v];
+    __shared__ double sh[kBhcBlock];
+    const double tot = reduce_value(partials, nparts, sh);
+    if (threadIdx.x != 0) return;
+    const int v = blockIdx.x, nh = E * (E + 1) / 2;
+    if (v < nh) {
+        if (!H) return;
+        int j = 0, rem = v;
+        while (rem >= E - j) {
+            rem -= E - j;
+            ++j;
         }
+        const int k = j + rem;
+        H[j * E + k] = tot;
+        H[k * E + j] = tot;
+    } else if (v < nh + E) {
+        if (g) g[v - nh] = tot;
+    } else if (f) {
+        *f = tot;
     }
 }
 
@@ -321,8 +391,12 @@ void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const f
 {
     const long long groups = bhc_ie_groups(count);
     if (groups <= 0) return;
-    hipLaunchKernelGGL(k_bhc_ie_ray, dim3((unsigned)groups), dim3(kBhcBlock), ie_lds_bytes(P.E), st,
-                       P, s, Imea, count, sscale, partials);
+    if (groups == bhc_blocks(count))
+        hipLaunchKernelGGL(k_bhc_ie_ray<true>, dim3((unsigned)groups), dim3(kBhcBlock),
+                           ie_lds_bytes(P.E), st, P, s, Imea, count, sscale, partials);
+    else
+        hipLaunchKernelGGL(k_bhc_ie_ray<false>, dim3((unsigned)groups), dim3(kBhcBlock),
+                           ie_lds_bytes(P.E), st, P, s, Imea, count, sscale, partials);
 }
 
 void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
@@ -339,12 +413,15 @@ void launch_reduce_partials(const double* partials, long long nparts, int nvals,
 {
     if (nvals <= 0) return;
     hipLaunchKernelGGL(k_reduce_partials, dim3(nvals), dim3(kBhcBlock), 0, st, partials, nparts,
-                       nvals, out);
+                       out);
 }
 
-void launch_unpack_ie(const double* red, int E, double* f, double* g, double* H, hipStream_t st)
+void launch_reduce_ie(const double* partials, long long nparts, int E, double* f, double* g,
+                      double* H, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_unpack_ie, dim3(1), dim3(256), 0, st, red, E, f, g, H);
+    hipLaunchKernelGGL(k_reduce_ie, dim3(bhc_ie_nvals(E)), dim3(kBhcBlock), 0, st, partials, nparts,
+                       E, f, g, H);
 }
+
 
 }  // namespace ctprj

@@ -391,3 +391,59 @@ def test_fused_store_matches_ray_pass(problem, case, noise):
     # value bit for bit
     f3 = fn(Imea, problem["Ie"], alpha, proj, Ts, problem["knots"], want_grad=False)
     assert f3.item() == f2.item()
+
+
+def _ie_case(rays, E_, seed):
+    """Synthetic ray values s (Ts = 1) and measurements near the model's for E_ bins."""
+    from oracle import bhc_oracle as B
+    rng = np.random.default_rng(seed)
+    knots = B.spline_knots(B.kappa_logspace(E_, 0.03, 30.0))
+    x = np.arange(E_)
+    Ie = np.exp(-0.5 * ((x - 0.4 * E_) / (0.15 * E_)) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    s = rng.uniform(0.0, 3.0, rays).astype(np.float32)
+    A = B.b1_iout_stable(knots, s.astype(np.float64), None)[0]
+    # measurements 5-25 % below the model, so that g = A'(1 - Imea/Ir) has no cancellation
+    Imea = (A @ Ie) * rng.uniform(0.75, 0.95, rays)
+    return knots, Ie, s, Imea, A
+
+
+@pytest.mark.parametrize("E_", [6, 11, 20, 26, 32])
+def test_ie_pass_block_layouts(E_):
+    """The I-epsilon pass at spectrum sizes that give 64, 32, 16, 8 and 4 lanes per
+    H block (E = 6, 11, 20, 26, 32), from given ray values."""
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import poissLI
+    from imgrecsrc_amd.projector import Projector
+    proj = Projector([GEO])
+    knots, Ie, s, Imea, A = _ie_case(proj.sino_elems, E_, 40 + E_)
+    f, g, H = poissLI(torch.from_numpy(Imea.astype(np.float32)).cuda(), Ie, proj, 1.0, knots,
+                      sraw=torch.from_numpy(s).cuda())
+    f64, g64, H64 = B.poiss_l_i(Imea.astype(np.float32).astype(np.float64), A, Ie)
+    ef = abs(f.item() - f64) / abs(f64)
+    eg, eh = rel_l2(g.cpu().numpy(), g64), rel_l2(H.cpu().numpy(), H64)
+    print(f"E={E_}: f rel {ef:.2e}, g rel {eg:.2e}, H rel {eh:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5 and eh <= 1e-5
+    Hn = H.cpu().numpy()
+    assert np.array_equal(Hn, Hn.T)
+
+
+def test_ie_pass_several_tiles_per_group():
+    """More than 4096 tiles of 256 rays (1100 x 1024 rays): the pass's groups take two
+    tiles each."""
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd.bhc import poissLI
+    from imgrecsrc_amd.projector import Projector
+    geo = (32, 1024, 1100, 1100, 1.0, 1.0, 4000.0)
+    proj = Projector([geo])
+    assert proj.sino_elems > 4096 * 256
+    knots, Ie, s, Imea, A = _ie_case(proj.sino_elems, E, 7)
+    f, g, H = poissLI(torch.from_numpy(Imea.astype(np.float32)).cuda(), Ie, proj, 1.0, knots,
+                      sraw=torch.from_numpy(s).cuda())
+    f64, g64, H64 = B.poiss_l_i(Imea.astype(np.float32).astype(np.float64), A, Ie)
+    ef = abs(f.item() - f64) / abs(f64)
+    eg, eh = rel_l2(g.cpu().numpy(), g64), rel_l2(H.cpu().numpy(), H64)
+    print(f"{proj.sino_elems} rays: f rel {ef:.2e}, g rel {eg:.2e}, H rel {eh:.2e}")
+    assert ef <= 1e-5 and eg <= 1e-5 and eh <= 1e-5
