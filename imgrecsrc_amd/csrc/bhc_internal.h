@@ -51,7 +51,8 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
                           long long count, float Ts, float* w, float* hw, double* partials,
                           hipStream_t st);
 // per-ray I-epsilon pass: per work-group partial [H upper, g, f], bhc_ie_groups(count)
-// = G work-groups, value v of group p at partials[v*G + p]
+// = G work-groups, value v of group p at partials[v*G + c(p)], c a permutation of the
+// groups
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
                        long long count, double* partials, hipStream_t st);
 // cost only of the I-epsilon pass: one FP64 partial per work-group

@@ -104,8 +104,8 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_alpha_ray(BhcParams P, const 
 // column spread over the LDS banks), in dynamic LDS sized for E: at E = 20, 26 KiB per
 // group.  The grid is capped (bhc_ie_groups), so the partials take at most
 // kIeMaxGroups * nvals doubles whatever the ray count; they are stored value-major (value
-// v of group p at v * G + p), so the reduction reads each value's G partials
-// contiguously.
+// v of group p at v * G + c(p), c ordering the groups XCD by XCD), so the reduction reads
+// each value's G partials contiguously.
 __host__ __device__ inline int ie_blocks(int E) { return (E + 3) / 4; }
 // floats per LDS basis row: 4 per 4-block, an odd number of 16-byte units
 __host__ __device__ inline int ie_row(int E) { return 4 * (ie_blocks(E) | 1); }
@@ -124,6 +124,23 @@ __device__ __forceinline__ double add_from_below(double a)
     const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), 0x110 + OFF, 0xf, 0xf, true);
     return a + __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
                                                 (unsigned)lo));
+}
+
+// q_i, r_i (stored in *qr) and f_i (returned) of one ray of the I-epsilon pass from Ir_i
+// and Imea_i, with one FP64 division: Poisson q = (Imea/Ir)/Ir, r = 1 - Imea/Ir,
+// f = (Ir - Imea) + Imea log(Imea/Ir) (bhc/poissLI.m; f = Ir where Imea = 0); Gaussian
+// (bhc/gaussLI.m) q = (1 - Err)/Ir^2, r = Err/Ir, f = Err^2/2 with Err = log(Ir/Imea).
+__device__ __forceinline__ double ie_ray_terms(int noise, double ir, double im, double2* qr)
+{
+    const double rinv = 1.0 / ir;
+    if (noise == CTPRJ_NOISE_GAUSSIAN) {
+        const double err = gauss_err(ir, im);
+        if (qr) *qr = make_double2((1.0 - err) * rinv * rinv, err * rinv);
+        return 0.5 * err * err;
+    }
+    const double ratio = im * rinv;
+    if (qr) *qr = make_double2(ratio * rinv, 1.0 - ratio);
+    return (im > 0.0) ? (ir - im) + im * log(ratio) : ir;
 }
 
 // One tile of the pass: the basis rows, q_i, r_i of rays tile * kBhcBlock + t into LDS,
@@ -146,14 +163,7 @@ __device__ __forceinline__ void ie_tile_rows(const BhcParams& P, const float* __
             my[j] = p;
             ir = fma(P.Ied[j], (double)p, ir);
         });
-        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {  // bhc/gaussLI.m:3-10
-            const double err = gauss_err(ir, im);
-            qr[t] = make_double2((1.0 - err) / (ir * ir), err / ir);
-            fs[t] += 0.5 * err * err;
-        } else {
-            qr[t] = make_double2(im / (ir * ir), 1.0 - im / ir);
-            fs[t] += nll_term(ir, im);
-        }
+        fs[t] += ie_ray_terms(P.noise, ir, im, &qr[t]);
     } else {  // past the last ray: finite zeros (times a zero weight)
         for (int j = 0; j < row; ++j) my[j] = 0.0f;
         qr[t] = make_double2(0.0, 0.0);
@@ -257,7 +267,11 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
     }
     const int nh = E * (E + 1) / 2, nv = nh + E + 1;
     const long long G = gridDim.x;
-    double* out = partials + blockIdx.x;  // value v at out[v * G]
+    // the group's column: groups go to the 8 XCDs round-robin (group b on XCD b % 8), and
+    // the columns are ordered XCD by XCD, so the 128-byte lines of a value's row are
+    // written from one XCD's L2 each and no line is written back from several
+    const long long xb = blockIdx.x % 8, xq = G / 8, xr = G % 8;
+    double* out = partials + (xb * xq + (xb < xr ? xb : xr) + blockIdx.x / 8);  // v at out[v * G]
     if (t == 0) out[(nv - 1) * G] = ftot;
     if (lane != L - 1 || slot >= nhb) return;
 #pragma unroll
@@ -291,19 +305,12 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_cost(BhcParams P, const fl
         spline_basis<1>(P, si, [&](int j, float p, float, float) {
             ir = fma(P.Ied[j], (double)p, ir);
         });
-        if (P.noise == CTPRJ_NOISE_GAUSSIAN) {
-            const double err = gauss_err(ir, im);
-            f = 0.5 * err * err;
-        } else {
-            f = nll_term(ir, im);
-        }
+        f = ie_ray_terms(P.noise, ir, im, nullptr);
     }
     const double tot = block_sum(f, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
 }
 
-// out[v] = sum over p of partials[p*nvals + v]: one work-group per value, each thread
-// sums a fixed stride of partials in order, then a fixed LDS tree.
 // Sum of value v = blockIdx.x over the nparts partials stored at partials[v * nparts + p]
 // (thread-strided partial sums, then block_sum: a fixed order).
 __device__ __forceinline__ double reduce_value(const double* __restrict__ partials, long long nparts,
