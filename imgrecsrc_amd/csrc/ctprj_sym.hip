@@ -682,6 +682,10 @@ struct AdjStage {
     float val[kStageBins][12];
 };
 
+// Byte offset past the end of every slice's sinogram buffer (whose size is below 2^31
+// bytes): a 16-byte raw buffer load there is out of range in every dword and reads 0.
+constexpr int kStageNoRow = 0x7ffffff0;
+
 // Wave minimum of lo and maximum of hi: prefix reductions inside each row of 16 lanes
 // (DPP row shifts; lanes with no source read the operation's identity, which lets the
 // compiler fold each shift into the min / max), then the four rows' last lanes.  The
@@ -741,13 +745,13 @@ __device__ __forceinline__ void adj_sym_chunk(const SliceDesc& S, const Ray* __r
     // the view loop as eight per-element sets
     const bool fl = (t & kRowFlip) != 0;
     const int b0 = L0 + 4 * ql;
-    float x[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-    if (!(t & kRowNone)) {
-        const int voff = (t & ~(kRowFlip | kRowNone)) + 4 * (fl ? 2 * S.pC - b0 - 3 : b0);
-        const auto v = __builtin_amdgcn_raw_buffer_load_b128(RS, voff, 0, 0);
+    // a row without an entry reads past the buffer's end, which gives zeros (no branch)
+    const int voff = (t & kRowNone) ? kStageNoRow
+                                    : (t & ~(kRowFlip | kRowNone)) + 4 * (fl ? 2 * S.pC - b0 - 3 : b0);
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(RS, voff, 0, 0);
+    float x[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) x[i] = __uint_as_float(v[i]);
-    }
+    for (int i = 0; i < 4; ++i) x[i] = __uint_as_float(v[i]);
     float4 rv;
     if (FAN) {  // ray records of bins L0 .. L0 + kStageBins - 1: 16 bytes per lane
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(RR, L0 * (int)sizeof(Ray) + lane * 16,
