@@ -672,7 +672,7 @@ __device__ __forceinline__ void adj_sym_elem_ct(const SliceDesc& S, const Ray* _
 // records, into a wave-private LDS area; the pixels then read their windows from LDS.
 // Every pixel still forms the same weights from the same ray records and sums its bins
 // in the same order, so the image is bit-identical.  A wave whose range exceeds
-// kStageBins bins (fine detector pitch) takes the direct gather for that element.
+// kStageBins bins (fine detector pitch) stages it in chunks of kStageBins bins.
 constexpr int kStageBins = 32;
 
 struct AdjStage {
@@ -749,9 +749,8 @@ __device__ __forceinline__ void adj_sym_chunk(const SliceDesc& S, const Ray* __r
     const int voff = (t & kRowNone) ? kStageNoRow
                                     : (t & ~(kRowFlip | kRowNone)) + 4 * (fl ? 2 * S.pC - b0 - 3 : b0);
     const auto v = __builtin_amdgcn_raw_buffer_load_b128(RS, voff, 0, 0);
-    float x[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) x[i] = __uint_as_float(v[i]);
+    const float x0 = __uint_as_float(v[0]), x1 = __uint_as_float(v[1]);
+    const float x2 = __uint_as_float(v[2]), x3 = __uint_as_float(v[3]);
     float4 rv;
     if (FAN) {  // ray records of bins L0 .. L0 + kStageBins - 1: 16 bytes per lane
         const auto v = __builtin_amdgcn_raw_buffer_load_b128(RR, L0 * (int)sizeof(Ray) + lane * 16,
@@ -759,9 +758,12 @@ __device__ __forceinline__ void adj_sym_chunk(const SliceDesc& S, const Ray* __r
         rv = make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
                          __uint_as_float(v[3]));
     }
-    // x[j] is bin b0 + j, or b0 + 3 - j of a reflected row
-#pragma unroll
-    for (int j = 0; j < 4; ++j) st->val[4 * ql + (fl ? 3 - j : j)][hl] = x[j];
+    // xj is bin b0 + j, or b0 + 3 - j of a reflected row: reversed in registers, so that
+    // the stores keep per-lane addresses
+    st->val[4 * ql + 0][hl] = fl ? x3 : x0;
+    st->val[4 * ql + 1][hl] = fl ? x2 : x1;
+    st->val[4 * ql + 2][hl] = fl ? x1 : x2;
+    st->val[4 * ql + 3][hl] = fl ? x0 : x3;
     if (FAN) st->ray[lane] = rv;
     wave_lds_sync();  // staged values visible to every lane
     if (blo > bhi) return;  // no candidate of this lane in the chunk
