@@ -148,19 +148,28 @@ __device__ __forceinline__ float pm(const float* mask, long long i, float v)
     return (!mask || mask[i] != 0.0f) ? v : 0.0f;
 }
 
-// S->moved is set when any pixel of xbar differs from alpha (the restart test's
-// ||xbar - alpha||_1 != 0, PNPG.m:104); k_pnpg_begin clears it.
-__global__ void k_pnpg_xbar(PnpgState* __restrict__ S, const float* __restrict__ alpha,
-                            const float* __restrict__ pre, float* __restrict__ xbar, long long n,
-                            const float* __restrict__ mask)
+// moved[work-group] (when given) = whether any pixel of the group's xbar differs from
+// alpha; k_pnpg_dot_reduce ORs the flags into S->moved (the restart test's
+// ||xbar - alpha||_1 != 0, PNPG.m:104).  A flag per group instead of an atomic on one
+// word per changed pixel: those serialised to ~14 ms on a 512 x 512^2 stack.
+__global__ __launch_bounds__(256) void k_pnpg_xbar(const PnpgState* __restrict__ S,
+                                                   const float* __restrict__ alpha,
+                                                   const float* __restrict__ pre,
+                                                   float* __restrict__ xbar, long long n,
+                                                   const float* __restrict__ mask,
+                                                   int* __restrict__ moved)
 {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float c = (float)S->coef;
-    const float a = alpha[i];
-    const float x = pm(mask, i, fmaxf(fmaf(c, a - pre[i], a), 0.0f));
-    xbar[i] = x;
-    if (x != a) atomicOr(&S->moved, 1);
+    int diff = 0;
+    if (i < n) {
+        const float c = (float)S->coef;
+        const float a = alpha[i];
+        const float x = pm(mask, i, fmaxf(fmaf(c, a - pre[i], a), 0.0f));
+        xbar[i] = x;
+        diff = x != a;
+    }
+    const int any = __syncthreads_or(diff);
+    if (moved && threadIdx.x == 0) moved[blockIdx.x] = any;
 }
 
 // newX = max(xbar - grad/t, 0) and the FP64 partials of <newX-xbar, grad>, ||newX-xbar||^2
@@ -200,16 +209,19 @@ __global__ __launch_bounds__(256) void k_pnpg_prox(const PnpgState* __restrict__
     }
 }
 
-__global__ void k_pnpg_dot_reduce(const double* __restrict__ partials, long long nparts,
-                                  PnpgState* S)
+__global__ void k_pnpg_dot_reduce(const double* __restrict__ partials,
+                                  const int* __restrict__ moved, long long nparts, PnpgState* S)
 {
     __shared__ double sg[256], sd[256];
     const int t = threadIdx.x;
     double a = 0.0, b = 0.0;
+    int m = 0;
     for (long long p = t; p < nparts; p += 256) {
         a += partials[2 * p];
         b += partials[2 * p + 1];
+        m |= moved[p];
     }
+    m = __syncthreads_or(m);
     sg[t] = a;
     sd[t] = b;
     __syncthreads();
@@ -223,6 +235,7 @@ __global__ void k_pnpg_dot_reduce(const double* __restrict__ partials, long long
     if (t == 0) {
         S->dg = sg[0];
         S->dd = sd[0];
+        S->moved = m;
     }
 }
 
@@ -383,6 +396,7 @@ struct ctprj_pnpg {
     float *d_pre = nullptr, *d_xbar = nullptr, *d_newx = nullptr, *d_grad = nullptr;
     float* d_s = nullptr;         // sinogram: projections, then gradient weights
     double* d_part = nullptr;     // prox partials (2 per work-group)
+    int* d_moved = nullptr;       // per work-group: xbar differs from alpha
     double* d_ray_part = nullptr; // NLL partials
     long long nimg = 0, nsino = 0, prox_blocks = 0, ray_blocks = 0;
     // wavelet prox (PNPG.m properties admmTol, maxInnerItr, maxPossibleInnerItr)
@@ -403,7 +417,7 @@ struct ctprj_pnpg {
     ~ctprj_pnpg()
     {
         for (void* p : {(void*)d_state, (void*)d_pre, (void*)d_xbar, (void*)d_newx, (void*)d_grad,
-                        (void*)d_s, (void*)d_part, (void*)d_ray_part, (void*)d_wpart,
+                        (void*)d_s, (void*)d_part, (void*)d_moved, (void*)d_ray_part, (void*)d_wpart,
                         (void*)d_ppart, (void*)d_mask})
             if (p) (void)hipFree(p);
     }
@@ -446,6 +460,7 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
         (rc = pnpg_alloc((void**)&s->d_grad, sizeof(float) * s->nimg, "pnpg grad")) ||
         (rc = pnpg_alloc((void**)&s->d_s, sizeof(float) * s->nsino, "pnpg sinogram")) ||
         (rc = pnpg_alloc((void**)&s->d_part, sizeof(double) * 2 * s->prox_blocks, "pnpg partials")) ||
+        (rc = pnpg_alloc((void**)&s->d_moved, sizeof(int) * s->prox_blocks, "pnpg moved flags")) ||
         (rc = pnpg_alloc((void**)&s->d_ray_part, sizeof(double) * s->ray_blocks, "pnpg NLL partials"))) {
         delete s;
         return rc;
@@ -588,7 +603,7 @@ static int pnpg_step_iterative(ctprj_pnpg* s, float* d_alpha, hipStream_t st)
     const unsigned gi = (unsigned)((s->nimg + 255) / 256);
     PnpgState* S = s->d_state;
     hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg,
-                       s->d_mask);
+                       s->d_mask, nullptr);  // the trial's own sums test xbar != alpha
     if ((rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_gradstep, dim3(gi), dim3(256), 0, st, S, s->d_xbar, s->d_grad,
                        s->d_newx, s->nimg, s->d_mask);
@@ -753,12 +768,13 @@ extern "C" int ctprj_pnpg_step(ctprj_pnpg* s, float* d_alpha, void* stream)
     PnpgState* S = s->d_state;
     hipLaunchKernelGGL(k_pnpg_begin, dim3(1), dim3(64), 0, st, S, s->pp);
     hipLaunchKernelGGL(k_pnpg_xbar, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_xbar, s->nimg,
-                       s->d_mask);
+                       s->d_mask, s->d_moved);
     int rc = pnpg_func(s, s->d_xbar, &S->f_old, s->d_grad, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_pnpg_prox, dim3((unsigned)s->prox_blocks), dim3(256), 0, st, S, s->d_xbar,
                        s->d_grad, s->d_newx, s->nimg, s->d_mask, s->d_part);
-    hipLaunchKernelGGL(k_pnpg_dot_reduce, dim3(1), dim3(256), 0, st, s->d_part, s->prox_blocks, S);
+    hipLaunchKernelGGL(k_pnpg_dot_reduce, dim3(1), dim3(256), 0, st, s->d_part, s->d_moved,
+                       s->prox_blocks, S);
     if ((rc = pnpg_func(s, s->d_newx, &S->f_new, nullptr, st))) return rc;
     hipLaunchKernelGGL(k_pnpg_decide, dim3(1), dim3(64), 0, st, S, s->pp);
     hipLaunchKernelGGL(k_pnpg_commit, dim3(gi), dim3(256), 0, st, S, d_alpha, s->d_pre, s->d_newx, s->nimg);
