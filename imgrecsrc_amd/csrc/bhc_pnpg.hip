@@ -488,7 +488,22 @@ extern "C" int ctprj_pnpg_create(ctprj_plan* plan, const float* d_Imea, const do
 static int pnpg_func(ctprj_pnpg* s, const float* x, double* dst, float* grad, hipStream_t st)
 {
     Plan* p = s->plan;
-    int rc = p->launch_forward(x, s->d_s, st, false);
+    int rc;
+    // plans whose slices all take the symmetric kernels: the ray pass runs in the forward's
+    // store and the adjoint's first wave adds the NLL partials (as ctprj_bhc_grad_alpha
+    // does; the same w, so the same gradient)
+    const long long fused = p->bhc_fused_parts();
+    if (fused >= 0 && fused <= p->red_cap) {
+        if ((rc = p->launch_forward_bhc(x, s->d_s, BhcStore{s->bp, s->d_Imea, p->d_red, s->Ts, 0},
+                                        st)))
+            return rc;
+        const RedJob job{p->d_red, fused, dst};
+        if (grad) return p->launch_adjoint(s->d_s, grad, st, false, job);
+        launch_wave_sum(job, st);
+        hipError_t e = hipGetLastError();
+        return e == hipSuccess ? 0 : hip_fail(e, "pnpg objective");
+    }
+    rc = p->launch_forward(x, s->d_s, st, false);
     if (rc) return rc;
     launch_bhc_alpha_ray(s->bp, s->d_s, s->Ts, s->d_Imea, s->nsino, s->Ts, grad ? s->d_s : nullptr,
                          nullptr, s->d_ray_part, st);
