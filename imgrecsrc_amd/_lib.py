@@ -126,6 +126,9 @@ SIGNATURES = [
     ("ctprj_pnpg_set_linesearch", _i, [_vp, _i]),
     ("ctprj_pnpg_stats", _i, [_vp, ctypes.POINTER(PnpgInfo), _vp]),
     ("ctprj_pnpg_destroy", _i, [_vp]),
+    ("ctprj_pnpg_snapshot_bytes", _i, [_vp, ctypes.POINTER(ctypes.c_longlong)]),
+    ("ctprj_pnpg_save", _i, [_vp, _vp, ctypes.c_longlong, _vp]),
+    ("ctprj_pnpg_load", _i, [_vp, _vp, ctypes.c_longlong, _vp]),
     ("ctprj_last_error", ctypes.c_char_p, []),
     ("ctprj_device_arch", _i, [ctypes.c_char_p, _i]),
     ("ctprj_version", ctypes.c_char_p, []),

@@ -14,6 +14,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 #include <new>
 #include <vector>
 
@@ -849,5 +850,142 @@ extern "C" int ctprj_debug_pnpg_decide(const double* in, int ls, int moved, doub
 extern "C" int ctprj_pnpg_destroy(ctprj_pnpg* s)
 {
     delete s;
+    return 0;
+}
+
+// ------------------------------------------------------------ snapshot / resume ---
+//
+// bhc/doBHC.m:78-91 runs the alpha step at five penalty levels, warm-starting each level
+// from the previous one's alpha and Ie and saving the workspace after each.  A snapshot
+// holds what a solver carries from one trial to the next: its device state, preAlpha
+// and the host-side schedule (prox tolerance and inner-iteration limits, the reset()
+// estimate, counters).  Loaded into a solver created with the same plan and arguments
+// it continues bit for bit.  alpha (the caller's array), Ie and the mask are the
+// caller's to keep; the wavelet and TV contexts carry nothing from one prox call to the
+// next.  Layout: SnapHeader, PnpgState, SnapHost, float preAlpha[nimg].
+
+namespace {
+
+constexpr unsigned kSnapMagic = 0x474e504eu;  // "NPNG" little-endian
+constexpr unsigned kSnapVersion = 1;
+
+struct SnapHeader {
+    unsigned magic, version;
+    long long nimg, nsino;
+    int E, prox;  // prox: 0 non-negativity, 1 wavelet, 2 TV
+    int state_bytes, host_bytes;
+};
+
+struct SnapHost {
+    PnpgParams pp;
+    PnpgState h_state;
+    double u, admmTol, difAlpha, t_reset;
+    int maxInnerItr, maxPossibleInnerItr, innerSearch, h_valid, resets, pad;
+};
+
+long long snapshot_bytes(const ctprj_pnpg* s)
+{
+    return (long long)(sizeof(SnapHeader) + sizeof(PnpgState) + sizeof(SnapHost)) +
+           (long long)sizeof(float) * s->nimg;
+}
+
+SnapHeader snapshot_header(const ctprj_pnpg* s)
+{
+    SnapHeader h{};
+    h.magic = kSnapMagic;
+    h.version = kSnapVersion;
+    h.nimg = s->nimg;
+    h.nsino = s->nsino;
+    h.E = s->bp.E;
+    h.prox = s->wv ? 1 : (s->tv ? 2 : 0);
+    h.state_bytes = (int)sizeof(PnpgState);
+    h.host_bytes = (int)sizeof(SnapHost);
+    return h;
+}
+
+}  // namespace
+
+extern "C" int ctprj_pnpg_snapshot_bytes(const ctprj_pnpg* s, long long* bytes)
+{
+    if (!s || !bytes) return fail(CTPRJ_EINVAL, "ctprj_pnpg_snapshot_bytes: NULL argument");
+    *bytes = snapshot_bytes(s);
+    return 0;
+}
+
+extern "C" int ctprj_pnpg_save(ctprj_pnpg* s, void* host, long long bytes, void* stream)
+{
+    if (!s || !host) return fail(CTPRJ_EINVAL, "ctprj_pnpg_save: NULL argument");
+    if (bytes != snapshot_bytes(s))
+        return fail(CTPRJ_EINVAL, "ctprj_pnpg_save: buffer of %lld bytes, the snapshot takes %lld",
+                    bytes, snapshot_bytes(s));
+    hipStream_t st = (hipStream_t)stream;
+    char* b = static_cast<char*>(host);
+    const SnapHeader h = snapshot_header(s);
+    std::memcpy(b, &h, sizeof h);
+    b += sizeof h;
+    // the device state and preAlpha after every trial queued on the stream
+    hipError_t e = hipMemcpyAsync(b, s->d_state, sizeof(PnpgState), hipMemcpyDeviceToHost, st);
+    char* pre = b + sizeof(PnpgState) + sizeof(SnapHost);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(pre, s->d_pre, sizeof(float) * s->nimg, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(e, "ctprj_pnpg_save");
+    SnapHost hs{};
+    hs.pp = s->pp;
+    hs.h_state = s->h_state;
+    hs.u = s->u;
+    hs.admmTol = s->admmTol;
+    hs.difAlpha = s->difAlpha;
+    hs.t_reset = s->t_reset;
+    hs.maxInnerItr = s->maxInnerItr;
+    hs.maxPossibleInnerItr = s->maxPossibleInnerItr;
+    hs.innerSearch = s->innerSearch;
+    hs.h_valid = s->h_valid ? 1 : 0;
+    hs.resets = s->resets;
+    std::memcpy(b + sizeof(PnpgState), &hs, sizeof hs);
+    return 0;
+}
+
+extern "C" int ctprj_pnpg_load(ctprj_pnpg* s, const void* host, long long bytes, void* stream)
+{
+    if (!s || !host) return fail(CTPRJ_EINVAL, "ctprj_pnpg_load: NULL argument");
+    if (bytes != snapshot_bytes(s))
+        return fail(CTPRJ_EINVAL, "ctprj_pnpg_load: %lld bytes, this solver's snapshot takes %lld",
+                    bytes, snapshot_bytes(s));
+    const char* b = static_cast<const char*>(host);
+    SnapHeader h;
+    std::memcpy(&h, b, sizeof h);
+    const SnapHeader want = snapshot_header(s);
+    if (h.magic != kSnapMagic || h.version != kSnapVersion)
+        return fail(CTPRJ_EINVAL, "ctprj_pnpg_load: not a PNPG snapshot (version %u)", kSnapVersion);
+    if (h.nimg != want.nimg || h.nsino != want.nsino || h.E != want.E ||
+        h.state_bytes != want.state_bytes || h.host_bytes != want.host_bytes)
+        return fail(CTPRJ_EINVAL,
+                    "ctprj_pnpg_load: snapshot of %lld pixels / %lld rays / E = %d, solver of "
+                    "%lld / %lld / %d", h.nimg, h.nsino, h.E, want.nimg, want.nsino, want.E);
+    if (h.prox != want.prox)
+        return fail(CTPRJ_EINVAL, "ctprj_pnpg_load: snapshot prox %d, solver prox %d (0 none, "
+                    "1 wavelet, 2 TV)", h.prox, want.prox);
+    b += sizeof h;
+    hipStream_t st = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(s->d_state, b, sizeof(PnpgState), hipMemcpyHostToDevice, st);
+    const char* pre = b + sizeof(PnpgState) + sizeof(SnapHost);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(s->d_pre, pre, sizeof(float) * s->nimg, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);  // the caller may free `host`
+    if (e != hipSuccess) return hip_fail(e, "ctprj_pnpg_load");
+    SnapHost hs;
+    std::memcpy(&hs, b + sizeof(PnpgState), sizeof hs);
+    s->pp = hs.pp;
+    s->h_state = hs.h_state;
+    s->u = hs.u;
+    s->admmTol = hs.admmTol;
+    s->difAlpha = hs.difAlpha;
+    s->t_reset = hs.t_reset;
+    s->maxInnerItr = hs.maxInnerItr;
+    s->maxPossibleInnerItr = hs.maxPossibleInnerItr;
+    s->innerSearch = hs.innerSearch;
+    s->h_valid = hs.h_valid != 0;
+    s->resets = hs.resets;
     return 0;
 }

@@ -7,7 +7,9 @@ run of trials can be captured once in a HIP graph (``torch.cuda.CUDAGraph``) and
 replayed; ``stats()`` reads the device scalars.  ``PNPG.set_wavelet(w, u)`` switches to
 the reference's default prox ('wvltADMM': non-negativity plus u ||W alpha||_1 solved by
 npg/admm.m), ``PNPG.set_tv(tv, u)`` to 'tvl1' / 'tviso' (utils/TV.m); their trials
-synchronise.
+synchronise.  ``PNPG.save(path, alpha)`` / ``PNPG.load(path, alpha)`` snapshot a run and
+resume it in a new solver bit for bit (bhc/doBHC.m:85-91 saves and warm-starts between
+penalty levels).
 """
 from __future__ import annotations
 
@@ -104,6 +106,42 @@ class PNPG:
         info = PnpgInfo()
         check(lib().ctprj_pnpg_stats(self._h, ctypes.byref(info), _stream_ptr(stream)))
         return {f: getattr(info, f) for f, _ in PnpgInfo._fields_}
+
+    def save_state(self, stream=None) -> np.ndarray:
+        """The solver's iteration state after every queued trial, as an opaque byte array
+        (ctprj_pnpg_save); alpha is not in it."""
+        n = ctypes.c_longlong()
+        check(lib().ctprj_pnpg_snapshot_bytes(self._h, ctypes.byref(n)))
+        blob = np.empty(n.value, np.uint8)
+        check(lib().ctprj_pnpg_save(self._h, blob.ctypes.data_as(ctypes.c_void_p), n.value,
+                                    _stream_ptr(stream)))
+        return blob
+
+    def load_state(self, blob, stream=None):
+        """Restores a save_state() blob into this solver (created with the same plan,
+        measurements, spectrum, Ts and prox): its next step continues the saved run."""
+        blob = np.ascontiguousarray(np.asarray(blob, np.uint8).reshape(-1))
+        check(lib().ctprj_pnpg_load(self._h, blob.ctypes.data_as(ctypes.c_void_p), blob.size,
+                                    _stream_ptr(stream)))
+
+    def save(self, path, alpha: torch.Tensor):
+        """Snapshot of a run to an .npz file: the solver state, alpha, Ie, knots and Ts
+        (bhc/doBHC.m:85 saves the workspace after each penalty level)."""
+        np.savez(path, state=self.save_state(), alpha=alpha.detach().reshape(-1).cpu().numpy(),
+                 Ie=self.Ie, knots=self.knots, Ts=np.float64(self.Ts))
+
+    def load(self, path, alpha: torch.Tensor):
+        """Resumes from save(): the solver state is restored and alpha (the plan's image
+        vector on the device) overwritten with the saved iterate."""
+        with np.load(path, allow_pickle=False) as z:
+            a = z["alpha"]
+            if a.size != alpha.numel():
+                raise ValueError(f"snapshot alpha has {a.size} pixels, alpha {alpha.numel()}")
+            if not (np.array_equal(z["Ie"], self.Ie) and np.array_equal(z["knots"], self.knots)
+                    and float(z["Ts"]) == self.Ts):
+                raise ValueError("snapshot of another spectrum (Ie, knots) or Ts")
+            self.load_state(z["state"])
+            alpha.copy_(torch.from_numpy(a).to(alpha.device).reshape(alpha.shape))
 
     def close(self):
         if getattr(self, "_h", None):

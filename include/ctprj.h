@@ -402,14 +402,28 @@ int ctprj_pnpg_set_mask(ctprj_pnpg* solver, const float* d_mask);
  * of equal size) and outlive the solver; w = NULL returns to the non-negativity prox.
  * Only before the first trial.  In this mode ctprj_pnpg_step synchronises `stream`
  * (the ADMM stopping tests run on the host) and cannot be captured in a graph.  On a
- * failed line search (PNPG.m:94-103) theta restarts at 1; the reference's reset()
- * additionally re-estimates t from the Hessian, which is not done here. */
+ * given-up line search theta restarts at 1 and t = min(t, the Hessian step estimate at
+ * alpha), as the reference's reset() (npg/PNPG.m:168-172, npg/Methods.m:108-115). */
 int ctprj_pnpg_set_wavelet(ctprj_pnpg* solver, ctprj_wavelet* w, double u);
 /* The same with the TV prox (opt.proximal = 'tvl1' / 'tviso', penalty u TV(alpha),
  * utils/TV.m denoise with innerThresh = admmTol * difAlpha and maxInnerItr).  tv = NULL
  * returns to the non-negativity prox. */
 int ctprj_pnpg_set_tv(ctprj_pnpg* solver, ctprj_tv* tv, double u);
 int ctprj_pnpg_destroy(ctprj_pnpg* solver);
+/* Snapshot and resume of a PNPG run (bhc/doBHC.m:78-91 saves its workspace after each
+ * penalty level and warm-starts the next level from it).  A snapshot is an opaque host
+ * blob of ctprj_pnpg_snapshot_bytes bytes holding the solver's iteration state (theta, t,
+ * the counters, preAlpha, the prox's tolerance and inner-iteration schedule, the last
+ * reset() estimate).  ctprj_pnpg_save fills it after every trial queued on `stream`
+ * (synchronises); ctprj_pnpg_load restores it into a solver created with the same plan,
+ * Imea, spectrum, Ts and prox (set_wavelet / set_tv / set_mask called as for the saved
+ * one), after which ctprj_pnpg_step continues exactly as the saved solver would (bit for
+ * bit).  alpha is the caller's array and is saved beside the blob.  A blob of another
+ * size, version, image / ray count, E or prox gives CTPRJ_EINVAL.  No counterpart call
+ * in the reference, which saves the MATLAB workspace (`save([filename '.mat'])`). */
+int ctprj_pnpg_snapshot_bytes(const ctprj_pnpg* solver, long long* bytes);
+int ctprj_pnpg_save(ctprj_pnpg* solver, void* host, long long bytes, void* stream);
+int ctprj_pnpg_load(ctprj_pnpg* solver, const void* host, long long bytes, void* stream);
 
 /* ------------------------------------------------------------------------------
  * Library state.

@@ -325,3 +325,61 @@ def test_pnpg_reset_step_estimate_matches_restatement(oracle_mod):
             assert abs(st["cost"] - ref.cost) <= 1e-5 * abs(ref.cost), log
     print("\n".join(map(str, log)))
     assert ref.resets >= 1
+
+
+@pytest.mark.parametrize("prior", ["nonneg", "wavelet"])
+def test_snapshot_resume_bit_for_bit(oracle_mod, prior, tmp_path):
+    """A run saved after 5 trials (PNPG.save: solver state + alpha) and resumed in a new
+    solver continues exactly as the original: 6 more trials give the same alpha and
+    statistics bit for bit (bhc/doBHC.m:85-91 saves and warm-starts between levels)."""
+    import torch
+    from oracle import bhc_oracle as B
+    from imgrecsrc_amd._lib import CtprjError
+    from imgrecsrc_amd.phantom import ellipse_params, rasterise
+    from imgrecsrc_amd.pnpg import PNPG
+    from imgrecsrc_amd.projector import Projector
+    from imgrecsrc_amd.wavelet import Wavelet, daubcqf
+    geo = (64, 91, 72, 72, 1.0, 1.0, 250.0)
+    n = geo[0]
+    knots = B.spline_knots(B.kappa_logspace(E, 0.05, 20.0))
+    Ie = np.exp(-0.5 * ((np.arange(E) - 3.0) / 1.5) ** 2) + 0.05
+    Ie = Ie / (((knots[2:] - knots[:-2]) / 2) @ Ie)
+    cf = oracle_mod.conf(*geo)
+    raw = oracle_mod.forward(cf, rasterise(ellipse_params(91, n), n))[0].ravel()
+    Ts = 2.0 / raw.max()
+    BL, _, _ = B.b1_iout_stable(knots, Ts * raw, Ie)
+    counts = np.random.default_rng(92).poisson(1e5 * BL)
+    Imea = torch.from_numpy((counts / counts.max()).astype(np.float32)).cuda()
+    proj = Projector([geo])
+
+    def make(alpha, t0=None):
+        s = PNPG(proj, Imea, knots, Ie, Ts, alpha, t0=t0)
+        if prior == "wavelet":
+            s.set_wavelet(Wavelet(n, n, daubcqf(4)[0], 3), 2e-3 * s.t0)
+        return s
+
+    alpha = torch.full((n * n,), 0.05, dtype=torch.float32, device="cuda")
+    a = make(alpha)
+    for _ in range(5):
+        a.step(alpha)
+    path = tmp_path / "run.npz"
+    a.save(path, alpha)
+    for _ in range(6):
+        a.step(alpha)
+    sa = a.stats()
+    alpha_b = torch.zeros_like(alpha)
+    b = make(alpha_b.fill_(0.05), t0=a.t0)
+    b.load(path, alpha_b)
+    for _ in range(6):
+        b.step(alpha_b)
+    sb = b.stats()
+    assert torch.equal(alpha, alpha_b)
+    assert sa == sb, (sa, sb)
+    assert sa["trials"] == 11
+    blob = a.save_state()
+    with pytest.raises(CtprjError):
+        b.load_state(blob[:-4])
+    other = PNPG(Projector([(48, 71, 60, 60, 1.0, 1.0, 200.0)]), torch.ones(60 * 71, device="cuda"),
+                 knots, Ie, Ts, torch.full((48 * 48,), 0.05, device="cuda"), t0=1.0)
+    with pytest.raises(CtprjError):
+        other.load_state(np.zeros(int(blob.size), np.uint8))
