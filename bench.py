@@ -683,6 +683,19 @@ def main():
     roofline = roofline_entry(tag, dom_name, dom_ms, launches, nnz, tune,
                               t_fwd if dom_name.startswith("k_adjoint") else t_adj)
 
+    # the pair replayed as one HIP graph next to the same pair launched eagerly: what
+    # the call's launches (transpose, forward, its combine, adjoint, its combine) cost
+    step()
+    torch.cuda.synchronize()
+    back_eager = back.clone()
+    t_pair = timed(step, reps, stream)
+    t_pair_graph, _, pair_graph = graph_timed(step, reps, stream)
+    pair_replay = {"eager_ms": round(t_pair, 4), "graph_ms": round(t_pair_graph, 4),
+                   "launches_per_pair": (2 + (tune["fwd_split"] > 1)) + (1 + (tune["adj_split"] > 1)),
+                   "replay_equals_eager": bool(torch.equal(back, back_eager))}
+    del pair_graph
+    log(f"pair eager {t_pair:.4f} ms, graph replay {t_pair_graph:.4f} ms")
+
     # <Ax, y> = <x, A^T y> on seeded Gaussian x, y
     x = torch.from_numpy(gaussian((n, n), 21)).to(dev).reshape(-1)
     y = torch.from_numpy(gaussian((npj, W), 22)).to(dev).reshape(-1)
@@ -702,6 +715,7 @@ def main():
                    "parallelism": "single GPU", "symmetry_group": P.modes()[0],
                    "tune": tune},
         "adjoint_test_rel": adj_rel,
+        "pair_graph_replay": pair_replay,
         "roofline": roofline,
     }
 
