@@ -324,6 +324,28 @@ extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* 
         if ((rc = p->launch_forward(d_alpha, p->d_sino_ws, st, false))) return rc;
         s = p->d_sino_ws;
     }
+    // the basis matrix of the fixed projection, formed once per call (as IeStep forms
+    // A = polyIout(Phi(alpha), []) once, bhc/beamhardenSpline.m:447) in a buffer the plan
+    // keeps for later calls, when it fits kIeBasisCacheMax; otherwise every pass
+    // evaluates the basis
+    const size_t phi_bytes = sizeof(float) * (size_t)rays * (size_t)E;
+    const float* phi = nullptr;
+    if (phi_bytes <= kIeBasisCacheMax) {
+        if (p->ie_phi_cap < phi_bytes) {
+            if (p->d_ie_phi) (void)hipFree(p->d_ie_phi);
+            p->d_ie_phi = nullptr;
+            p->ie_phi_cap = 0;
+            if (hipMalloc((void**)&p->d_ie_phi, phi_bytes) == hipSuccess)
+                p->ie_phi_cap = phi_bytes;
+            else
+                (void)hipGetLastError();  // a failed allocation is not the passes' error
+        }
+        if (p->d_ie_phi) {
+            launch_bhc_ie_basis(P, s, Ts, rays, p->d_ie_phi, st);
+            if ((rc = last_launch("ie basis launch"))) return rc;
+            phi = p->d_ie_phi;
+        }
+    }
     std::vector<double> x(E), red(nv);
     for (int j = 0; j < E; ++j) x[j] = std::max(Ie[j], 0.0);
     int passes = 0;
@@ -336,7 +358,7 @@ extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* 
     // f, g, H at x (FP64 ray sums on the device)
     auto full = [&](double* f, std::vector<double>* g, std::vector<double>* H) -> int {
         set_ie(x);
-        launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st);
+        launch_bhc_ie_ray(P, s, Ts, d_Imea, rays, p->d_red, st, phi);
         launch_reduce_partials(p->d_red, groups, nv, p->d_red_out, st);
         hipError_t e = hipMemcpyAsync(red.data(), p->d_red_out, sizeof(double) * nv,
                                       hipMemcpyDeviceToHost, st);
@@ -355,7 +377,7 @@ extern "C" int ctprj_ie_solve(ctprj_plan* p, const float* d_alpha, const float* 
     };
     auto cost = [&](const std::vector<double>& v, double* f) -> int {
         set_ie(v);
-        launch_bhc_ie_cost(P, s, Ts, d_Imea, rays, p->d_red, st);
+        launch_bhc_ie_cost(P, s, Ts, d_Imea, rays, p->d_red, st, phi);
         launch_reduce_partials(p->d_red, blocks, 1, p->d_red_out, st);
         hipError_t e = hipMemcpyAsync(f, p->d_red_out, sizeof(double), hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -52,12 +52,21 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
                           hipStream_t st);
 // per-ray I-epsilon pass: per work-group partial [H upper, g, f], bhc_ie_groups(count)
 // = G work-groups, value v of group p at partials[v*G + c(p)], c a permutation of the
-// groups
+// groups; with phi (launch_bhc_ie_basis's matrix of the same s and spectrum basis) the
+// basis values are read instead of evaluated, with the same results
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
-                       long long count, double* partials, hipStream_t st);
-// cost only of the I-epsilon pass: one FP64 partial per work-group
+                       long long count, double* partials, hipStream_t st,
+                       const float* phi = nullptr);
+// cost only of the I-epsilon pass: one FP64 partial per work-group (phi as above)
 void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
-                        long long count, double* partials, hipStream_t st);
+                        long long count, double* partials, hipStream_t st,
+                        const float* phi = nullptr);
+// the I-epsilon pass's basis matrix: value j of ray i at phi[j * count + i]
+void launch_bhc_ie_basis(const BhcParams& P, const float* s, float sscale, long long count,
+                         float* phi, hipStream_t st);
+// largest basis matrix ctprj_ie_solve keeps for its passes (beyond it they evaluate the
+// basis each time)
+constexpr size_t kIeBasisCacheMax = (size_t)1 << 30;
 // out[v] = sum_p partials[v*nparts + p] in a fixed order, v < nvals
 void launch_reduce_partials(const double* partials, long long nparts, int nvals, double* out,
                             hipStream_t st);

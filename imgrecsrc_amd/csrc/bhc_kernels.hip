@@ -144,9 +144,13 @@ __device__ __forceinline__ double ie_ray_terms(int noise, double ir, double im, 
 }
 
 // One tile of the pass: the basis rows, q_i, r_i of rays tile * kBhcBlock + t into LDS,
-// f_i added to the thread's LDS sum.
+// f_i added to the thread's LDS sum.  CACHED: the basis rows are read from phi_g (value
+// j of ray i at phi_g[j * count + i], written by k_bhc_ie_basis with the same
+// spline_basis) instead of evaluated, so every sum is the same bit for bit.
+template <bool CACHED>
 __device__ __forceinline__ void ie_tile_rows(const BhcParams& P, const float* __restrict__ s,
-                                             const float* __restrict__ Imea, long long count,
+                                             const float* __restrict__ Imea,
+                                             const float* __restrict__ phi_g, long long count,
                                              float sscale, long long tile, int row, float* phi,
                                              double2* qr, double* fs)
 {
@@ -156,13 +160,20 @@ __device__ __forceinline__ void ie_tile_rows(const BhcParams& P, const float* __
     // spline_basis writes columns 0..E-1 of a ray's row; the padding columns feed only
     // block entries beyond E, which are never stored, so they are left as they are
     if (i < count) {
-        const float si = sscale * s[i];
         const double im = (double)Imea[i];
         double ir = 0.0;
-        spline_basis<1>(P, si, [&](int j, float p, float, float) {
-            my[j] = p;
-            ir = fma(P.Ied[j], (double)p, ir);
-        });
+        if constexpr (CACHED) {
+            for (int j = 0; j < P.E; ++j) {
+                const float p = phi_g[j * count + i];
+                my[j] = p;
+                ir = fma(P.Ied[j], (double)p, ir);
+            }
+        } else {
+            spline_basis<1>(P, sscale * s[i], [&](int j, float p, float, float) {
+                my[j] = p;
+                ir = fma(P.Ied[j], (double)p, ir);
+            });
+        }
         fs[t] += ie_ray_terms(P.noise, ir, im, &qr[t]);
     } else {  // past the last ray: finite zeros (times a zero weight)
         for (int j = 0; j < row; ++j) my[j] = 0.0f;
@@ -197,10 +208,11 @@ __device__ __forceinline__ void ie_tile_sums(int lane, int L, int row, int bi, i
 // groups per CU, so the 1026 tiles of a 512^2 fan sinogram run in one round instead of
 // two.  With several tiles per group (more than kIeMaxGroups tiles) the grid takes
 // several rounds anyway, and four waves per SIMD leave room for the live sums.
-template <bool ONE>
+template <bool ONE, bool CACHED>
 __global__ __launch_bounds__(kBhcBlock) __attribute__((amdgpu_waves_per_eu(ONE ? 5 : 4)))
 void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restrict__ Imea,
-                  long long count, float sscale, double* __restrict__ partials)
+                  const float* __restrict__ phi_g, long long count, float sscale,
+                  double* __restrict__ partials)
 {
     constexpr int kAcc = 20;  // 16 H entries, 4 g entries
     extern __shared__ __attribute__((aligned(16))) char ie_lds[];
@@ -222,7 +234,7 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
     fs[t] = 0.0;
     double acc[kAcc];
     if constexpr (ONE) {
-        ie_tile_rows(P, s, Imea, count, sscale, blockIdx.x, row, phi, qr, fs);
+        ie_tile_rows<CACHED>(P, s, Imea, phi_g, count, sscale, blockIdx.x, row, phi, qr, fs);
         __syncthreads();
 #pragma unroll
         for (int v = 0; v < kAcc; ++v) acc[v] = 0.0;
@@ -233,7 +245,7 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
         for (int v = 0; v < kAcc; ++v) acc[v] = 0.0;
         const long long ntiles = (count + kBhcBlock - 1) / kBhcBlock;
         for (long long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-            ie_tile_rows(P, s, Imea, count, sscale, tile, row, phi, qr, fs);
+            ie_tile_rows<CACHED>(P, s, Imea, phi_g, count, sscale, tile, row, phi, qr, fs);
             __syncthreads();
             if (slot < nhb) ie_tile_sums(lane, L, row, bi, bj, phi, qr, acc);
             __syncthreads();  // every basis value of the tile read
@@ -289,9 +301,11 @@ void k_bhc_ie_ray(BhcParams P, const float* __restrict__ s, const float* __restr
 }
 
 // Cost only of the I-epsilon pass: f_i of every ray with Ir_i formed exactly as in
-// k_bhc_ie_ray (FP32 basis values, FP64 sum over j in order), one partial per group.
+// k_bhc_ie_ray (FP32 basis values, FP64 sum over j in order), one partial per group;
+// the basis values from phi_g when it is given.
 __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_cost(BhcParams P, const float* __restrict__ s,
                                                            const float* __restrict__ Imea,
+                                                           const float* __restrict__ phi_g,
                                                            long long count, float sscale,
                                                            double* __restrict__ partials)
 {
@@ -299,12 +313,15 @@ __global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_cost(BhcParams P, const fl
     const long long i = (long long)blockIdx.x * kBhcBlock + threadIdx.x;
     double f = 0.0;
     if (i < count) {
-        const float si = sscale * s[i];
         const double im = (double)Imea[i];
         double ir = 0.0;
-        spline_basis<1>(P, si, [&](int j, float p, float, float) {
-            ir = fma(P.Ied[j], (double)p, ir);
-        });
+        if (phi_g) {
+            for (int j = 0; j < P.E; ++j) ir = fma(P.Ied[j], (double)phi_g[j * count + i], ir);
+        } else {
+            spline_basis<1>(P, sscale * s[i], [&](int j, float p, float, float) {
+                ir = fma(P.Ied[j], (double)p, ir);
+            });
+        }
         f = ie_ray_terms(P.noise, ir, im, nullptr);
     }
     const double tot = block_sum(f, sh);
@@ -394,25 +411,58 @@ void launch_bhc_alpha_ray(const BhcParams& P, const float* s, float sscale, cons
 }
 
 void launch_bhc_ie_ray(const BhcParams& P, const float* s, float sscale, const float* Imea,
-                       long long count, double* partials, hipStream_t st)
+                       long long count, double* partials, hipStream_t st, const float* phi)
 {
     const long long groups = bhc_ie_groups(count);
     if (groups <= 0) return;
-    if (groups == bhc_blocks(count))
-        hipLaunchKernelGGL(k_bhc_ie_ray<true>, dim3((unsigned)groups), dim3(kBhcBlock),
-                           ie_lds_bytes(P.E), st, P, s, Imea, count, sscale, partials);
-    else
-        hipLaunchKernelGGL(k_bhc_ie_ray<false>, dim3((unsigned)groups), dim3(kBhcBlock),
-                           ie_lds_bytes(P.E), st, P, s, Imea, count, sscale, partials);
+    const bool one = groups == bhc_blocks(count);
+    const size_t lds = ie_lds_bytes(P.E);
+    const dim3 grid((unsigned)groups), block(kBhcBlock);
+    if (phi) {
+        if (one)
+            hipLaunchKernelGGL((k_bhc_ie_ray<true, true>), grid, block, lds, st, P, s, Imea, phi,
+                               count, sscale, partials);
+        else
+            hipLaunchKernelGGL((k_bhc_ie_ray<false, true>), grid, block, lds, st, P, s, Imea, phi,
+                               count, sscale, partials);
+    } else {
+        if (one)
+            hipLaunchKernelGGL((k_bhc_ie_ray<true, false>), grid, block, lds, st, P, s, Imea, phi,
+                               count, sscale, partials);
+        else
+            hipLaunchKernelGGL((k_bhc_ie_ray<false, false>), grid, block, lds, st, P, s, Imea,
+                               phi, count, sscale, partials);
+    }
 }
 
 void launch_bhc_ie_cost(const BhcParams& P, const float* s, float sscale, const float* Imea,
-                        long long count, double* partials, hipStream_t st)
+                        long long count, double* partials, hipStream_t st, const float* phi)
 {
     const long long blocks = bhc_blocks(count);
     if (blocks <= 0) return;
     hipLaunchKernelGGL(k_bhc_ie_cost, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s, Imea,
-                       count, sscale, partials);
+                       phi, count, sscale, partials);
+}
+
+// The basis matrix of the I-epsilon pass, value j of ray i at phi[j * count + i] (so a
+// wave's stores and loads of one j are contiguous), by the same spline_basis as the
+// pass evaluates.
+__global__ __launch_bounds__(kBhcBlock) void k_bhc_ie_basis(BhcParams P, const float* __restrict__ s,
+                                                            long long count, float sscale,
+                                                            float* __restrict__ phi)
+{
+    const long long i = (long long)blockIdx.x * kBhcBlock + threadIdx.x;
+    if (i >= count) return;
+    spline_basis<1>(P, sscale * s[i], [&](int j, float p, float, float) { phi[j * count + i] = p; });
+}
+
+void launch_bhc_ie_basis(const BhcParams& P, const float* s, float sscale, long long count,
+                         float* phi, hipStream_t st)
+{
+    const long long blocks = bhc_blocks(count);
+    if (blocks <= 0) return;
+    hipLaunchKernelGGL(k_bhc_ie_basis, dim3((unsigned)blocks), dim3(kBhcBlock), 0, st, P, s, count,
+                       sscale, phi);
 }
 
 void launch_reduce_partials(const double* partials, long long nparts, int nvals, double* out,

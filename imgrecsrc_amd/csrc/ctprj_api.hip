@@ -200,6 +200,7 @@ Plan::~Plan()
     free_ptr(d_red);
     free_ptr(d_red_out);
     free_ptr(d_sim);
+    free_ptr(d_ie_phi);
     free_ptr(d_split);
 }
 

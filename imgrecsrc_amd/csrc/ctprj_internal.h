@@ -91,6 +91,8 @@ struct Plan : Tables {
     int bhc_basis = CTPRJ_BASIS_B1, bhc_noise = CTPRJ_NOISE_POISSON;  // spectrum model
     void* d_sim = nullptr;       // measurement simulator workspace
     size_t sim_cap = 0;          // bytes held by d_sim
+    float* d_ie_phi = nullptr;   // I-epsilon step: basis matrix of its projection
+    size_t ie_phi_cap = 0;       // bytes held by d_ie_phi
     float* d_split = nullptr;    // partial sums of split symmetric work items
     size_t split_cap = 0;        // floats held by d_split
 

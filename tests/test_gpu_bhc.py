@@ -316,7 +316,7 @@ def test_ie_solve_device(problem, oracle_mod, noise):
     empty bin found; on the realistic one (E = 20) it lowers the FP64 objective."""
     import torch
     from oracle import bhc_oracle as B
-    from imgrecsrc_amd.bhc import ie_solve
+    from imgrecsrc_amd.bhc import gaussLI, ie_solve, poissLI
     from imgrecsrc_amd.projector import Projector
     P = problem
     proj = Projector([GEO])
@@ -336,6 +336,10 @@ def test_ie_solve_device(problem, oracle_mod, noise):
         Ie0 = np.full(Ek, 1.0) / ((kn[2:] - kn[:-2]) / 2).sum()
         im = torch.from_numpy(Imea.astype(np.float32)).cuda()
         x, info = ie_solve(im, Ie0, proj, P["Ts"], kn, alpha=P["alpha_t"], noise=noise, maxit=100)
+        # the step's passes read the basis matrix it formed once; the gradient call
+        # evaluates the basis: the same cost bit for bit at the returned spectrum
+        fn = gaussLI if noise == "gaussian" else poissLI
+        assert fn(im, x, proj, P["Ts"], kn, alpha=P["alpha_t"])[0].item() == info["cost"]
         f_dev = obj(Imea.astype(np.float32).astype(np.float64), A, x)[0]
         f0 = obj(Imea.astype(np.float32).astype(np.float64), A, Ie0)[0]
         print(f"{noise} E={Ek}: f0 {f0:.6e} -> {f_dev:.9e} ({info})")
