@@ -238,7 +238,11 @@ int ctprj_bhc_grad_ie(ctprj_plan* plan, const float* d_alpha, const float* d_sra
  * budget of iterations.  Ie (host, E doubles) is the start point on entry and the
  * iterate on return.  Stops when the projected Newton decrement is at most
  * tol*max(|f|, 1), after maxit iterations, or when the damping exceeds 1e12.  Blocks
- * until done (one small device-to-host copy per pass). */
+ * until done (one small device-to-host copy per pass).  Like IeStep, which forms
+ * A = polyIout(Phi(alpha), []) once (bhc/beamhardenSpline.m:447), a call evaluates the
+ * basis values of its projection once (rays x E floats) and its passes read them; the
+ * plan keeps that buffer for later calls when it is at most 1 GiB (otherwise every pass
+ * evaluates the basis; the results are the same either way). */
 typedef struct ctprj_ie_info {
     double cost;       /* objective at the returned Ie */
     double decrement;  /* last Newton decrement g.d */
